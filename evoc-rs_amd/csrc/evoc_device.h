@@ -1,0 +1,124 @@
+// Device-side helpers shared by the gfx950 kernels.
+//
+// Ordering contract (the exact-mode answer): a candidate c of query q is ranked
+// by (score descending, self first, index ascending), where `score` is the
+// similarity form of the reference's raw heap distance (raw = -score):
+//   f32 : dot if dot > 0 else -FLT_MIN        (src/fast_cosine.rs:33-45)
+//   i8  : (float)dot_i32                      (src/nndescent/int8.rs:15-21)
+//   u8  : inter / union, 0 if union == 0      (src/nndescent/uint8.rs:21-36)
+// and self carries the reference's NN-descent seed value raw = -1.0, i.e.
+// score 1.0 (src/nndescent/float.rs:210-214).  A (score, index) pair is packed
+// into one 64-bit key whose unsigned order is that ranking, so selection is a
+// plain descending sort of keys.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define EVOC_WAVE 64
+
+namespace evoc {
+
+// float -> uint32 with the same order; -0.0 is folded onto +0.0 so the two
+// compare equal, as they do as IEEE floats in the reference.
+__device__ __forceinline__ uint32_t ord_f32(float s) {
+  uint32_t u = __float_as_uint(s);
+  if ((u & 0x7fffffffu) == 0u) u = 0u;
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f32(uint32_t o) {
+  uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+  return __uint_as_float(u);
+}
+
+// rank slot: self -> 0, candidate c -> c + 1; smaller slot wins ties.
+__device__ __forceinline__ uint64_t make_key(float score, int32_t cand, bool self) {
+  uint32_t slot = self ? 0u : (uint32_t)cand + 1u;
+  return ((uint64_t)ord_f32(score) << 32) | (uint64_t)(0xffffffffu - slot);
+}
+__device__ __forceinline__ int32_t key_index(uint64_t key, int32_t query) {
+  uint32_t slot = 0xffffffffu - (uint32_t)(key & 0xffffffffu);
+  return slot == 0u ? query : (int32_t)(slot - 1u);
+}
+__device__ __forceinline__ float key_score(uint64_t key) { return unord_f32((uint32_t)(key >> 32)); }
+
+// Empty slot: below every real key.
+#define EVOC_KEY_NONE 0ull
+
+// f32 similarity with the reference's FLT_MIN rule (src/fast_cosine.rs:33-45).
+__device__ __forceinline__ float f32_score(float dot) { return dot > 0.0f ? dot : -1.17549435e-38f; }
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo = (uint32_t)__shfl_xor((int)lo, m, EVOC_WAVE);
+  hi = (uint32_t)__shfl_xor((int)hi, m, EVOC_WAVE);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Bitonic sort of 64*E keys held E per lane, register-major: element
+// i = e*64 + lane lives in register e of lane `lane`.  Result is DESCENDING
+// (element 0 = largest).  Distances j < 64 exchange across lanes, j >= 64
+// swap registers within a lane.  Fully unrolled for compile-time E.
+template <int E>
+__device__ __forceinline__ void wave_sort_desc(uint64_t (&v)[E]) {
+  const int lane = threadIdx.x & (EVOC_WAVE - 1);
+  constexpr int N = EVOC_WAVE * E;
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= EVOC_WAVE) {
+        const int em = j / EVOC_WAVE;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const int p = e ^ em;
+          if (p > e) {
+            const int i = e * EVOC_WAVE + lane;  // the lower element of the pair
+            // segments with (i & k) == 0 sort descending, the others ascending
+            const bool desc = (i & k) == 0;
+            const uint64_t a = v[e], b = v[p];
+            const bool sw = desc ? (a < b) : (a > b);
+            v[e] = sw ? b : a;
+            v[p] = sw ? a : b;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const int i = e * EVOC_WAVE + lane;
+          const uint64_t o = shfl_xor_u64(v[e], j);
+          const bool lower = (i & j) == 0;
+          const bool desc = (i & k) == 0;
+          const bool keep_max = (lower == desc);
+          v[e] = keep_max ? (v[e] > o ? v[e] : o) : (v[e] < o ? v[e] : o);
+        }
+      }
+    }
+  }
+}
+
+// Element i of a register-major wave array, broadcast to every lane.
+template <int E>
+__device__ __forceinline__ uint64_t wave_elem(const uint64_t (&v)[E], int i) {
+  uint64_t mine = 0;
+#pragma unroll
+  for (int e = 0; e < E; e++)
+    if (e == i / EVOC_WAVE) mine = v[e];
+  const int src = i % EVOC_WAVE;
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)mine, src, EVOC_WAVE);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(mine >> 32), src, EVOC_WAVE);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Sum over the wave with a fixed butterfly order (deterministic).
+__device__ __forceinline__ float wave_sum_f32(float x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, EVOC_WAVE);
+  return x;
+}
+__device__ __forceinline__ int wave_sum_i32(int x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) x += __shfl_xor(x, m, EVOC_WAVE);
+  return x;
+}
+
+}  // namespace evoc

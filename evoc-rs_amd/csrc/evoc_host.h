@@ -1,0 +1,48 @@
+// Internal host-side types of the MI355X kNN-graph library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+
+struct evoc_mi355x_ctx {
+  int device = -1;
+  int n_cu = 0;
+  uint64_t budget = 0;   // HBM bytes this context may hold at once
+  uint64_t in_use = 0;   // bytes currently held
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  std::mutex mu;
+  std::string last_error;
+
+  int fail(int code, const std::string& msg);
+  int hip_fail(hipError_t e, const char* what);
+  ~evoc_mi355x_ctx();
+};
+typedef evoc_mi355x_ctx Ctx;
+
+// Device allocation accounted against the context's HBM budget.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  Ctx* owner = nullptr;
+  int alloc(Ctx* c, size_t nbytes, const char* what);
+  void release();
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+};
+
+struct ExactStats {
+  int64_t extra_rounds = 0;  // re-scoring rounds caused by buffer overflow
+  int64_t slabs = 0;
+  double pass2_ms = 0.0;     // device time of the scoring pass (all slabs)
+};
+
+// Exact K best candidates per query over all rows (see knn_exact.hip).
+// rows: n x row_bytes scored rows; pop: per-row popcounts (binary only).
+// keys: n x K output keys, descending.
+int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, const uint32_t* pop,
+                 int64_t n, int K, bool seed_self, uint64_t* keys, ExactStats* stats);

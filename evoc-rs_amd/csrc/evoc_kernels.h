@@ -1,0 +1,80 @@
+// Kernel argument blocks and host-side launchers of the gfx950 kernels.
+// Shared by the .hip translation units and the C++ host library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "evoc_mi355x.h"  // enum evoc_dtype
+
+// element kinds of the MFMA tile engine
+#define EVOC_ELEM_BF16 0
+#define EVOC_ELEM_F32 1
+#define EVOC_ELEM_I8 2
+#define EVOC_ELEM_U8BITS 3  // binary rows, VALU popcount engine
+
+namespace evoc {
+
+// One 128-query block against a range of 128-candidate tiles.
+struct TileArgs {
+  const uint8_t* q_rows;   // query rows (row_bytes apart, zero padded)
+  const uint8_t* c_rows;   // candidate rows; candidate c is row c * c_stride
+  int64_t row_bytes;       // padded row pitch in bytes, multiple of 128
+  int32_t n_q;             // queries
+  int32_t n_c;             // candidates in candidate index space
+  int32_t c_stride;        // row stride of the candidate sample (pass 1)
+  int32_t n_ctiles;        // candidate tiles of 128
+  int32_t tiles_per_split; // pass 2: tiles per blockIdx.y split
+  int32_t q_offset;        // global index of query 0 (candidates are global)
+  const int32_t* block_list;  // optional list of query blocks (re-scoring)
+  const uint32_t* row_pop;    // binary rows: popcount per row
+  // pass 1
+  float* gmax;   // [n_q][ng]
+  int32_t ng;
+  int32_t skip_self;  // 1: a query never lists itself (brute_force_knn, src/knn.rs:204-206)
+  // pass 2
+  const float* tau;          // [n_q] score bound (>= passes)
+  const uint64_t* tau_key;   // [n_q] optional key bound (re-scoring rounds)
+  uint64_t* bufs;            // [splits][n_q][caps]
+  int32_t* counts;           // [splits][n_q]
+  int32_t caps;
+  int32_t pad2;
+};
+
+struct SelectArgs {
+  const uint64_t* bufs;
+  const int32_t* counts;
+  int32_t n_q;
+  int32_t n_splits;
+  int32_t caps;
+  int32_t K;
+  const int32_t* query_list;  // optional
+  int32_t n_list;
+  int32_t q_offset;
+  uint64_t* out_keys;     // [n_q][K]
+  int32_t* overflow;      // [n_q]
+  uint64_t* tau_key_out;  // [n_q]
+};
+
+// ---- launchers (knn_exact.hip) ----
+size_t tile_lds_bytes(int kind, int pass);
+hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, hipStream_t s);
+hipError_t launch_pass2(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);
+hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int K, float* tau, hipStream_t s);
+hipError_t launch_select(const SelectArgs& a, int max_entries, hipStream_t s);
+hipError_t launch_rescore(const float* X, int64_t ld, int d, int n_q, int q_offset,
+                          const uint64_t* cand, int Kc, int k, uint64_t* out, hipStream_t s);
+hipError_t launch_finalize(const uint64_t* keys, int n_q, int q_offset, int k, int dtype,
+                           int32_t* idx, float* dist, hipStream_t s);
+
+// ---- launchers (prep.hip) ----
+// prepare_float (src/knn.rs:78-113) on the device, fused with padding and the
+// optional BF16 copy: norms/flag are scratch ([n] floats, one int).
+hipError_t launch_prepare_float(const float* X, int64_t ld, int64_t n, int d, float* out,
+                                int64_t out_ld, uint16_t* out_bf16, int64_t bf_ld, float* norms,
+                                int32_t* flag, hipStream_t s);
+hipError_t launch_pad_rows(const uint8_t* X, int64_t ld_bytes, int64_t n, int row_bytes,
+                           uint8_t* out, int64_t out_ld, hipStream_t s);
+hipError_t launch_row_popcount(const uint8_t* X, int64_t ld, int64_t n, int row_bytes,
+                               uint32_t* pop, hipStream_t s);
+
+}  // namespace evoc

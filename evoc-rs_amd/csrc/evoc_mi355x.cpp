@@ -1,0 +1,467 @@
+// Host side of the MI355X kNN-graph library: the C ABI declared in
+// include/evoc_mi355x.h.
+//
+// knn_graph_impl (src/knn.rs:377-454) is restated here over device kernels:
+// argument validation, the derived defaults (src/knn.rs:402-414), the caller's
+// RNG advance (src/knn.rs:416, :124-129), prepare_float (src/knn.rs:95-113),
+// the search itself, and the distance transform (src/knn.rs:156-187).
+#include "evoc_mi355x.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "evoc_host.h"
+#include "evoc_kernels.h"
+
+using namespace evoc;
+
+// ---------------------------------------------------------------------------
+// NumPy RandomState (MT19937 + legacy gauss): src/numpy_rng.rs:12-226.
+// ---------------------------------------------------------------------------
+struct evoc_numpy_rng {
+  uint32_t key[624];
+  int32_t pos;
+  int32_t has_gauss;
+  double gauss;
+};
+
+static void mt_seed(evoc_numpy_rng* r, uint32_t seed) {  // numpy_rng.rs:35-45
+  r->key[0] = seed;
+  for (uint32_t i = 1; i < 624; i++) {
+    seed = 1812433253u * (seed ^ (seed >> 30)) + i;
+    r->key[i] = seed;
+  }
+  r->pos = 624;
+  r->has_gauss = 0;
+  r->gauss = 0.0;
+}
+
+static void mt_refill(evoc_numpy_rng* r) {  // numpy_rng.rs:47-59
+  static const uint32_t A = 0x9908b0dfu, UP = 0x80000000u, LO = 0x7fffffffu;
+  uint32_t* k = r->key;
+  for (int i = 0; i < 624; i++) {
+    const uint32_t y = (k[i] & UP) | (k[(i + 1) % 624] & LO);
+    k[i] = k[(i + 397) % 624] ^ (y >> 1) ^ ((0u - (y & 1u)) & A);
+  }
+  r->pos = 0;
+}
+
+static uint32_t mt_next(evoc_numpy_rng* r) {  // numpy_rng.rs:61-72
+  if (r->pos >= 624) mt_refill(r);
+  uint32_t y = r->key[r->pos++];
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= y >> 18;
+  return y;
+}
+
+static double mt_double(evoc_numpy_rng* r) {  // numpy_rng.rs:74-78
+  const int32_t a = (int32_t)(mt_next(r) >> 5), b = (int32_t)(mt_next(r) >> 6);
+  return ((double)a * 67108864.0 + (double)b) / 9007199254740992.0;
+}
+
+extern "C" {
+
+evoc_numpy_rng* evoc_numpy_rng_new(uint32_t seed) {
+  evoc_numpy_rng* r = new (std::nothrow) evoc_numpy_rng;
+  if (r) mt_seed(r, seed);
+  return r;
+}
+
+void evoc_numpy_rng_free(evoc_numpy_rng* r) { delete r; }
+
+int64_t evoc_numpy_rng_randint(evoc_numpy_rng* r, int64_t low, int64_t high) {  // :81-112
+  const uint64_t range = (uint64_t)(high - low);
+  if (range == 0) return low;
+  const uint64_t max = range - 1;
+  if (max == 0) return low;
+  uint64_t mask = max;
+  mask |= mask >> 1;
+  mask |= mask >> 2;
+  mask |= mask >> 4;
+  mask |= mask >> 8;
+  mask |= mask >> 16;
+  mask |= mask >> 32;
+  for (;;) {
+    const uint64_t v = (uint64_t)mt_next(r) & mask;
+    if (v <= max) return low + (int64_t)v;
+  }
+}
+
+double evoc_numpy_rng_random(evoc_numpy_rng* r) { return mt_double(r); }
+
+double evoc_numpy_rng_normal(evoc_numpy_rng* r) {  // numpy_rng.rs:115-133
+  if (r->has_gauss) {
+    r->has_gauss = 0;
+    const double t = r->gauss;
+    r->gauss = 0.0;
+    return t;
+  }
+  for (;;) {
+    const double x1 = 2.0 * mt_double(r) - 1.0, x2 = 2.0 * mt_double(r) - 1.0;
+    const double r2 = x1 * x1 + x2 * x2;
+    if (r2 < 1.0 && r2 != 0.0) {
+      const double f = std::sqrt(-2.0 * std::log(r2) / r2);
+      r->gauss = f * x1;
+      r->has_gauss = 1;
+      return f * x2;
+    }
+  }
+}
+
+void evoc_numpy_rng_get_state(const evoc_numpy_rng* r, uint32_t key[624], int32_t* pos,
+                              int32_t* has_gauss, double* gauss) {
+  std::memcpy(key, r->key, sizeof(r->key));
+  if (pos) *pos = r->pos;
+  if (has_gauss) *has_gauss = r->has_gauss;
+  if (gauss) *gauss = r->gauss;
+}
+
+void evoc_numpy_rng_set_state(evoc_numpy_rng* r, const uint32_t key[624], int32_t pos,
+                              int32_t has_gauss, double gauss) {
+  std::memcpy(r->key, key, sizeof(r->key));
+  r->pos = pos;
+  r->has_gauss = has_gauss;
+  r->gauss = gauss;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Context, errors, device memory under an HBM budget
+// ---------------------------------------------------------------------------
+evoc_mi355x_ctx::~evoc_mi355x_ctx() {
+  if (device >= 0) {
+    (void)hipSetDevice(device);
+    if (ev_begin) (void)hipEventDestroy(ev_begin);
+    if (ev_end) (void)hipEventDestroy(ev_end);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+}
+
+int Ctx::fail(int code, const std::string& msg) {
+  last_error = msg;
+  return code;
+}
+
+int Ctx::hip_fail(hipError_t e, const char* what) {
+  last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return EVOC_ERR_HIP;
+}
+
+DevBuf::~DevBuf() { release(); }
+
+void DevBuf::release() {
+  if (ptr) {
+    (void)hipFree(ptr);
+    if (owner) owner->in_use -= bytes;
+    ptr = nullptr;
+    bytes = 0;
+  }
+}
+
+int DevBuf::alloc(Ctx* c, size_t nbytes, const char* what) {
+  release();
+  if (nbytes == 0) nbytes = 256;
+  nbytes = (nbytes + 255) & ~(size_t)255;
+  if (c->in_use + nbytes > c->budget) {
+    char buf[256];
+    snprintf(buf, sizeof(buf), "HBM budget exceeded allocating %s: %zu bytes requested, %zu of %zu in use",
+             what, nbytes, (size_t)c->in_use, (size_t)c->budget);
+    return c->fail(EVOC_ERR_OUT_OF_BUDGET, buf);
+  }
+  hipError_t e = hipMalloc(&ptr, nbytes);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    (void)hipGetLastError();
+    return c->hip_fail(e, what);
+  }
+  owner = c;
+  bytes = nbytes;
+  c->in_use += nbytes;
+  return EVOC_OK;
+}
+
+extern "C" {
+
+const char* evoc_mi355x_version(void) { return "evoc_mi355x 0.1.0 (gfx950, abi 1)"; }
+
+int evoc_mi355x_device_count(int32_t* count) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *count = 0;
+    return EVOC_ERR_NO_DEVICE;
+  }
+  *count = c;
+  return EVOC_OK;
+}
+
+int evoc_mi355x_create(int32_t device, uint64_t hbm_budget_bytes, evoc_mi355x_ctx** out) {
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    (void)hipGetLastError();
+    return EVOC_ERR_NO_DEVICE;
+  }
+  if (device < 0 || device >= count) return EVOC_ERR_INVALID_ARGUMENT;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return EVOC_ERR_HIP;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return EVOC_ERR_NO_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return EVOC_ERR_HIP;
+  Ctx* c = new (std::nothrow) Ctx;
+  if (!c) return EVOC_ERR_OUT_OF_BUDGET;
+  c->device = device;
+  c->n_cu = prop.multiProcessorCount;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = total_b = 0;
+  c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)((double)free_b * 0.9);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+    delete c;
+    return EVOC_ERR_HIP;
+  }
+  *out = c;
+  return EVOC_OK;
+}
+
+void evoc_mi355x_destroy(evoc_mi355x_ctx* ctx) { delete ctx; }
+
+const char* evoc_mi355x_last_error(const evoc_mi355x_ctx* ctx) {
+  return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+void evoc_knn_options_default(evoc_knn_options* o) {  // src/knn.rs:60-76
+  std::memset(o, 0, sizeof(*o));
+  o->n_neighbors = 30;
+  o->max_rptree_depth = 200;
+  o->delta = 0.001f;
+  o->delta_improv = 0.001f;
+  o->use_sorted_updates = 1;
+  o->brute_force_max_n = 0;
+  o->deterministic = 0;
+  o->mode = EVOC_MODE_EXACT;
+  o->precision = EVOC_PREC_BF16_RESCORE;
+  o->margin = 16;
+  o->strict_precision = 0;
+  o->rayon_threads = 0;
+}
+
+int64_t evoc_knn_effective_k(int64_t n_neighbors, int64_t n) {  // src/knn.rs:402
+  int64_t k = std::min(n_neighbors, n > 0 ? n - 1 : 0);
+  return k < 1 ? 1 : k;
+}
+
+static int64_t default_n_trees(const evoc_knn_options* o) {  // src/knn.rs:404-407
+  if (o->n_trees > 0) return o->n_trees;
+  int64_t t = o->deterministic ? 1 : (o->rayon_threads > 0 ? o->rayon_threads : 1);
+  return std::min<int64_t>(std::max<int64_t>(t, 4), 8);
+}
+
+static bool brute_force_branch(int32_t dtype, int64_t n, const evoc_knn_options* o) {
+  return n <= o->brute_force_max_n && dtype != EVOC_DTYPE_F32;  // src/knn.rs:418-419
+}
+
+int64_t evoc_knn_rng_draws(int32_t dtype, int64_t n, const evoc_knn_options* o) {
+  if (brute_force_branch(dtype, n, o)) return 3;
+  return 3 + 3 * default_n_trees(o);
+}
+
+int evoc_mi355x_knn_graph(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const evoc_knn_options* o,
+                          evoc_numpy_rng* rng, evoc_knn_output* out) {
+  if (!ctx || !in || !o || !rng || !out) return EVOC_ERR_INVALID_ARGUMENT;
+  // Validation happens before any draw, as in the reference (src/knn.rs:382-393).
+  if (o->n_neighbors <= 0) return ctx->fail(EVOC_ERR_INVALID_NEIGHBORS, "n_neighbors must be positive");
+  if (in->n <= 0) return ctx->fail(EVOC_ERR_EMPTY_DATA, "empty embedding matrix");
+  const int64_t LO = (int64_t)INT32_MIN + 1, HI = (int64_t)INT32_MAX - 1;  // src/knn.rs:11-12
+  int64_t tau[3];
+  for (int i = 0; i < 3; i++) tau[i] = evoc_numpy_rng_randint(rng, LO, HI);
+  std::vector<int64_t> trees;
+  if (!brute_force_branch(in->dtype, in->n, o)) {
+    const int64_t nt = default_n_trees(o);
+    trees.resize((size_t)(3 * nt));
+    for (auto& v : trees) v = evoc_numpy_rng_randint(rng, LO, HI);
+  }
+  return evoc_mi355x_knn_graph_seeded(ctx, in, o, tau, trees.empty() ? nullptr : trees.data(),
+                                      (int64_t)(trees.size() / 3), out);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// knn_graph on the device
+// ---------------------------------------------------------------------------
+static size_t elem_size(int dtype) { return dtype == EVOC_DTYPE_F32 ? 4 : 1; }
+
+static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
+                                            const evoc_knn_options* o, const int64_t tau_state[3],
+                                            const int64_t* tree_states, int64_t n_tree_rows,
+                                            evoc_knn_output* out) {
+  if (!ctx || !in || !o || !out) return EVOC_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  const auto t_start = std::chrono::steady_clock::now();
+  ctx->last_error.clear();
+  if (o->n_neighbors <= 0) return ctx->fail(EVOC_ERR_INVALID_NEIGHBORS, "n_neighbors must be positive");
+  if (in->n <= 0) return ctx->fail(EVOC_ERR_EMPTY_DATA, "empty embedding matrix");
+  if (in->dtype < EVOC_DTYPE_F32 || in->dtype > EVOC_DTYPE_U8)
+    return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "unknown dtype");
+  if (in->d <= 0 || in->ld < in->d || !in->data)
+    return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "bad matrix shape (need d > 0, ld >= d, data != NULL)");
+  if (in->n > (int64_t)INT32_MAX - 1)
+    return ctx->fail(EVOC_ERR_UNSUPPORTED, "more than 2^31-2 rows");
+  if (o->strict_precision)
+    return ctx->fail(EVOC_ERR_NOT_BITWISE,
+                     "strict_precision=true asks for results bitwise identical to the CPU strict "
+                     "backend; the mi355x backend reproduces its neighbour sets with a different "
+                     "f32 summation order (distances agree within 1e-6). Set strict_precision=false.");
+  const int64_t n = in->n, d = in->d;
+  const int64_t k = evoc_knn_effective_k(o->n_neighbors, n);
+  if (out->capacity_k < k) return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "output capacity_k < effective k");
+  if (!out->indices || !out->distances) return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "null output arrays");
+  (void)tau_state;
+  (void)tree_states;
+  (void)n_tree_rows;
+  out->k_out = k;
+  out->score_seconds = 0.0;
+  out->iterations = 0;
+  out->rescore_rounds = 0;
+  if (o->mode != EVOC_MODE_EXACT && o->mode != EVOC_MODE_NNDESCENT)
+    return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "unknown mode");
+  if (o->mode == EVOC_MODE_NNDESCENT)
+    return ctx->fail(EVOC_ERR_UNSUPPORTED, "NN-descent mode is not built into this library version");
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(EVOC_ERR_HIP, "hipSetDevice failed");
+
+  hipStream_t st = ctx->stream;
+  const int dtype = in->dtype;
+  const bool bf16 = dtype == EVOC_DTYPE_F32 && o->precision == EVOC_PREC_BF16_RESCORE;
+  // brute_force_knn (src/knn.rs:189-281) skips self; the NN-descent layout seeds it.
+  const bool seed_self = !brute_force_branch(dtype, n, o);
+
+  // ---- stage the input on the device ----
+  const int64_t in_row_bytes = d * (int64_t)elem_size(dtype);
+  DevBuf src;
+  const uint8_t* d_src;
+  if (in->location == EVOC_MEM_DEVICE) {
+    d_src = (const uint8_t*)in->data;
+  } else {
+    // the caller's matrix spans (n - 1) * ld + d elements
+    const size_t span = (size_t)((n - 1) * in->ld + d) * elem_size(dtype);
+    int rc = src.alloc(ctx, span, "input copy");
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(src.ptr, in->data, span, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "input upload");
+    d_src = (const uint8_t*)src.ptr;
+  }
+  (void)hipEventRecord(ctx->ev_begin, st);
+
+  // scored rows: padded to a multiple of 128 bytes of the scored element type
+  int kind;
+  int64_t elem_bytes;
+  if (dtype == EVOC_DTYPE_F32) {
+    kind = bf16 ? EVOC_ELEM_BF16 : EVOC_ELEM_F32;
+    elem_bytes = bf16 ? 2 : 4;
+  } else if (dtype == EVOC_DTYPE_I8) {
+    kind = EVOC_ELEM_I8;
+    elem_bytes = 1;
+  } else {
+    kind = EVOC_ELEM_U8BITS;
+    elem_bytes = 1;
+  }
+  const int64_t row_bytes = round_up(d * elem_bytes, 128);
+  DevBuf rows, xf32, norms, flag, pop;
+  int rc = rows.alloc(ctx, (size_t)(n * row_bytes), "scored rows");
+  if (rc) return rc;
+  hipError_t e = hipSuccess;
+  int64_t f32_ld = 0;
+  if (dtype == EVOC_DTYPE_F32) {
+    // prepared f32 rows (kept for the FP32 re-score) + the scored copy
+    f32_ld = round_up(d, 32);
+    if (bf16) {
+      if ((rc = xf32.alloc(ctx, (size_t)(n * f32_ld * 4), "prepared f32 rows"))) return rc;
+    }
+    if ((rc = norms.alloc(ctx, (size_t)n * 4, "row norms"))) return rc;
+    if ((rc = flag.alloc(ctx, 4, "renorm flag"))) return rc;
+    float* fout = bf16 ? (float*)xf32.ptr : (float*)rows.ptr;
+    e = launch_prepare_float((const float*)d_src, in->ld, n, (int)d, fout, f32_ld,
+                             bf16 ? (uint16_t*)rows.ptr : nullptr, bf16 ? row_bytes / 2 : 0,
+                             (float*)norms.ptr, (int32_t*)flag.ptr, st);
+  } else {
+    e = launch_pad_rows(d_src, in->ld, n, (int)in_row_bytes, (uint8_t*)rows.ptr, row_bytes, st);
+    if (e == hipSuccess && dtype == EVOC_DTYPE_U8) {
+      if ((rc = pop.alloc(ctx, (size_t)n * 4, "row popcounts"))) return rc;
+      e = launch_row_popcount((const uint8_t*)rows.ptr, row_bytes, n, (int)row_bytes,
+                              (uint32_t*)pop.ptr, st);
+    }
+  }
+  if (e != hipSuccess) return ctx->hip_fail(e, "input preparation");
+
+  // ---- exact search ----
+  const int64_t K = bf16 ? std::min<int64_t>(k + std::max(0, o->margin), n) : k;
+  if (bf16 && K > 256)
+    return ctx->fail(EVOC_ERR_UNSUPPORTED, "BF16 re-score keeps at most 256 candidates: need k + margin <= 256");
+  if (K > 1024) return ctx->fail(EVOC_ERR_UNSUPPORTED, "exact search supports k <= 1024");
+  DevBuf keys;
+  if ((rc = keys.alloc(ctx, (size_t)(n * K * 8), "candidate keys"))) return rc;
+  ExactStats xs;
+  rc = exact_search(ctx, kind, (const uint8_t*)rows.ptr, row_bytes, (const uint32_t*)pop.ptr, n, (int)K,
+                    seed_self, (uint64_t*)keys.ptr, &xs);
+  if (rc) return rc;
+  out->rescore_rounds = xs.extra_rounds;
+  out->score_seconds = xs.pass2_ms * 1e-3;
+
+  DevBuf fkeys;
+  const uint64_t* final_keys = (const uint64_t*)keys.ptr;
+  if (bf16) {
+    if ((rc = fkeys.alloc(ctx, (size_t)(n * k * 8), "rescored keys"))) return rc;
+    e = launch_rescore((const float*)xf32.ptr, f32_ld, (int)d, (int)n, 0, (const uint64_t*)keys.ptr, (int)K,
+                       (int)k, (uint64_t*)fkeys.ptr, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "rescore");
+    final_keys = (const uint64_t*)fkeys.ptr;
+  }
+
+  // ---- transform + write out ----
+  DevBuf oidx, odist;
+  int32_t* d_idx;
+  float* d_dist;
+  const bool dev_out = out->location == EVOC_MEM_DEVICE && out->capacity_k == k;
+  if (dev_out) {
+    d_idx = out->indices;
+    d_dist = out->distances;
+  } else {
+    if ((rc = oidx.alloc(ctx, (size_t)(n * k * 4), "output indices"))) return rc;
+    if ((rc = odist.alloc(ctx, (size_t)(n * k * 4), "output distances"))) return rc;
+    d_idx = (int32_t*)oidx.ptr;
+    d_dist = (float*)odist.ptr;
+  }
+  e = launch_finalize(final_keys, (int)n, 0, (int)k, dtype, d_idx, d_dist, st);
+  if (e != hipSuccess) return ctx->hip_fail(e, "finalize");
+  (void)hipEventRecord(ctx->ev_end, st);
+  if (!dev_out) {
+    const hipMemcpyKind kd = out->location == EVOC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    e = hipMemcpy2DAsync(out->indices, (size_t)out->capacity_k * 4, d_idx, (size_t)k * 4, (size_t)k * 4,
+                         (size_t)n, kd, st);
+    if (e == hipSuccess)
+      e = hipMemcpy2DAsync(out->distances, (size_t)out->capacity_k * 4, d_dist, (size_t)k * 4,
+                           (size_t)k * 4, (size_t)n, kd, st);
+    if (e != hipSuccess) return ctx->hip_fail(e, "output copy");
+  }
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return ctx->hip_fail(e, "knn_graph");
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end);
+  out->device_seconds = ms * 1e-3;
+  out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  return EVOC_OK;
+}

@@ -1,0 +1,226 @@
+// Exact search orchestration: plans the three passes of knn_exact.hip for a
+// matrix, processes queries in slabs that fit the HBM budget, and re-scores
+// query blocks whose candidate buffers overflowed with a tighter bound until
+// every query's K best are exact.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "evoc_host.h"
+#include "evoc_kernels.h"
+#include "evoc_mi355x.h"
+
+using namespace evoc;
+
+namespace {
+
+constexpr int TILE = 128;
+constexpr int MAX_SORT = 2048;  // largest per-query candidate list one wave sorts
+constexpr int MAX_ROUNDS = 8;
+
+int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+int64_t pow2ceil(int64_t v) {
+  int64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+struct Plan {
+  int n_splits = 1, tiles_per_split = 1, caps = 0;
+  bool pass1 = false;
+  int stride = 1, n_sample = 0, ng = 0;
+  int max_entries = 0;
+  size_t scratch_bytes(int64_t nq, int K) const {
+    size_t b = (size_t)nq * (4 + 4 + 8) + (size_t)n_splits * nq * ((size_t)caps * 8 + 4);
+    if (pass1) b += (size_t)nq * ng * 4;
+    (void)K;
+    return b;
+  }
+};
+
+Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
+  Plan p;
+  const int64_t n_ctiles = cdiv(n, TILE);
+  const int64_t nqb = cdiv(nq, TILE);
+  // enough workgroups to fill the chip several times over
+  int64_t ns = std::max<int64_t>(1, cdiv(4LL * n_cu, nqb));
+  ns = std::min<int64_t>({ns, 16, n_ctiles});
+  p.tiles_per_split = (int)cdiv(n_ctiles, ns);
+  p.n_splits = (int)cdiv(n_ctiles, p.tiles_per_split);
+  if (n <= MAX_SORT) {
+    // every candidate fits one sorted list: no threshold pass
+    p.pass1 = false;
+    p.caps = p.tiles_per_split * TILE;
+    p.max_entries = (int)std::min<int64_t>(n, (int64_t)p.n_splits * p.caps);
+    return p;
+  }
+  p.pass1 = true;
+  const int groups_per_tile = kind == EVOC_ELEM_U8BITS ? 16 : 8;
+  int64_t stride = std::min<int64_t>(std::max<int64_t>(cdiv(n, 16384), 1), 16);
+  for (;;) {
+    const int64_t nsamp = cdiv(n, stride);
+    const int64_t ng = cdiv(nsamp, TILE) * groups_per_tile;
+    if (ng <= MAX_SORT || stride >= n) {
+      p.stride = (int)stride;
+      p.n_sample = (int)nsamp;
+      p.ng = (int)ng;
+      break;
+    }
+    stride++;
+  }
+  // a query's bound ranks near its K-th .. 2K-th sample group, i.e. about
+  // stride * 2K candidates of the full set pass it; keep 4x headroom
+  const int64_t expect = (int64_t)p.stride * 2 * K;
+  int64_t caps = std::max<int64_t>(pow2ceil(cdiv(4 * expect, p.n_splits)), 128);
+  // the kept subset must hold 2K keys so an overflow round tightens the bound
+  caps = std::max<int64_t>(caps, pow2ceil(cdiv(2LL * K, p.n_splits)));
+  p.caps = (int)std::min<int64_t>(caps, MAX_SORT);
+  p.max_entries = (int)std::min<int64_t>((int64_t)p.n_splits * p.caps, MAX_SORT);
+  return p;
+}
+
+}  // namespace
+
+int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, const uint32_t* pop,
+                 int64_t n, int K, bool seed_self, uint64_t* keys, ExactStats* stats) {
+  hipStream_t s = ctx->stream;
+  if (K < 1 || K > MAX_SORT / 2 || K > n)
+    return ctx->fail(EVOC_ERR_UNSUPPORTED, "exact search needs 1 <= K <= min(n, 1024)");
+  // slab of queries whose scratch fits a quarter of the remaining budget
+  const uint64_t avail = ctx->budget > ctx->in_use ? ctx->budget - ctx->in_use : 0;
+  int64_t slab = n;
+  for (;;) {
+    Plan p = make_plan(kind, n, slab, K, ctx->n_cu);
+    if (p.scratch_bytes(slab, K) <= avail / 4 || slab <= 4 * TILE) break;
+    slab = std::max<int64_t>(4 * TILE, cdiv(slab / 2, TILE) * TILE);
+  }
+  const int64_t n_ctiles = cdiv(n, TILE);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  int rc = EVOC_OK;
+  for (int64_t q0 = 0; q0 < n && rc == EVOC_OK; q0 += slab) {
+    const int64_t nq = std::min(slab, n - q0);
+    const int64_t nqb = cdiv(nq, TILE);
+    const Plan p = make_plan(kind, n, nq, K, ctx->n_cu);
+    DevBuf gmax, tau, bufs, counts, ovf, tkey, list;
+    if (p.pass1) {
+      if ((rc = gmax.alloc(ctx, (size_t)nq * p.ng * 4, "group maxima"))) break;
+      if ((rc = tau.alloc(ctx, (size_t)nq * 4, "score bounds"))) break;
+    }
+    if ((rc = bufs.alloc(ctx, (size_t)p.n_splits * nq * p.caps * 8, "candidate buffers"))) break;
+    if ((rc = counts.alloc(ctx, (size_t)p.n_splits * nq * 4, "candidate counts"))) break;
+    if ((rc = ovf.alloc(ctx, (size_t)nq * 4, "overflow flags"))) break;
+    if ((rc = tkey.alloc(ctx, (size_t)nq * 8, "key bounds"))) break;
+    if ((rc = list.alloc(ctx, (size_t)(nqb + nq) * 4, "re-score lists"))) break;
+
+    TileArgs a = {};
+    a.q_rows = rows + q0 * row_bytes;
+    a.c_rows = rows;
+    a.row_bytes = row_bytes;
+    a.n_q = (int32_t)nq;
+    a.q_offset = (int32_t)q0;
+    a.row_pop = pop;
+    a.skip_self = seed_self ? 0 : 1;
+    hipError_t e = hipSuccess;
+    if (p.pass1) {
+      TileArgs a1 = a;
+      a1.n_c = p.n_sample;
+      a1.c_stride = p.stride;
+      a1.n_ctiles = (int32_t)cdiv(p.n_sample, TILE);
+      a1.gmax = (float*)gmax.ptr;
+      a1.ng = p.ng;
+      e = launch_pass1(kind, a1, (int)nqb, s);
+      if (e == hipSuccess) e = launch_tau_select((const float*)gmax.ptr, (int)nq, p.ng, K, (float*)tau.ptr, s);
+      if (e != hipSuccess) {
+        rc = ctx->hip_fail(e, "threshold pass");
+        break;
+      }
+    }
+    TileArgs a2 = a;
+    a2.n_c = (int32_t)n;
+    a2.c_stride = 1;
+    a2.n_ctiles = (int32_t)n_ctiles;
+    a2.tiles_per_split = p.tiles_per_split;
+    a2.tau = p.pass1 ? (const float*)tau.ptr : nullptr;
+    a2.bufs = (uint64_t*)bufs.ptr;
+    a2.counts = (int32_t*)counts.ptr;
+    a2.caps = p.caps;
+    SelectArgs sa = {};
+    sa.bufs = (const uint64_t*)bufs.ptr;
+    sa.counts = (const int32_t*)counts.ptr;
+    sa.n_q = (int32_t)nq;
+    sa.n_splits = p.n_splits;
+    sa.caps = p.caps;
+    sa.K = K;
+    sa.q_offset = (int32_t)q0;
+    sa.out_keys = keys + q0 * K;
+    sa.overflow = (int32_t*)ovf.ptr;
+    sa.tau_key_out = (uint64_t*)tkey.ptr;
+
+    (void)hipEventRecord(e0, s);
+    e = launch_pass2(kind, a2, (int)nqb, p.n_splits, s);
+    (void)hipEventRecord(e1, s);
+    if (e == hipSuccess) e = launch_select(sa, p.max_entries, s);
+    if (e != hipSuccess) {
+      rc = ctx->hip_fail(e, "scoring pass");
+      break;
+    }
+    std::vector<int32_t> flags((size_t)nq);
+    for (int round = 0;; round++) {
+      e = hipMemcpyAsync(flags.data(), ovf.ptr, (size_t)nq * 4, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) {
+        rc = ctx->hip_fail(e, "scoring pass");
+        break;
+      }
+      if (round == 0) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (stats) stats->pass2_ms += ms;
+      }
+      std::vector<int32_t> blocks;
+      for (int64_t b = 0; b < nqb; b++) {
+        bool any = false;
+        for (int64_t q = b * TILE; q < std::min<int64_t>(nq, (b + 1) * TILE); q++) any |= flags[(size_t)q] != 0;
+        if (any) blocks.push_back((int32_t)b);
+      }
+      if (blocks.empty()) break;
+      if (round >= MAX_ROUNDS) {
+        rc = ctx->fail(EVOC_ERR_UNSUPPORTED,
+                       "exact search: candidate lists still overflow after 8 rounds (massive exact ties)");
+        break;
+      }
+      if (stats) stats->extra_rounds++;
+      std::vector<int32_t> qlist;
+      for (int32_t b : blocks)
+        for (int64_t q = (int64_t)b * TILE; q < std::min<int64_t>(nq, ((int64_t)b + 1) * TILE); q++)
+          qlist.push_back((int32_t)q);
+      int32_t* d_blocks = (int32_t*)list.ptr;
+      int32_t* d_q = d_blocks + nqb;
+      e = hipMemcpyAsync(d_blocks, blocks.data(), blocks.size() * 4, hipMemcpyHostToDevice, s);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(d_q, qlist.data(), qlist.size() * 4, hipMemcpyHostToDevice, s);
+      TileArgs ar = a2;
+      ar.block_list = d_blocks;
+      ar.tau = nullptr;
+      ar.tau_key = (const uint64_t*)tkey.ptr;
+      if (e == hipSuccess) e = launch_pass2(kind, ar, (int)blocks.size(), p.n_splits, s);
+      SelectArgs sr = sa;
+      sr.query_list = d_q;
+      sr.n_list = (int32_t)qlist.size();
+      if (e == hipSuccess) e = launch_select(sr, p.max_entries, s);
+      if (e != hipSuccess) {
+        rc = ctx->hip_fail(e, "re-scoring pass");
+        break;
+      }
+      // queries outside the re-scored blocks keep flags == 0 from the last full pass
+    }
+    if (stats) stats->slabs++;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}

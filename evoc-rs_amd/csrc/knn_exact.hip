@@ -1,0 +1,659 @@
+// Tiled exact all-pairs scoring with fused candidate filtering (gfx950).
+//
+// This is the GPU form of the reference's exact search (brute_force_knn,
+// src/knn.rs:189-281) and of the answer its NN-descent converges to
+// (src/nndescent/float.rs:775-879): for every query, the K best candidates
+// under the ranking in evoc_device.h.
+//
+// Three passes, all over row tiles of 128 queries x 128 candidates:
+//   pass 1 (threshold): queries x a strided sample of candidates, MFMA scores
+//          max-pooled over groups of 16 candidates per lane.  The K-th largest
+//          group maximum of a query is a lower bound tau_q on its true K-th
+//          best score (K groups hold at least K candidates >= tau_q).
+//   pass 2 (score):     queries x all candidates; a score >= tau_q is appended
+//          as a 64-bit key to the query's bounded buffer (LDS counter, rare).
+//   pass 3 (select):    per query, one wave sorts its buffered keys and keeps
+//          the K best.  A buffer that overflowed yields a tighter bound from
+//          its kept entries and its query block is scored again.
+// The f32 path can score in BF16 (MFMA 32x32x16, FP32 accumulate) and then
+// re-score the K = k + margin survivors in FP32 (rescore_kernel); the int8 path
+// is exact i8 MFMA into i32; binary Jaccard uses VALU popcounts.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "evoc_device.h"
+#include "evoc_kernels.h"
+
+namespace evoc {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+typedef __attribute__((ext_vector_type(4))) int i32x4;
+typedef __attribute__((ext_vector_type(16))) int i32x16;
+
+// ---------------------------------------------------------------------------
+// Tile geometry.  256 threads = 4 waves in a 2 (candidates) x 2 (queries)
+// arrangement; each wave owns a 64 x 64 sub-tile = 2 x 2 MFMA 32x32 blocks.
+// MFMA orientation: M (rows) = candidates (A operand), N (cols) = queries
+// (B operand), so lane l of a 32x32 block holds query column l % 32 and the 16
+// candidate rows 8*(j/4) + 4*(l/32) + j%4, j = 0..15.
+// ---------------------------------------------------------------------------
+constexpr int TILE = 128;            // rows per tile (both operands)
+constexpr int STAGE_BYTES = 128;     // bytes of each row staged per K step
+constexpr int ROW_PITCH = 144;       // padded LDS row pitch (bytes)
+constexpr int OPERAND_LDS = TILE * ROW_PITCH;
+
+template <int KIND> struct Mfma;
+
+template <> struct Mfma<EVOC_ELEM_BF16> {
+  static constexpr int KSTEP_BYTES = 32;  // 16 bf16 per MFMA K step
+  static constexpr int LANE_BYTES = 16;   // 8 bf16 per lane
+  typedef f32x16 acc_t;
+  __device__ static acc_t zero() { return acc_t{}; }
+  __device__ static void step(acc_t& c, const uint8_t* a, const uint8_t* b) {
+    bf16x8 av = __builtin_bit_cast(bf16x8, *(const i32x4*)a);
+    bf16x8 bv = __builtin_bit_cast(bf16x8, *(const i32x4*)b);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
+  }
+  __device__ static float value(const acc_t& c, int j) { return f32_score(c[j]); }
+};
+
+template <> struct Mfma<EVOC_ELEM_F32> {
+  static constexpr int KSTEP_BYTES = 8;  // 2 f32 per MFMA K step
+  static constexpr int LANE_BYTES = 4;
+  typedef f32x16 acc_t;
+  __device__ static acc_t zero() { return acc_t{}; }
+  __device__ static void step(acc_t& c, const uint8_t* a, const uint8_t* b) {
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(*(const float*)a, *(const float*)b, c, 0, 0, 0);
+  }
+  __device__ static float value(const acc_t& c, int j) { return f32_score(c[j]); }
+};
+
+template <> struct Mfma<EVOC_ELEM_I8> {
+  static constexpr int KSTEP_BYTES = 32;  // 32 i8 per MFMA K step
+  static constexpr int LANE_BYTES = 16;
+  typedef i32x16 acc_t;
+  __device__ static acc_t zero() { return acc_t{}; }
+  __device__ static void step(acc_t& c, const uint8_t* a, const uint8_t* b) {
+    c = __builtin_amdgcn_mfma_i32_32x32x32_i8(*(const i32x4*)a, *(const i32x4*)b, c, 0, 0, 0);
+  }
+  // int8.rs:15-21: -(dot as f32); the score is dot as f32 (exact below 2^24)
+  __device__ static float value(const acc_t& c, int j) { return (float)c[j]; }
+};
+
+// Cooperative 16-byte loads of one 128-row x 128-byte stage of an operand into
+// registers (4 chunks per thread), then into LDS.
+struct StageRegs {
+  i32x4 v[4];
+};
+
+__device__ __forceinline__ void load_stage(StageRegs& r, const uint8_t* __restrict__ base,
+                                           int64_t row_bytes, int row0, int nrows, int rstride,
+                                           int kbyte) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int id = t + 256 * i;
+    int row = id >> 3, c16 = id & 7;
+    int g = row0 + row;
+    if (g < nrows) {
+      const uint8_t* p = base + (int64_t)g * rstride * row_bytes + kbyte + c16 * 16;
+      r.v[i] = *(const i32x4*)p;
+    } else {
+      r.v[i] = i32x4{0, 0, 0, 0};
+    }
+  }
+}
+
+__device__ __forceinline__ void store_stage(const StageRegs& r, uint8_t* lds) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int id = t + 256 * i;
+    int row = id >> 3, c16 = id & 7;
+    *(i32x4*)(lds + row * ROW_PITCH + c16 * 16) = r.v[i];
+  }
+}
+
+// Accumulate the 128 x 128 score tile (candidates c0.., queries q0..) over all
+// K stages.  acc[bm][bn] are this wave's four 32x32 blocks.
+template <int KIND>
+__device__ __forceinline__ void score_tile(typename Mfma<KIND>::acc_t (&acc)[2][2], uint8_t* lds,
+                                           const TileArgs& a, int c0, int q0) {
+  using M = Mfma<KIND>;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = M::zero();
+
+  const int nst = (int)(a.row_bytes / STAGE_BYTES);
+  StageRegs ra, rb;
+  load_stage(ra, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, 0);
+  load_stage(rb, a.q_rows, a.row_bytes, q0, a.n_q, 1, 0);
+  store_stage(ra, lds);
+  store_stage(rb, lds + OPERAND_LDS);
+  __syncthreads();
+  for (int s = 0; s < nst; s++) {
+    uint8_t* cur = lds + (s & 1) * 2 * OPERAND_LDS;
+    if (s + 1 < nst) {
+      load_stage(ra, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, (s + 1) * STAGE_BYTES);
+      load_stage(rb, a.q_rows, a.row_bytes, q0, a.n_q, 1, (s + 1) * STAGE_BYTES);
+    }
+    const uint8_t* A = cur + (wm * 64 + (lane & 31)) * ROW_PITCH + (lane >> 5) * M::LANE_BYTES;
+    const uint8_t* B =
+        cur + OPERAND_LDS + (wn * 64 + (lane & 31)) * ROW_PITCH + (lane >> 5) * M::LANE_BYTES;
+#pragma unroll
+    for (int ks = 0; ks < STAGE_BYTES / M::KSTEP_BYTES; ks++) {
+      const int kb = ks * M::KSTEP_BYTES;
+      M::step(acc[0][0], A + kb, B + kb);
+      M::step(acc[0][1], A + kb, B + 32 * ROW_PITCH + kb);
+      M::step(acc[1][0], A + 32 * ROW_PITCH + kb, B + kb);
+      M::step(acc[1][1], A + 32 * ROW_PITCH + kb, B + 32 * ROW_PITCH + kb);
+    }
+    if (s + 1 < nst) {
+      uint8_t* nxt = lds + ((s + 1) & 1) * 2 * OPERAND_LDS;
+      store_stage(ra, nxt);
+      store_stage(rb, nxt + OPERAND_LDS);
+    }
+    __syncthreads();
+  }
+}
+
+// Query block handled by this workgroup.
+__device__ __forceinline__ int query_block(const TileArgs& a) {
+  return a.block_list ? a.block_list[blockIdx.x] : (int)blockIdx.x;
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1: group maxima over a strided candidate sample.
+// ---------------------------------------------------------------------------
+template <int KIND>
+__global__ __launch_bounds__(256) void pass1_groupmax_kernel(TileArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  using M = Mfma<KIND>;
+  typename M::acc_t acc[2][2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int qb = query_block(a);
+  const int q0 = qb * TILE;
+  for (int t = 0; t < a.n_ctiles; t++) {
+    const int c0 = t * TILE;
+    score_tile<KIND>(acc, lds, a, c0, q0);
+#pragma unroll
+    for (int bm = 0; bm < 2; bm++) {
+#pragma unroll
+      for (int bn = 0; bn < 2; bn++) {
+        const int q = q0 + wn * 64 + bn * 32 + (lane & 31);
+        float m = -INFINITY;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const int c = c0 + wm * 64 + bm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+          const bool self = (int64_t)c * a.c_stride == (int64_t)q + a.q_offset;
+          if (c < a.n_c && !(self && a.skip_self)) {
+            const float s = self ? 1.0f : M::value(acc[bm][bn], j);
+            m = fmaxf(m, s);
+          }
+        }
+        const int g = ((t * 2 + wm) * 2 + bm) * 2 + (lane >> 5);
+        if (q < a.n_q) a.gmax[(int64_t)q * a.ng + g] = m;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 2: threshold filter + append.  blockIdx.y = candidate split.
+// ---------------------------------------------------------------------------
+template <int KIND>
+__global__ __launch_bounds__(256) void pass2_append_kernel(TileArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  using M = Mfma<KIND>;
+  int* cnt = (int*)(lds + 4 * OPERAND_LDS);
+  typename M::acc_t acc[2][2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int qb = query_block(a);
+  const int q0 = qb * TILE;
+  const int split = blockIdx.y;
+  if (threadIdx.x < TILE) cnt[threadIdx.x] = 0;
+  float tau[2];
+  uint64_t tkey[2];
+#pragma unroll
+  for (int bn = 0; bn < 2; bn++) {
+    const int q = q0 + wn * 64 + bn * 32 + (lane & 31);
+    tau[bn] = (q < a.n_q && a.tau) ? a.tau[q] : -INFINITY;
+    tkey[bn] = (q < a.n_q && a.tau_key) ? a.tau_key[q] : 0ull;
+  }
+  const int t_begin = split * a.tiles_per_split;
+  const int t_end = min(a.n_ctiles, t_begin + a.tiles_per_split);
+  for (int t = t_begin; t < t_end; t++) {
+    const int c0 = t * TILE;
+    score_tile<KIND>(acc, lds, a, c0, q0);
+#pragma unroll
+    for (int bm = 0; bm < 2; bm++) {
+#pragma unroll
+      for (int bn = 0; bn < 2; bn++) {
+        const int ql = wn * 64 + bn * 32 + (lane & 31);
+        const int q = q0 + ql;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const int c = c0 + wm * 64 + bm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+          const bool self = (c == q + a.q_offset);
+          const float s = self ? 1.0f : M::value(acc[bm][bn], j);
+          bool take = (c < a.n_c) && (q < a.n_q) && (s >= tau[bn]) && !(self && a.skip_self);
+          if (take && a.tau_key) take = make_key(s, c, self) >= tkey[bn];
+          if (take) {
+            int pos = atomicAdd(&cnt[ql], 1);
+            if (pos < a.caps)
+              a.bufs[((int64_t)split * a.n_q + q) * a.caps + pos] = make_key(s, c, self);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < TILE) {
+    const int q = q0 + threadIdx.x;
+    if (q < a.n_q) a.counts[(int64_t)split * a.n_q + q] = cnt[threadIdx.x];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Binary Jaccard tiles (uint8.rs:21-36) on the VALU: 16 x 16 threads, each
+// owning 8 candidates x 8 queries; popcount of AND per 32-bit word, union from
+// the per-row popcounts.  Same two epilogues as the MFMA kernels.
+// ---------------------------------------------------------------------------
+constexpr int JAC_PITCH = STAGE_BYTES + 4;  // LDS row pitch in bytes (word-padded)
+
+__device__ __forceinline__ void jac_tile(uint32_t (&inter)[8][8], uint8_t* lds, const TileArgs& a,
+                                         int c0, int q0) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) inter[i][j] = 0;
+  const int nst = (int)(a.row_bytes / STAGE_BYTES);
+  uint8_t* A = lds;
+  uint8_t* B = lds + TILE * JAC_PITCH;
+  for (int s = 0; s < nst; s++) {
+    // 128 rows x 128 B per operand: 8 x 16 B per row, 4 chunks per thread
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      int id = threadIdx.x + 256 * i;
+      int row = id >> 3, c16 = id & 7;
+      int cg = c0 + row, qg = q0 + row;
+      i32x4 va = i32x4{0, 0, 0, 0}, vb = i32x4{0, 0, 0, 0};
+      if (cg < a.n_c)
+        va = *(const i32x4*)(a.c_rows + (int64_t)cg * a.c_stride * a.row_bytes + s * STAGE_BYTES + c16 * 16);
+      if (qg < a.n_q)
+        vb = *(const i32x4*)(a.q_rows + (int64_t)qg * a.row_bytes + s * STAGE_BYTES + c16 * 16);
+      uint32_t* pa = (uint32_t*)(A + row * JAC_PITCH + c16 * 16);
+      uint32_t* pb = (uint32_t*)(B + row * JAC_PITCH + c16 * 16);
+      pa[0] = va[0]; pa[1] = va[1]; pa[2] = va[2]; pa[3] = va[3];
+      pb[0] = vb[0]; pb[1] = vb[1]; pb[2] = vb[2]; pb[3] = vb[3];
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int w = 0; w < STAGE_BYTES / 4; w++) {
+      uint32_t av[8], bv[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) av[i] = *(const uint32_t*)(A + (ty + 16 * i) * JAC_PITCH + 4 * w);
+#pragma unroll
+      for (int j = 0; j < 8; j++) bv[j] = *(const uint32_t*)(B + (tx + 16 * j) * JAC_PITCH + 4 * w);
+#pragma unroll
+      for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) inter[i][j] += __popc(av[i] & bv[j]);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ float jac_score(uint32_t inter, uint32_t pa, uint32_t pb) {
+  uint32_t uni = pa + pb - inter;
+  return uni > 0 ? __fdiv_rn((float)inter, (float)uni) : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void jac_pass1_kernel(TileArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  uint32_t inter[8][8];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int qb = query_block(a);
+  const int q0 = qb * TILE;
+  for (int t = 0; t < a.n_ctiles; t++) {
+    const int c0 = t * TILE;
+    jac_tile(inter, lds, a, c0, q0);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int q = q0 + tx + 16 * j;
+      if (q >= a.n_q) continue;
+      const uint32_t pq = a.row_pop[q + a.q_offset];
+      float m = -INFINITY;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int c = c0 + ty + 16 * i;
+        const int64_t cg = (int64_t)c * a.c_stride;
+        const bool self = cg == (int64_t)q + a.q_offset;
+        if (c < a.n_c && !(self && a.skip_self)) {
+          const float s = self ? 1.0f : jac_score(inter[i][j], a.row_pop[cg], pq);
+          m = fmaxf(m, s);
+        }
+      }
+      // 16 groups (one per ty) of 8 candidates per tile
+      a.gmax[(int64_t)q * a.ng + t * 16 + ty] = m;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void jac_pass2_kernel(TileArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  int* cnt = (int*)(lds + 2 * TILE * JAC_PITCH);
+  uint32_t inter[8][8];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int qb = query_block(a);
+  const int q0 = qb * TILE;
+  const int split = blockIdx.y;
+  if (threadIdx.x < TILE) cnt[threadIdx.x] = 0;
+  float tau[8];
+  uint64_t tkey[8];
+  uint32_t pq[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int q = q0 + tx + 16 * j;
+    tau[j] = (q < a.n_q && a.tau) ? a.tau[q] : -INFINITY;
+    tkey[j] = (q < a.n_q && a.tau_key) ? a.tau_key[q] : 0ull;
+    pq[j] = q < a.n_q ? a.row_pop[q + a.q_offset] : 0u;
+  }
+  const int t_begin = split * a.tiles_per_split;
+  const int t_end = min(a.n_ctiles, t_begin + a.tiles_per_split);
+  for (int t = t_begin; t < t_end; t++) {
+    const int c0 = t * TILE;
+    jac_tile(inter, lds, a, c0, q0);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int c = c0 + ty + 16 * i;
+      if (c >= a.n_c) continue;
+      const uint32_t pc = a.row_pop[c];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int ql = tx + 16 * j;
+        const int q = q0 + ql;
+        if (q >= a.n_q) continue;
+        const bool self = (c == q + a.q_offset);
+        const float s = self ? 1.0f : jac_score(inter[i][j], pc, pq[j]);
+        bool take = s >= tau[j] && !(self && a.skip_self);
+        if (take && a.tau_key) take = make_key(s, c, self) >= tkey[j];
+        if (take) {
+          int pos = atomicAdd(&cnt[ql], 1);
+          if (pos < a.caps) a.bufs[((int64_t)split * a.n_q + q) * a.caps + pos] = make_key(s, c, self);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < TILE) {
+    const int q = q0 + threadIdx.x;
+    if (q < a.n_q) a.counts[(int64_t)split * a.n_q + q] = cnt[threadIdx.x];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Threshold from group maxima: tau_q = K-th largest of gmax[q, 0..ng).
+// One wave per query, ng <= 64 * E.
+// ---------------------------------------------------------------------------
+template <int E>
+__global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict__ gmax, int n_q,
+                                                         int ng, int K, float* __restrict__ tau) {
+  const int lane = threadIdx.x & 63;
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= n_q) return;
+  uint64_t v[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int g = e * 64 + lane;
+    const float m = g < ng ? gmax[(int64_t)q * ng + g] : -INFINITY;
+    v[e] = (uint64_t)ord_f32(m);
+  }
+  wave_sort_desc<E>(v);
+  const uint64_t kth = wave_elem<E>(v, K - 1);
+  if (lane == 0) tau[q] = (K <= ng) ? unord_f32((uint32_t)kth) : -INFINITY;
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3: per query, gather the buffered keys of all splits, sort, keep K.
+// Overflow (a split dropped keys, or more than 64*E keys in total) marks the
+// query and stores the K-th kept key as a tighter bound for the next round:
+// the K-th best of any subset is <= the true K-th best.
+// ---------------------------------------------------------------------------
+template <int E>
+__global__ __launch_bounds__(256) void select_topk_kernel(SelectArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
+  int q;
+  if (a.query_list) {
+    if (qi >= a.n_list) return;
+    q = a.query_list[qi];
+  } else {
+    q = qi;
+    if (q >= a.n_q) return;
+  }
+  constexpr int CAP = 64 * E;
+  uint64_t v[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) v[e] = EVOC_KEY_NONE;
+  int filled = 0;
+  bool overflow = false;
+  for (int s = 0; s < a.n_splits; s++) {
+    int c = a.counts[(int64_t)s * a.n_q + q];
+    if (c > a.caps) {
+      overflow = true;
+      c = a.caps;
+    }
+    const uint64_t* src = a.bufs + ((int64_t)s * a.n_q + q) * a.caps;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int f = e * 64 + lane;
+      if (f >= filled && f < filled + c) v[e] = src[f - filled];
+    }
+    filled += c;
+  }
+  if (filled > CAP) overflow = true;
+  wave_sort_desc<E>(v);
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int i = e * 64 + lane;
+    if (i < a.K) a.out_keys[(int64_t)q * a.K + i] = v[e];
+  }
+  if (lane == 0) a.overflow[q] = overflow ? 1 : 0;
+  // the K-th kept key bounds the true K-th key from below (kept keys are a
+  // subset of all candidates); it drives the next round for this query block
+  const uint64_t kth = wave_elem<E>(v, a.K - 1);
+  if (lane == 0) a.tau_key_out[q] = kth;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 re-score of the BF16 survivors with a fixed reduction order: lane l
+// accumulates elements l, l+64, ... with fmaf, then a butterfly sum.  The k
+// best by exact FP32 score are kept (Kc <= 64*E candidates per query).
+// ---------------------------------------------------------------------------
+template <int E>
+__global__ __launch_bounds__(256) void rescore_kernel(const float* __restrict__ X, int64_t ld, int d,
+                                                      int n_q, int q_offset,
+                                                      const uint64_t* __restrict__ cand_keys, int Kc,
+                                                      int k, uint64_t* __restrict__ out_keys) {
+  const int lane = threadIdx.x & 63;
+  const int ql = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (ql >= n_q) return;
+  const int q = ql + q_offset;
+  const float* xq = X + (int64_t)q * ld;
+  uint64_t v[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) v[e] = EVOC_KEY_NONE;
+  for (int i = 0; i < Kc; i++) {
+    const uint64_t key = cand_keys[(int64_t)ql * Kc + i];
+    if (key == EVOC_KEY_NONE) continue;
+    const int c = key_index(key, q);
+    const bool self = (c == q);
+    float part = 0.0f;
+    const float* xc = X + (int64_t)c * ld;
+    for (int t = lane; t < d; t += 64) part = fmaf(xq[t], xc[t], part);
+    const float dot = wave_sum_f32(part);
+    const uint64_t nk = make_key(self ? 1.0f : f32_score(dot), c, self);
+#pragma unroll
+    for (int e = 0; e < E; e++)
+      if (e * 64 + lane == i) v[e] = nk;
+  }
+  wave_sort_desc<E>(v);
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int i = e * 64 + lane;
+    if (i < k) out_keys[(int64_t)ql * k + i] = v[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Keys -> (indices, transformed distances) (src/knn.rs:156-187).
+// ---------------------------------------------------------------------------
+__global__ void finalize_kernel(const uint64_t* __restrict__ keys, int n_q, int q_offset, int k,
+                                int dtype, int32_t* __restrict__ out_idx,
+                                float* __restrict__ out_dist) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n_q * k) return;
+  const int q = (int)(i / k) + q_offset;
+  const uint64_t key = keys[i];
+  if (key == EVOC_KEY_NONE) {
+    out_idx[i] = -1;
+    out_dist[i] = 0.0f;  // an empty slot holds raw INF (f32::MAX), which maps to 0
+    return;
+  }
+  const int c = key_index(key, q);
+  const float score = key_score(key);
+  const float raw = -score;  // f32 self: -1.0; FLT_MIN rule: -(-FLT_MIN)
+  out_idx[i] = c;
+  float d;
+  if (dtype == EVOC_DTYPE_F32) {
+    if (raw >= 0.0f) {
+      d = 0.0f;
+    } else {
+      const double sim = (double)(-raw);
+      if (sim >= 1.0) {
+        d = 0.0f;
+      } else {
+        const double t = -log2(sim);
+        d = (float)(t > 0.0 ? t : 0.0);
+      }
+    }
+  } else if (dtype == EVOC_DTYPE_I8) {
+    d = raw >= 0.0f ? 0.0f : __fdiv_rn(1.0f, -raw);
+  } else {
+    d = raw >= 0.0f ? 0.0f : (float)(-log2((double)(-raw)));
+  }
+  out_dist[i] = d;
+}
+
+}  // namespace evoc
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+namespace evoc {
+
+size_t tile_lds_bytes(int kind, int pass) {
+  const size_t counters = pass == 2 ? TILE * sizeof(int) : 0;
+  if (kind == EVOC_ELEM_U8BITS) return 2 * TILE * JAC_PITCH + counters;
+  return 4 * OPERAND_LDS + counters;
+}
+
+template <class Kern>
+static hipError_t launch_tile(Kern kern, size_t lds, dim3 grid, hipStream_t s, const TileArgs& a) {
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, hipStream_t s) {
+  const size_t lds = tile_lds_bytes(kind, 1);
+  const dim3 g((unsigned)n_blocks);
+  switch (kind) {
+    case EVOC_ELEM_BF16: return launch_tile(pass1_groupmax_kernel<EVOC_ELEM_BF16>, lds, g, s, a);
+    case EVOC_ELEM_F32: return launch_tile(pass1_groupmax_kernel<EVOC_ELEM_F32>, lds, g, s, a);
+    case EVOC_ELEM_I8: return launch_tile(pass1_groupmax_kernel<EVOC_ELEM_I8>, lds, g, s, a);
+    case EVOC_ELEM_U8BITS: return launch_tile(jac_pass1_kernel, lds, g, s, a);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_pass2(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s) {
+  const size_t lds = tile_lds_bytes(kind, 2);
+  const dim3 g((unsigned)n_blocks, (unsigned)n_splits);
+  switch (kind) {
+    case EVOC_ELEM_BF16: return launch_tile(pass2_append_kernel<EVOC_ELEM_BF16>, lds, g, s, a);
+    case EVOC_ELEM_F32: return launch_tile(pass2_append_kernel<EVOC_ELEM_F32>, lds, g, s, a);
+    case EVOC_ELEM_I8: return launch_tile(pass2_append_kernel<EVOC_ELEM_I8>, lds, g, s, a);
+    case EVOC_ELEM_U8BITS: return launch_tile(jac_pass2_kernel, lds, g, s, a);
+  }
+  return hipErrorInvalidValue;
+}
+
+static int pow2_per_lane(int entries) {
+  int e = 1;
+  while (e * 64 < entries) e <<= 1;
+  return e;
+}
+
+hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int K, float* tau, hipStream_t s) {
+  const dim3 g((unsigned)((n_q + 3) / 4)), b(256);
+  switch (pow2_per_lane(ng)) {
+    case 1: hipLaunchKernelGGL(tau_select_kernel<1>, g, b, 0, s, gmax, n_q, ng, K, tau); break;
+    case 2: hipLaunchKernelGGL(tau_select_kernel<2>, g, b, 0, s, gmax, n_q, ng, K, tau); break;
+    case 4: hipLaunchKernelGGL(tau_select_kernel<4>, g, b, 0, s, gmax, n_q, ng, K, tau); break;
+    case 8: hipLaunchKernelGGL(tau_select_kernel<8>, g, b, 0, s, gmax, n_q, ng, K, tau); break;
+    case 16: hipLaunchKernelGGL(tau_select_kernel<16>, g, b, 0, s, gmax, n_q, ng, K, tau); break;
+    case 32: hipLaunchKernelGGL(tau_select_kernel<32>, g, b, 0, s, gmax, n_q, ng, K, tau); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_select(const SelectArgs& a, int max_entries, hipStream_t s) {
+  const int nq = a.query_list ? a.n_list : a.n_q;
+  const dim3 g((unsigned)((nq + 3) / 4)), b(256);
+  int E = pow2_per_lane(max_entries < a.K ? a.K : max_entries);
+  if (E > 32) E = 32;
+  switch (E) {
+    case 1: hipLaunchKernelGGL(select_topk_kernel<1>, g, b, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(select_topk_kernel<2>, g, b, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(select_topk_kernel<4>, g, b, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(select_topk_kernel<8>, g, b, 0, s, a); break;
+    case 16: hipLaunchKernelGGL(select_topk_kernel<16>, g, b, 0, s, a); break;
+    case 32: hipLaunchKernelGGL(select_topk_kernel<32>, g, b, 0, s, a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_rescore(const float* X, int64_t ld, int d, int n_q, int q_offset,
+                          const uint64_t* cand, int Kc, int k, uint64_t* out, hipStream_t s) {
+  const dim3 g((unsigned)((n_q + 3) / 4)), b(256);
+  switch (pow2_per_lane(Kc)) {
+    case 1: hipLaunchKernelGGL(rescore_kernel<1>, g, b, 0, s, X, ld, d, n_q, q_offset, cand, Kc, k, out); break;
+    case 2: hipLaunchKernelGGL(rescore_kernel<2>, g, b, 0, s, X, ld, d, n_q, q_offset, cand, Kc, k, out); break;
+    case 4: hipLaunchKernelGGL(rescore_kernel<4>, g, b, 0, s, X, ld, d, n_q, q_offset, cand, Kc, k, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const uint64_t* keys, int n_q, int q_offset, int k, int dtype,
+                           int32_t* idx, float* dist, hipStream_t s) {
+  const int64_t tot = (int64_t)n_q * k;
+  hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, keys, n_q,
+                     q_offset, k, dtype, idx, dist);
+  return hipGetLastError();
+}
+
+}  // namespace evoc

@@ -1,0 +1,119 @@
+// Input preparation on the device.
+//
+// prepare_float (src/knn.rs:78-113): the reference sums v*v serially in f32 per
+// row, tests |sqrt(sum) - 1| > 1e-5 on every row, and if any row fails divides
+// every row by its norm (a zero norm becomes 1).  The row norm is computed here
+// by one thread per row in the same serial order with non-contracted IEEE
+// operations, so the renormalisation decision and the divided values are the
+// reference's bit for bit.  The element-wise pass is coalesced and also writes
+// the zero-padded BF16 copy used by the MFMA scorer.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "evoc_kernels.h"
+
+namespace evoc {
+
+__global__ void row_norm_serial_kernel(const float* __restrict__ X, int64_t ld, int64_t n, int d,
+                                       float* __restrict__ norms, int32_t* __restrict__ flag) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const float* x = X + r * ld;
+  float s = 0.0f;
+  for (int t = 0; t < d; t++) s = __fadd_rn(s, __fmul_rn(x[t], x[t]));
+  const float nr = __fsqrt_rn(s);
+  if (fabsf(__fsub_rn(nr, 1.0f)) > 1e-5f) atomicOr(flag, 1);
+  norms[r] = nr == 0.0f ? 1.0f : nr;
+}
+
+__global__ void zero_i32_kernel(int32_t* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__device__ __forceinline__ uint16_t bf16_rne(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)0x7fc0u;  // NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// grid: x over padded columns (256 per block), y over rows
+__global__ void normalize_rows_kernel(const float* __restrict__ X, int64_t ld, int64_t n, int d,
+                                      const float* __restrict__ norms,
+                                      const int32_t* __restrict__ flag, float* __restrict__ out,
+                                      int64_t out_ld, uint16_t* __restrict__ out_bf16,
+                                      int64_t bf_ld) {
+  const int64_t r = blockIdx.y;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const bool renorm = *flag != 0;
+  float v = c < d ? X[r * ld + c] : 0.0f;
+  if (renorm && c < d) v = __fdiv_rn(v, norms[r]);
+  if (out && c < out_ld) out[r * out_ld + c] = v;
+  if (out_bf16 && c < bf_ld) out_bf16[r * bf_ld + c] = bf16_rne(v);
+}
+
+__global__ void pad_rows_kernel(const uint8_t* __restrict__ X, int64_t ld, int64_t n, int row_bytes,
+                                uint8_t* __restrict__ out, int64_t out_ld) {
+  const int64_t r = blockIdx.y;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n || c >= out_ld) return;
+  out[r * out_ld + c] = c < row_bytes ? X[r * ld + c] : (uint8_t)0;
+}
+
+__global__ void row_popcount_kernel(const uint8_t* __restrict__ X, int64_t ld, int64_t n,
+                                    int row_bytes, uint32_t* __restrict__ pop) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  uint32_t s = 0;
+  for (int t = 0; t < row_bytes; t++) s += __popc((uint32_t)X[r * ld + t]);
+  pop[r] = s;
+}
+
+static inline dim3 grid_rows(int64_t n, int64_t cols, int bs) {
+  return dim3((unsigned)((cols + bs - 1) / bs), (unsigned)n);
+}
+
+hipError_t launch_prepare_float(const float* X, int64_t ld, int64_t n, int d, float* out,
+                                int64_t out_ld, uint16_t* out_bf16, int64_t bf_ld, float* norms,
+                                int32_t* flag, hipStream_t s) {
+  hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, flag, (int64_t)1);
+  hipLaunchKernelGGL(row_norm_serial_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X,
+                     ld, n, d, norms, flag);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t cols = out_ld > bf_ld ? out_ld : bf_ld;
+  if (n > 65535) {
+    // y-dimension limit: launch in row slabs
+    for (int64_t r0 = 0; r0 < n; r0 += 65535) {
+      const int64_t nr = (n - r0) < 65535 ? (n - r0) : 65535;
+      hipLaunchKernelGGL(normalize_rows_kernel, grid_rows(nr, cols, 256), dim3(256), 0, s,
+                         X + r0 * ld, ld, nr, d, norms + r0, flag, out ? out + r0 * out_ld : nullptr,
+                         out_ld, out_bf16 ? out_bf16 + r0 * bf_ld : nullptr, bf_ld);
+    }
+  } else {
+    hipLaunchKernelGGL(normalize_rows_kernel, grid_rows(n, cols, 256), dim3(256), 0, s, X, ld, n, d,
+                       norms, flag, out, out_ld, out_bf16, bf_ld);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pad_rows(const uint8_t* X, int64_t ld, int64_t n, int row_bytes, uint8_t* out,
+                           int64_t out_ld, hipStream_t s) {
+  for (int64_t r0 = 0; r0 < n; r0 += 65535) {
+    const int64_t nr = (n - r0) < 65535 ? (n - r0) : 65535;
+    hipLaunchKernelGGL(pad_rows_kernel, grid_rows(nr, out_ld, 256), dim3(256), 0, s, X + r0 * ld, ld,
+                       nr, row_bytes, out + r0 * out_ld, out_ld);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_row_popcount(const uint8_t* X, int64_t ld, int64_t n, int row_bytes, uint32_t* pop,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(row_popcount_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X, ld,
+                     n, row_bytes, pop);
+  return hipGetLastError();
+}
+
+}  // namespace evoc

@@ -1,0 +1,413 @@
+"""Python mirror of the reference's kNN-graph API over the MI355X C ABI.
+
+    knn_graph(data, options, rng) -> (indices, distances)      src/knn.rs:345-359
+    KnnGraphOptions                                            src/knn.rs:43-76
+    NumpyRandomState / check_random_state                      src/numpy_rng.rs:12-226
+    KnnError(EmptyData | InvalidNeighbors | Backend(...))      src/knn.rs:33-41
+
+Everything runs in ``libevoc_mi355x.so`` (gfx950 HIP kernels behind the C ABI
+of include/evoc_mi355x.h).  There is no CPU fallback: if the library or the GPU
+is missing, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libevoc_mi355x.so")
+
+DTYPE_F32, DTYPE_I8, DTYPE_U8 = 0, 1, 2
+MEM_HOST, MEM_DEVICE = 0, 1
+MODE_EXACT, MODE_NNDESCENT = 0, 1
+PREC_FP32, PREC_BF16_RESCORE = 0, 1
+
+OK = 0
+ERR_EMPTY_DATA = 1
+ERR_INVALID_NEIGHBORS = 2
+ERR_INVALID_ARGUMENT = 3
+ERR_UNSUPPORTED = 4
+ERR_OUT_OF_BUDGET = 5
+ERR_HIP = 6
+ERR_CANCELLED = 7
+ERR_NOT_BITWISE = 8
+ERR_NO_DEVICE = 9
+
+_DTYPES = {np.dtype(np.float32): DTYPE_F32, np.dtype(np.int8): DTYPE_I8, np.dtype(np.uint8): DTYPE_U8}
+
+INT32_MIN_P1 = -(2**31) + 1  # src/knn.rs:11
+INT32_MAX_M1 = 2**31 - 2     # src/knn.rs:12
+
+
+class KnnError(RuntimeError):
+    """src/knn.rs:33-41 plus the backend's own failures (status code in .code)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class EmptyData(KnnError):
+    pass
+
+
+class InvalidNeighbors(KnnError):
+    pass
+
+
+class BackendError(KnnError):
+    pass
+
+
+class _Options(C.Structure):
+    _fields_ = [
+        ("n_neighbors", C.c_int64), ("n_trees", C.c_int64), ("leaf_size", C.c_int64),
+        ("max_candidates", C.c_int64), ("max_rptree_depth", C.c_int64), ("n_iters", C.c_int64),
+        ("delta", C.c_float), ("delta_improv", C.c_float), ("use_sorted_updates", C.c_int32),
+        ("deterministic", C.c_int32), ("brute_force_max_n", C.c_int64), ("mode", C.c_int32),
+        ("precision", C.c_int32), ("margin", C.c_int32), ("strict_precision", C.c_int32),
+        ("rayon_threads", C.c_int32), ("reserved0", C.c_int32),
+    ]
+
+
+class _Input(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("location", C.c_int32), ("data", C.c_void_p),
+                ("n", C.c_int64), ("d", C.c_int64), ("ld", C.c_int64)]
+
+
+class _Output(C.Structure):
+    _fields_ = [("location", C.c_int32), ("reserved0", C.c_int32), ("indices", C.c_void_p),
+                ("distances", C.c_void_p), ("capacity_k", C.c_int64), ("k_out", C.c_int64),
+                ("seconds", C.c_double), ("device_seconds", C.c_double), ("score_seconds", C.c_double),
+                ("iterations", C.c_int64), ("rescore_rounds", C.c_int64)]
+
+
+_lib = None
+
+# Every function include/evoc_mi355x.h declares, with its ctypes signature.
+_P = C.c_void_p
+SIGNATURES = {
+    "evoc_mi355x_version": ([], C.c_char_p),
+    "evoc_mi355x_device_count": ([C.POINTER(C.c_int32)], C.c_int),
+    "evoc_mi355x_create": ([C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
+    "evoc_mi355x_destroy": ([_P], None),
+    "evoc_mi355x_last_error": ([_P], C.c_char_p),
+    "evoc_knn_options_default": ([C.POINTER(_Options)], None),
+    "evoc_knn_effective_k": ([C.c_int64, C.c_int64], C.c_int64),
+    "evoc_knn_rng_draws": ([C.c_int32, C.c_int64, C.POINTER(_Options)], C.c_int64),
+    "evoc_mi355x_knn_graph": ([_P, C.POINTER(_Input), C.POINTER(_Options), _P, C.POINTER(_Output)],
+                              C.c_int),
+    "evoc_mi355x_knn_graph_seeded": ([_P, C.POINTER(_Input), C.POINTER(_Options), _P, _P, C.c_int64,
+                                      C.POINTER(_Output)], C.c_int),
+    "evoc_numpy_rng_new": ([C.c_uint32], _P),
+    "evoc_numpy_rng_free": ([_P], None),
+    "evoc_numpy_rng_randint": ([_P, C.c_int64, C.c_int64], C.c_int64),
+    "evoc_numpy_rng_normal": ([_P], C.c_double),
+    "evoc_numpy_rng_random": ([_P], C.c_double),
+    "evoc_numpy_rng_get_state": ([_P, _P, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_double)], None),
+    "evoc_numpy_rng_set_state": ([_P, _P, C.c_int32, C.c_int32, C.c_double], None),
+}
+
+
+def lib():
+    """Load the in-tree HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it first (python __graft_entry__.py build "
+                              "or make -C evoc-rs_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def version() -> str:
+    return lib().evoc_mi355x_version().decode()
+
+
+def device_count() -> int:
+    c = C.c_int32(0)
+    lib().evoc_mi355x_device_count(C.byref(c))
+    return int(c.value)
+
+
+class NumpyRandomState:
+    """``numpy.random.RandomState`` as restated by src/numpy_rng.rs (MT19937)."""
+
+    def __init__(self, seed: int = 0):
+        self._h = lib().evoc_numpy_rng_new(C.c_uint32(seed & 0xFFFFFFFF))
+        if not self._h:
+            raise MemoryError("evoc_numpy_rng_new")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.evoc_numpy_rng_free(self._h)
+            self._h = None
+
+    def randint(self, low: int, high: int) -> int:
+        return int(lib().evoc_numpy_rng_randint(self._h, low, high))
+
+    def normal(self) -> float:
+        return float(lib().evoc_numpy_rng_normal(self._h))
+
+    def random(self) -> float:
+        return float(lib().evoc_numpy_rng_random(self._h))
+
+    def randint3_for_tau(self) -> np.ndarray:
+        return np.array([self.randint(INT32_MIN_P1, INT32_MAX_M1) for _ in range(3)], np.int64)
+
+    def get_state(self):
+        key = np.zeros(624, np.uint32)
+        pos, hg, g = C.c_int32(), C.c_int32(), C.c_double()
+        lib().evoc_numpy_rng_get_state(self._h, key.ctypes.data, C.byref(pos), C.byref(hg), C.byref(g))
+        return key, int(pos.value), int(hg.value), float(g.value)
+
+    def set_state(self, key: np.ndarray, pos: int, has_gauss: int = 0, gauss: float = 0.0):
+        k = np.ascontiguousarray(key, np.uint32)
+        lib().evoc_numpy_rng_set_state(self._h, k.ctypes.data, pos, has_gauss, gauss)
+
+
+def check_random_state(seed: Optional[int]) -> NumpyRandomState:
+    """src/numpy_rng.rs:216-226."""
+    if seed is None:
+        import time
+
+        seed = time.time_ns() & 0xFFFFFFFF
+    return NumpyRandomState(seed)
+
+
+@dataclass
+class KnnGraphOptions:
+    """src/knn.rs:43-76 (None = derived default, src/knn.rs:402-414) + mi355x extensions."""
+
+    n_neighbors: int = 30
+    n_trees: Optional[int] = None
+    leaf_size: Optional[int] = None
+    max_candidates: Optional[int] = None
+    max_rptree_depth: int = 200
+    n_iters: Optional[int] = None
+    delta: float = 0.001
+    delta_improv: float = 0.001
+    use_sorted_updates: bool = True
+    brute_force_max_n: int = 0
+    deterministic: bool = False
+    # mi355x extensions
+    mode: int = MODE_EXACT
+    precision: int = PREC_BF16_RESCORE
+    margin: int = 16
+    strict_precision: bool = False
+    rayon_threads: int = 0
+
+    def _c(self) -> _Options:
+        o = _Options()
+        lib().evoc_knn_options_default(C.byref(o))
+        o.n_neighbors = self.n_neighbors
+        o.n_trees = self.n_trees or 0
+        o.leaf_size = self.leaf_size or 0
+        o.max_candidates = self.max_candidates or 0
+        o.max_rptree_depth = self.max_rptree_depth
+        o.n_iters = self.n_iters or 0
+        o.delta = self.delta
+        o.delta_improv = self.delta_improv
+        o.use_sorted_updates = int(self.use_sorted_updates)
+        o.brute_force_max_n = self.brute_force_max_n
+        o.deterministic = int(self.deterministic)
+        o.mode = self.mode
+        o.precision = self.precision
+        o.margin = self.margin
+        o.strict_precision = int(self.strict_precision)
+        o.rayon_threads = self.rayon_threads
+        return o
+
+
+def effective_k(n_neighbors: int, n: int) -> int:
+    return int(lib().evoc_knn_effective_k(n_neighbors, n))
+
+
+def rng_draws(dtype_code: int, n: int, options: KnnGraphOptions) -> int:
+    o = options._c()
+    return int(lib().evoc_knn_rng_draws(dtype_code, n, C.byref(o)))
+
+
+def _raise(code: int, ctx) -> None:
+    msg = lib().evoc_mi355x_last_error(ctx).decode() if ctx else ""
+    if code == ERR_EMPTY_DATA:
+        raise EmptyData(code, msg or "empty embedding matrix")
+    if code == ERR_INVALID_NEIGHBORS:
+        raise InvalidNeighbors(code, msg or "n_neighbors must be positive")
+    raise BackendError(code, msg or "mi355x backend error")
+
+
+class Context:
+    """One GPU, one stream, one HBM budget (evoc_mi355x_ctx)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        h = C.c_void_p()
+        rc = lib().evoc_mi355x_create(device, hbm_budget_bytes, C.byref(h))
+        if rc != OK:
+            raise BackendError(rc, f"evoc_mi355x_create(device={device}) failed with status {rc} "
+                                   "(no gfx950 device visible?)")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().evoc_mi355x_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def last_error(self) -> str:
+        return lib().evoc_mi355x_last_error(self._h).decode()
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+@dataclass
+class CallStats:
+    seconds: float = 0.0
+    device_seconds: float = 0.0
+    score_seconds: float = 0.0
+    iterations: int = 0
+    rescore_rounds: int = 0
+    extra: dict = field(default_factory=dict)
+
+
+def _input_from(data, dtype_code=None):
+    """numpy array (host) or torch tensor (device) -> (_Input, keepalive)."""
+    try:
+        import torch  # noqa: F401
+
+        is_torch = type(data).__module__.startswith("torch")
+    except ImportError:
+        is_torch = False
+    if is_torch:
+        t = data
+        tmap = {"torch.float32": DTYPE_F32, "torch.int8": DTYPE_I8, "torch.uint8": DTYPE_U8}
+        code = tmap.get(str(t.dtype))
+        if code is None or t.dim() != 2 or t.stride(1) != 1:
+            raise BackendError(ERR_INVALID_ARGUMENT, "expected a 2-D row-major float32/int8/uint8 tensor")
+        loc = MEM_DEVICE if t.is_cuda else MEM_HOST
+        inp = _Input(code, loc, C.c_void_p(t.data_ptr()), t.shape[0], t.shape[1], t.stride(0))
+        return inp, t
+    a = np.asarray(data)
+    if a.ndim != 2:
+        raise BackendError(ERR_INVALID_ARGUMENT, "expected a 2-D array")
+    if a.dtype not in _DTYPES:
+        raise BackendError(ERR_INVALID_ARGUMENT, f"unsupported dtype {a.dtype} (float32, int8, uint8)")
+    a = np.ascontiguousarray(a)
+    inp = _Input(_DTYPES[a.dtype], MEM_HOST, C.c_void_p(a.ctypes.data if a.size else 1), a.shape[0],
+                 a.shape[1], a.shape[1])
+    return inp, a
+
+
+def knn_graph(data, options: KnnGraphOptions, rng: NumpyRandomState,
+              ctx: Optional[Context] = None, stats: Optional[CallStats] = None,
+              out=None):
+    """src/knn.rs:345-359 on the MI355X.  Advances `rng` exactly like the reference.
+
+    `data`: (n, d) float32 / int8 / uint8 numpy array, or a torch tensor on the
+    GPU (the graph is then built from HBM-resident rows).  Returns numpy
+    (indices int32 [n, k], distances float32 [n, k]) unless `out` = (idx, dist)
+    device tensors are given, which are filled in place.
+    """
+    ctx = ctx or default_context()
+    inp, keep = _input_from(data)
+    o = options._c()
+    n = inp.n
+    if options.n_neighbors <= 0:
+        _raise(ERR_INVALID_NEIGHBORS, None)
+    if n <= 0:
+        _raise(ERR_EMPTY_DATA, None)
+    k = effective_k(options.n_neighbors, n)
+    res = _Output()
+    if out is not None:
+        idx_t, dist_t = out
+        res.location = MEM_DEVICE if idx_t.is_cuda else MEM_HOST
+        res.indices = C.c_void_p(idx_t.data_ptr())
+        res.distances = C.c_void_p(dist_t.data_ptr())
+        idx = dist = None
+    else:
+        idx = np.empty((n, k), np.int32)
+        dist = np.empty((n, k), np.float32)
+        res.location = MEM_HOST
+        res.indices = C.c_void_p(idx.ctypes.data)
+        res.distances = C.c_void_p(dist.ctypes.data)
+    res.capacity_k = k
+    rc = lib().evoc_mi355x_knn_graph(ctx.handle, C.byref(inp), C.byref(o), rng._h, C.byref(res))
+    del keep
+    if rc != OK:
+        _raise(rc, ctx.handle)
+    if stats is not None:
+        stats.seconds = res.seconds
+        stats.device_seconds = res.device_seconds
+        stats.score_seconds = res.score_seconds
+        stats.iterations = res.iterations
+        stats.rescore_rounds = res.rescore_rounds
+    if out is not None:
+        return out
+    return idx, dist
+
+
+def knn_graph_seeded(data, options: KnnGraphOptions, tau_state, tree_states=None,
+                     ctx: Optional[Context] = None, stats: Optional[CallStats] = None, out=None):
+    """evoc_mi355x_knn_graph_seeded: the caller made the RNG draws itself."""
+    ctx = ctx or default_context()
+    inp, keep = _input_from(data)
+    o = options._c()
+    n = inp.n
+    k = effective_k(options.n_neighbors, n) if n > 0 else 1
+    tau = np.ascontiguousarray(tau_state, np.int64)
+    trees = None if tree_states is None else np.ascontiguousarray(tree_states, np.int64)
+    res = _Output()
+    if out is not None:
+        idx_t, dist_t = out
+        res.location = MEM_DEVICE if idx_t.is_cuda else MEM_HOST
+        res.indices = C.c_void_p(idx_t.data_ptr())
+        res.distances = C.c_void_p(dist_t.data_ptr())
+        idx = dist = None
+    else:
+        idx = np.empty((max(n, 0), k), np.int32)
+        dist = np.empty((max(n, 0), k), np.float32)
+        res.location = MEM_HOST
+        res.indices = C.c_void_p(idx.ctypes.data)
+        res.distances = C.c_void_p(dist.ctypes.data)
+    res.capacity_k = k
+    rc = lib().evoc_mi355x_knn_graph_seeded(
+        ctx.handle, C.byref(inp), C.byref(o), C.c_void_p(tau.ctypes.data),
+        C.c_void_p(trees.ctypes.data) if trees is not None else None,
+        0 if trees is None else trees.shape[0], C.byref(res))
+    del keep
+    if rc != OK:
+        _raise(rc, ctx.handle)
+    if stats is not None:
+        stats.seconds = res.seconds
+        stats.device_seconds = res.device_seconds
+        stats.score_seconds = res.score_seconds
+        stats.iterations = res.iterations
+        stats.rescore_rounds = res.rescore_rounds
+    if out is not None:
+        return out
+    return idx, dist

@@ -1,0 +1,153 @@
+/*
+ * evoc_mi355x — C ABI of the MI355X (gfx950) kNN-graph library.
+ *
+ * Drop-in for the reference's kNN-graph path (eugenehp/evoc-rs 0.0.1):
+ *   - knn_graph(data, options, rng)               src/knn.rs:345-359
+ *   - RlxBackend::knn_graph(data, opts, rng, sp)  src/rlx_backend/mod.rs:135-144
+ * A Rust `Mi355xBackend` implements that trait method by calling
+ * evoc_mi355x_knn_graph_seeded() after drawing the same NumPy-RNG values the
+ * reference draws (src/knn.rs:416 and :124-129).
+ *
+ * Plain pointers and sizes only.  All functions are thread-safe per context
+ * (calls on one context are serialised by an internal mutex).
+ */
+#ifndef EVOC_MI355X_H
+#define EVOC_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EVOC_MI355X_ABI_VERSION 1
+
+/* EmbeddingData::{Float32, Int8, UInt8}, src/knn.rs:16-20.  UInt8 rows are
+ * packed bits compared with bitwise Jaccard; d counts bytes. */
+enum evoc_dtype { EVOC_DTYPE_F32 = 0, EVOC_DTYPE_I8 = 1, EVOC_DTYPE_U8 = 2 };
+
+/* where a caller buffer lives */
+enum evoc_location { EVOC_MEM_HOST = 0, EVOC_MEM_DEVICE = 1 };
+
+/* search method */
+enum evoc_mode {
+  EVOC_MODE_EXACT = 0,     /* tiled exact all-pairs + top-k (default) */
+  EVOC_MODE_NNDESCENT = 1  /* NN-descent local join on the device */
+};
+
+/* f32 scoring precision (ignored for int8 / binary, which are exact) */
+enum evoc_precision {
+  EVOC_PREC_FP32 = 0,         /* FP32 MFMA scoring */
+  EVOC_PREC_BF16_RESCORE = 1  /* BF16 MFMA candidates, FP32 re-score of k + margin */
+};
+
+/* status codes (KnnError, src/knn.rs:33-41; BackendError, src/rlx_backend/mod.rs:191-202) */
+enum evoc_status {
+  EVOC_OK = 0,
+  EVOC_ERR_EMPTY_DATA = 1,        /* KnnError::EmptyData */
+  EVOC_ERR_INVALID_NEIGHBORS = 2, /* KnnError::InvalidNeighbors */
+  EVOC_ERR_INVALID_ARGUMENT = 3,
+  EVOC_ERR_UNSUPPORTED = 4,
+  EVOC_ERR_OUT_OF_BUDGET = 5,     /* request does not fit the context's HBM budget */
+  EVOC_ERR_HIP = 6,               /* HIP runtime error (message in last_error) */
+  EVOC_ERR_CANCELLED = 7,
+  EVOC_ERR_NOT_BITWISE = 8,       /* strict_precision requested: results are not bitwise the CPU path's */
+  EVOC_ERR_NO_DEVICE = 9
+};
+
+/* KnnGraphOptions, src/knn.rs:43-76, plus the mi355x extensions.
+ * Fields <= 0 marked "default" are derived exactly as src/knn.rs:402-414. */
+typedef struct evoc_knn_options {
+  int64_t n_neighbors;       /* default 30 */
+  int64_t n_trees;           /* <= 0: clamp(rayon_threads, 4, 8) */
+  int64_t leaf_size;         /* <= 0: max(10, k) */
+  int64_t max_candidates;    /* <= 0: min(60, floor(1.5 k)) */
+  int64_t max_rptree_depth;  /* default 200 */
+  int64_t n_iters;           /* <= 0: max(5, round(log2 n)) */
+  float delta;               /* default 0.001 */
+  float delta_improv;        /* default 0.001 */
+  int32_t use_sorted_updates;/* default 1 */
+  int32_t deterministic;     /* default 0 */
+  int64_t brute_force_max_n; /* default 0 */
+  /* mi355x extensions */
+  int32_t mode;              /* enum evoc_mode, default EXACT */
+  int32_t precision;         /* enum evoc_precision, default BF16_RESCORE */
+  int32_t margin;            /* BF16 candidates kept beyond k for the FP32 re-score, default 16 */
+  int32_t strict_precision;  /* RlxBackend strict_precision; nonzero -> EVOC_ERR_NOT_BITWISE */
+  int32_t rayon_threads;     /* caller's rayon::current_num_threads() for the n_trees default (0 = 1) */
+  int32_t reserved0;
+} evoc_knn_options;
+
+typedef struct evoc_knn_input {
+  int32_t dtype;      /* enum evoc_dtype */
+  int32_t location;   /* enum evoc_location */
+  const void* data;   /* row-major; row r starts at data + r * ld elements */
+  int64_t n;          /* rows */
+  int64_t d;          /* columns (u8: bytes per row) */
+  int64_t ld;         /* leading dimension in elements, >= d */
+} evoc_knn_input;
+
+typedef struct evoc_knn_output {
+  int32_t location;   /* enum evoc_location of indices/distances */
+  int32_t reserved0;
+  int32_t* indices;   /* n x k_out, row-major (Array2<i32>) */
+  float* distances;   /* n x k_out, transformed as src/knn.rs:156-187 */
+  int64_t capacity_k; /* columns the caller allocated; must be >= evoc_knn_effective_k() */
+  int64_t k_out;      /* set: min(n_neighbors, n - 1).max(1) */
+  /* statistics of the call */
+  double seconds;          /* wall time of the call */
+  double device_seconds;   /* device time between the first and last kernel */
+  double score_seconds;    /* device time of the all-pairs scoring kernel (HIP events) */
+  int64_t iterations;      /* NN-descent iterations run (0 for exact) */
+  int64_t rescore_rounds;  /* exact: extra scoring rounds for overflowing buffers */
+} evoc_knn_output;
+
+typedef struct evoc_mi355x_ctx evoc_mi355x_ctx;
+typedef struct evoc_numpy_rng evoc_numpy_rng;
+
+/* ---- library / context ---- */
+const char* evoc_mi355x_version(void);
+int evoc_mi355x_device_count(int32_t* count);
+/* hbm_budget_bytes = 0: 90% of the device's free memory at creation */
+int evoc_mi355x_create(int32_t device, uint64_t hbm_budget_bytes, evoc_mi355x_ctx** out);
+void evoc_mi355x_destroy(evoc_mi355x_ctx* ctx);
+const char* evoc_mi355x_last_error(const evoc_mi355x_ctx* ctx);
+
+/* ---- options and contract helpers ---- */
+void evoc_knn_options_default(evoc_knn_options* opts);           /* KnnGraphOptions::default() */
+int64_t evoc_knn_effective_k(int64_t n_neighbors, int64_t n);   /* src/knn.rs:402 */
+/* number of randint(INT32_MIN+1, INT32_MAX-1) draws knn_graph takes from the
+ * caller's RNG: 3 (randint3_for_tau) + 3 * n_trees, or 3 on the brute-force
+ * branch (src/knn.rs:416-431) */
+int64_t evoc_knn_rng_draws(int32_t dtype, int64_t n, const evoc_knn_options* opts);
+
+/* ---- kNN graph ----
+ * knn_graph (src/knn.rs:345-454) with the caller's NumPy RandomState: advances
+ * `rng` exactly as the reference does, then builds the graph on the device. */
+int evoc_mi355x_knn_graph(evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
+                          const evoc_knn_options* opts, evoc_numpy_rng* rng,
+                          evoc_knn_output* out);
+
+/* Same, with the draws already made by the caller (the Rust binding draws
+ * from its own NumpyRandomState): tau_state = randint3_for_tau(), tree_states =
+ * the n_tree_rows x 3 forest seeds of make_forest (may be NULL). */
+int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
+                                 const evoc_knn_options* opts, const int64_t tau_state[3],
+                                 const int64_t* tree_states, int64_t n_tree_rows,
+                                 evoc_knn_output* out);
+
+/* ---- NumPy RandomState (src/numpy_rng.rs:12-226), for C / Python callers ---- */
+evoc_numpy_rng* evoc_numpy_rng_new(uint32_t seed);        /* check_random_state(Some(seed)) */
+void evoc_numpy_rng_free(evoc_numpy_rng* rng);
+int64_t evoc_numpy_rng_randint(evoc_numpy_rng* rng, int64_t low, int64_t high);
+double evoc_numpy_rng_normal(evoc_numpy_rng* rng);
+double evoc_numpy_rng_random(evoc_numpy_rng* rng);
+void evoc_numpy_rng_get_state(const evoc_numpy_rng* rng, uint32_t key[624], int32_t* pos,
+                              int32_t* has_gauss, double* gauss);
+void evoc_numpy_rng_set_state(evoc_numpy_rng* rng, const uint32_t key[624], int32_t pos,
+                              int32_t has_gauss, double gauss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EVOC_MI355X_H */

@@ -1,0 +1,219 @@
+"""Exact-mode parity on the MI355X, through the C ABI.
+
+Checked against: the reference's golden fixtures (tests/parity.rs:34-68 rule),
+the reference's RNG-advance goldens, and the CPU oracle on the same seeded
+inputs (bit-exact for int8 and binary rows; the tie-aware 1e-6 rule for f32,
+whose summation order differs from the reference's fast-math C loop).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from parity_util import (FIXTURES, GOLDEN, assert_knn_parity, boundary_mismatch_rows,
+                         knn_parity_counts, load_fixture, recall)
+
+pytestmark = pytest.mark.gpu
+
+E = pytest.importorskip("evoc_mi355x")
+from evoc_mi355x import synth  # noqa: E402
+
+PRECS = [("fp32", E.PREC_FP32), ("bf16", E.PREC_BF16_RESCORE)]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    print(f"[gpu] evoc_mi355x {E.version()}", flush=True)
+    c = E.Context(0)
+    yield c
+    c.close()
+
+
+def run(ctx, x, seed=0, **kw):
+    return E.knn_graph(x, E.KnnGraphOptions(**kw), E.NumpyRandomState(seed), ctx=ctx)
+
+
+# --- reference goldens ---------------------------------------------------------------
+
+@pytest.mark.parametrize("prec_name,prec", PRECS)
+@pytest.mark.parametrize("name,seed", FIXTURES)
+def test_goldens(ctx, name, seed, prec_name, prec):
+    """tests/parity.rs:135-189 through the mi355x backend."""
+    print(f"[gpu] golden {name} {prec_name}", flush=True)
+    x, ei, ed, _ = load_fixture(name)
+    rng = E.NumpyRandomState(seed)
+    idx, dist = E.knn_graph(x, E.KnnGraphOptions(n_neighbors=15, deterministic=True, precision=prec),
+                            rng, ctx=ctx)
+    assert idx.shape == ei.shape and dist.shape == ed.shape
+    assert_knn_parity(idx, dist, ei, ed)
+    # RNG left exactly where the reference leaves it (parity_intermediates.rs:656-687)
+    key, pos, _, _ = rng.get_state()
+    assert np.array_equal(key, np.load(os.path.join(GOLDEN, name, "rng_after_knn_key.npy")))
+    assert pos == int(np.load(os.path.join(GOLDEN, name, "rng_after_knn_meta.npz"))["pos"])
+
+
+def test_small_200_default_options_exact_indices(ctx):
+    """tests/parity.rs:235-260: default options, index equality."""
+    x, ei, _, _ = load_fixture("small_200")
+    idx, _ = run(ctx, x, 42, n_neighbors=15)
+    assert np.array_equal(idx, ei)
+
+
+# --- config 1: 2,000 x 128 f32, 10 blobs, k = 15 -------------------------------------
+
+@pytest.mark.parametrize("prec_name,prec", PRECS)
+def test_config1_vs_oracle(ctx, prec_name, prec):
+    print(f"[gpu] config1 {prec_name}", flush=True)
+    x = synth.blobs_f32(2000, 128, 10, 0.3, seed=42)
+    idx, dist = run(ctx, x, 42, n_neighbors=15, precision=prec)
+    # the reference's deterministic path (C fast-math cosine, NN-descent)
+    oi, od, _ = O.knn_graph(x, O.KnnGraphOptions(n_neighbors=15, deterministic=True),
+                            O.NumpyRandomState(42))
+    dist_m, bad, tie = knn_parity_counts(idx, dist, oi, od)
+    assert dist_m == 0 and bad == 0 and tie <= 8, (dist_m, bad, tie)
+    # and the exact oracle with the same ranking rule
+    ei, ed = O.exact_knn(x, 15, acc=O.ACC_FASTMATH)
+    et = O.transform(O.F32, ed.astype(np.float32))
+    assert boundary_mismatch_rows(idx, dist, ei, et, 1e-6) == 0
+    assert (idx[:, 0] == np.arange(2000)).all()
+
+
+# --- int8 and binary: bit-exact --------------------------------------------------------
+
+@pytest.mark.parametrize("n,d,k", [(3000, 512, 15), (1000, 64, 30), (257, 3, 7)])
+def test_int8_exact_bitwise(ctx, n, d, k):
+    print(f"[gpu] int8 n={n} d={d} k={k}", flush=True)
+    x = synth.blobs_i8(n, d, 20, 0.2, seed=n)
+    idx, dist = run(ctx, x, 1, n_neighbors=k)
+    ei, ed = O.exact_knn(x, k)
+    np.testing.assert_array_equal(idx, ei)
+    np.testing.assert_array_equal(dist, O.transform(O.I8, ed.astype(np.float32)))
+
+
+@pytest.mark.parametrize("n,nbytes,k", [(3000, 128, 15), (700, 16, 10), (130, 5, 4)])
+def test_binary_exact_bitwise(ctx, n, nbytes, k):
+    print(f"[gpu] binary n={n} bytes={nbytes} k={k}", flush=True)
+    x = synth.bits_u8(n, nbytes * 8, 25, 0.3, 0.08, seed=n)
+    idx, dist = run(ctx, x, 1, n_neighbors=k)
+    ei, ed = O.exact_knn(x, k)
+    np.testing.assert_array_equal(idx, ei)
+    np.testing.assert_array_equal(dist.view(np.uint32),
+                                  O.transform(O.U8, ed.astype(np.float32)).view(np.uint32))
+
+
+@pytest.mark.parametrize("dtype", ["i8", "u8"])
+def test_brute_force_branch_skips_self(ctx, dtype):
+    """src/knn.rs:418-422 (n <= brute_force_max_n): self skipped, 3 RNG draws."""
+    if dtype == "i8":
+        x = synth.blobs_i8(600, 40, 6, 0.3, seed=3)
+    else:
+        x = synth.bits_u8(600, 128, 6, 0.3, 0.1, seed=3)
+    rng = E.NumpyRandomState(9)
+    idx, dist = E.knn_graph(x, E.KnnGraphOptions(n_neighbors=10, brute_force_max_n=1000), rng, ctx=ctx)
+    ref = E.NumpyRandomState(9)
+    for _ in range(3):
+        ref.randint(E.INT32_MIN_P1, E.INT32_MAX_M1)
+    assert np.array_equal(rng.get_state()[0], ref.get_state()[0])
+    ei, ed = O.exact_knn(x, 10, include_self=False)
+    np.testing.assert_array_equal(idx, ei)
+    # the reference's own brute force (unsorted heap order) holds the same distances
+    _, bd = O.brute_force_knn(x, 10)
+    code = O.I8 if dtype == "i8" else O.U8
+    np.testing.assert_array_equal(dist, O.transform(code, np.sort(bd, axis=1)))
+    assert not (idx == np.arange(600)[:, None]).any()
+
+
+# --- edge cases ----------------------------------------------------------------------
+
+def test_single_row(ctx):
+    idx, dist = run(ctx, np.ones((1, 8), np.float32), n_neighbors=5)
+    assert idx.shape == (1, 1) and idx[0, 0] == 0 and dist[0, 0] == 0.0
+
+
+def test_k_larger_than_n(ctx):
+    x = synth.blobs_f32(5, 16, 2, 0.3, seed=1)
+    idx, dist = run(ctx, x, n_neighbors=50)
+    assert idx.shape == (5, 4)
+    for r in range(5):
+        assert idx[r, 0] == r and len(set(idx[r].tolist())) == 4
+
+
+def test_non_unit_rows_are_renormalised(ctx):
+    """prepare_float (src/knn.rs:95-113): one off-unit row renormalises all rows."""
+    x = synth.blobs_f32(900, 40, 5, 0.3, seed=2, normalize=False)
+    idx, dist = run(ctx, x, n_neighbors=12, precision=E.PREC_FP32)
+    xp = O.prepare_float(x)
+    ei, ed = O.exact_knn(xp, 12, acc=O.ACC_FASTMATH)
+    et = O.transform(O.F32, ed.astype(np.float32))
+    assert boundary_mismatch_rows(idx, dist, ei, et, 1e-6) == 0
+
+
+def test_zero_rows_and_negative_similarity(ctx):
+    """dot <= 0 scores FLT_MIN (src/fast_cosine.rs:33-45): such neighbours tie, lowest index wins."""
+    x = np.zeros((300, 8), np.float32)
+    x[:150, 0] = 1.0
+    x[150:, 0] = -1.0
+    x[:, 1] = np.linspace(0.0, 0.01, 300, dtype=np.float32)
+    x[5] = 0.0
+    idx, dist = run(ctx, x, n_neighbors=6, precision=E.PREC_FP32)
+    xp = O.prepare_float(x)
+    ei, ed = O.exact_knn(xp, 6, acc=O.ACC_SERIAL)
+    et = O.transform(O.F32, ed.astype(np.float32))
+    # the zero row only has FLT_MIN neighbours: itself, then indices 0, 1, 2, ...
+    assert idx[5].tolist() == [5, 0, 1, 2, 3, 4]
+    assert boundary_mismatch_rows(idx, dist, ei, et, 1e-6) == 0
+
+
+@pytest.mark.parametrize("dtype", ["i8", "u8"])
+def test_massive_exact_ties(ctx, dtype):
+    """3000 rows that are 30 distinct rows repeated: every candidate list overflows and is
+    resolved by the key-bound rounds; exact arithmetic makes the answer unique."""
+    if dtype == "i8":
+        base = synth.blobs_i8(30, 64, 5, 0.3, seed=4)
+    else:
+        base = synth.bits_u8(30, 256, 5, 0.3, 0.1, seed=4)
+    x = np.ascontiguousarray(np.tile(base, (100, 1)))
+    st = E.CallStats()
+    idx, dist = E.knn_graph(x, E.KnnGraphOptions(n_neighbors=20), E.NumpyRandomState(0), ctx=ctx,
+                            stats=st)
+    ei, ed = O.exact_knn(x, 20)
+    np.testing.assert_array_equal(idx, ei)
+
+
+def test_errors(ctx):
+    with pytest.raises(E.InvalidNeighbors):
+        run(ctx, np.ones((4, 4), np.float32), n_neighbors=0)
+    with pytest.raises(E.EmptyData):
+        run(ctx, np.ones((0, 4), np.float32), n_neighbors=3)
+    with pytest.raises(E.BackendError) as ei:
+        run(ctx, np.ones((4, 4), np.float32), n_neighbors=3, strict_precision=True)
+    assert ei.value.code == E.ERR_NOT_BITWISE
+    small = E.Context(0, hbm_budget_bytes=1 << 16)
+    with pytest.raises(E.BackendError) as ei:
+        E.knn_graph(synth.blobs_f32(5000, 64, 4, 0.3, 1), E.KnnGraphOptions(n_neighbors=5),
+                    E.NumpyRandomState(0), ctx=small)
+    assert ei.value.code == E.ERR_OUT_OF_BUDGET
+    small.close()
+
+
+# --- recall against an FP64 brute force at a C2-like shape ----------------------------
+
+def test_recall_vs_fp64_20k_768(ctx):
+    torch = pytest.importorskip("torch")
+    print("[gpu] recall 20k x 768", flush=True)
+    n, d, k = 20000, 768, 30
+    x = synth.blobs_f32(n, d, 50, 0.25, seed=42, zipf=1.1)
+    xt = torch.from_numpy(x).cuda()
+    x64 = xt.double()
+    truth = np.empty((n, k), np.int64)
+    for s in range(0, n, 2048):
+        e = min(n, s + 2048)
+        sim = x64[s:e] @ x64.T
+        sim[torch.arange(e - s), torch.arange(s, e)] = 2.0  # self first (NN-descent layout)
+        truth[s:e] = torch.topk(sim, k, dim=1).indices.cpu().numpy()
+    for name, prec in PRECS:
+        idx, dist = run(ctx, x, n_neighbors=k, precision=prec)
+        r = recall(idx, truth)
+        print(f"[gpu] recall {name} = {r:.5f}", flush=True)
+        assert r >= 0.99
