@@ -121,6 +121,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it first (python __graft_entry__.py build "
                               "or make -C evoc-rs_amd/csrc)")
+        # One HIP runtime per process: when PyTorch is installed, load its bundled
+        # libamdhip64.so.7 first so this library binds to the same runtime (same
+        # SONAME) instead of a second copy that cannot see the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (args, res) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -1,0 +1,99 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function that
+include/evoc_mi355x.h declares, and its host-side contract helpers behave like the
+reference (options defaults, effective k, RNG draws, NumPy RandomState)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import evoc_mi355x as E
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "evoc_mi355x.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(evoc_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    E.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", E.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (evoc_[a-z0-9_]+)", out))
+    declared = declared_functions()
+    assert declared, "no declarations parsed"
+    missing = [f for f in declared if f not in exported]
+    assert not missing, missing
+    # the Python mirror binds every declared function
+    assert set(declared) == set(E.SIGNATURES)
+
+
+def test_library_is_gfx950_only():
+    """The offload bundle ids embedded in the library name only gfx950."""
+    blob = open(E.LIB_PATH, "rb").read()
+    targets = set(re.findall(rb"amdgcn-amd-amdhsa--(gfx[0-9a-z]+)", blob))
+    assert targets == {b"gfx950"}
+
+
+def test_numpy_rng_matches_reference_kat():
+    """src/numpy_rng.rs:232-252 through the ABI."""
+    r = E.NumpyRandomState(42)
+    lo, hi = E.INT32_MIN_P1, E.INT32_MAX_M1
+    assert [r.randint(lo, hi) for _ in range(3)] == [-538846105, 1273642420, 1935803229]
+    assert [r.randint(lo, hi) for _ in range(5)] == [-1359637233, 996406379, 1201263688, 423734973,
+                                                      415968277]
+    nrm = [r.normal() for _ in range(3)]
+    np.testing.assert_allclose(nrm, [-0.23415337472333597, -0.23413695694918055, 1.5792128155073915],
+                               atol=1e-12)
+
+
+def test_numpy_rng_state_roundtrip_and_oracle_agreement():
+    a = E.NumpyRandomState(7)
+    b = O.NumpyRandomState(7)
+    for _ in range(1000):  # crosses an MT19937 refill
+        assert a.randint(0, 1 << 31) == b.randint(0, 1 << 31)
+    key, pos, hg, g = a.get_state()
+    c = E.NumpyRandomState(0)
+    c.set_state(key, pos, hg, g)
+    assert [c.randint(-5, 5) for _ in range(20)] == [a.randint(-5, 5) for _ in range(20)]
+
+
+def test_options_default_mirror_reference():
+    """KnnGraphOptions::default(), src/knn.rs:60-76."""
+    o = E.KnnGraphOptions()._c()
+    assert o.n_neighbors == 30 and o.max_rptree_depth == 200
+    assert abs(o.delta - 0.001) < 1e-9 and abs(o.delta_improv - 0.001) < 1e-9
+    assert o.use_sorted_updates == 1 and o.brute_force_max_n == 0 and o.deterministic == 0
+
+
+@pytest.mark.parametrize("k,n,exp", [(15, 200, 15), (15, 1, 1), (15, 2, 1), (50, 5, 4), (1, 10, 1)])
+def test_effective_k(k, n, exp):
+    """src/knn.rs:402: min(k, n - 1).max(1)."""
+    assert E.effective_k(k, n) == exp
+
+
+def test_rng_draws_contract():
+    """src/knn.rs:416 (3 draws) + src/knn.rs:124-129 (3 per tree); 3 on the brute-force branch."""
+    assert E.rng_draws(E.DTYPE_F32, 1000, E.KnnGraphOptions(deterministic=True)) == 15
+    assert E.rng_draws(E.DTYPE_F32, 1000, E.KnnGraphOptions(rayon_threads=6)) == 21
+    assert E.rng_draws(E.DTYPE_F32, 1000, E.KnnGraphOptions(rayon_threads=32)) == 27
+    assert E.rng_draws(E.DTYPE_F32, 1000, E.KnnGraphOptions(n_trees=2)) == 9
+    bf = E.KnnGraphOptions(brute_force_max_n=5000)
+    assert E.rng_draws(E.DTYPE_I8, 1000, bf) == 3
+    assert E.rng_draws(E.DTYPE_U8, 1000, bf) == 3
+    assert E.rng_draws(E.DTYPE_F32, 1000, bf) == 15  # f32 never takes the brute-force branch
+
+
+def test_errors_before_device_use():
+    """Invalid arguments are reported before any GPU work (src/knn.rs:382-393)."""
+    if E.device_count() > 0:
+        pytest.skip("a device is visible; covered by the GPU tests")
+    with pytest.raises(E.BackendError) as ei:
+        E.Context(0)
+    assert ei.value.code == E.ERR_NO_DEVICE
