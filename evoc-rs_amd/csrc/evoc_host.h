@@ -5,6 +5,7 @@
 
 #include <mutex>
 #include <string>
+#include <vector>
 
 struct evoc_mi355x_ctx {
   int device = -1;
@@ -40,6 +41,40 @@ struct ExactStats {
   int64_t slabs = 0;
   double pass2_ms = 0.0;     // device time of the scoring pass (all slabs)
 };
+
+// Host -> device copy on the context stream that returns only when the copy is
+// done, so the source may be a short-lived host object.
+inline hipError_t upload_sync(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e;
+}
+
+struct NndParams {
+  int n_iters = 10;
+  int max_candidates = 22;
+  int n_trees = 4;
+  int64_t leaf_size = 15;
+  int64_t max_depth = 200;
+  float delta = 0.001f;
+  float delta_improv = 0.001f;
+  bool use_delta_improv = true;
+  int k = 15;                       // neighbours the caller asked for (stopping rule)
+  uint64_t seed = 0;                // candidate priorities / random fill
+  std::vector<uint64_t> tree_seeds; // one per tree
+};
+
+struct NndStats {
+  int64_t iterations = 0;
+  int64_t leaves = 0;
+  uint32_t dropped = 0;
+  std::vector<uint32_t> changed;
+};
+
+// NN-descent over all rows (nndescent.hip); out_keys: n x Kg exact-format keys.
+int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, const uint32_t* pop,
+                     const float* frows, int64_t frow_ld, int dim, int64_t n, int Kg,
+                     const NndParams& P, uint64_t* out_keys, NndStats* st);
 
 // Exact K best candidates per query over all rows (see knn_exact.hip).
 // rows: n x row_bytes scored rows; pop: per-row popcounts (binary only).

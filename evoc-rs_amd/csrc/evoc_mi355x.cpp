@@ -334,13 +334,12 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
   (void)tree_states;
   (void)n_tree_rows;
   out->k_out = k;
+  out->offers_dropped = 0;
   out->score_seconds = 0.0;
   out->iterations = 0;
   out->rescore_rounds = 0;
   if (o->mode != EVOC_MODE_EXACT && o->mode != EVOC_MODE_NNDESCENT)
     return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "unknown mode");
-  if (o->mode == EVOC_MODE_NNDESCENT)
-    return ctx->fail(EVOC_ERR_UNSUPPORTED, "NN-descent mode is not built into this library version");
   if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(EVOC_ERR_HIP, "hipSetDevice failed");
 
   hipStream_t st = ctx->stream;
@@ -407,26 +406,67 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
   }
   if (e != hipSuccess) return ctx->hip_fail(e, "input preparation");
 
-  // ---- exact search ----
-  const int64_t K = bf16 ? std::min<int64_t>(k + std::max(0, o->margin), n) : k;
+  // ---- search ----
+  // NN-descent mode runs the device NN-descent; the brute-force branch
+  // (src/knn.rs:418-422) and the exact mode run the tiled exact search.
+  const bool nnd = o->mode == EVOC_MODE_NNDESCENT && seed_self;
+  int64_t K = bf16 ? std::min<int64_t>(k + std::max(0, o->margin), n) : k;
+  if (nnd) K = std::min<int64_t>(K, 64);
   if (bf16 && K > 256)
     return ctx->fail(EVOC_ERR_UNSUPPORTED, "BF16 re-score keeps at most 256 candidates: need k + margin <= 256");
   if (K > 1024) return ctx->fail(EVOC_ERR_UNSUPPORTED, "exact search supports k <= 1024");
+  if (nnd && k > 64) return ctx->fail(EVOC_ERR_UNSUPPORTED, "NN-descent mode supports k <= 64");
   DevBuf keys;
   if ((rc = keys.alloc(ctx, (size_t)(n * K * 8), "candidate keys"))) return rc;
-  ExactStats xs;
-  rc = exact_search(ctx, kind, (const uint8_t*)rows.ptr, row_bytes, (const uint32_t*)pop.ptr, n, (int)K,
-                    seed_self, (uint64_t*)keys.ptr, &xs);
-  if (rc) return rc;
-  out->rescore_rounds = xs.extra_rounds;
-  out->score_seconds = xs.pass2_ms * 1e-3;
+  if (nnd) {
+    NndParams P;
+    P.k = (int)k;
+    P.n_iters = o->n_iters > 0 ? (int)o->n_iters
+                               : (int)std::max<int64_t>(5, (int64_t)std::llround(std::log2((double)n)));
+    P.max_candidates = o->max_candidates > 0 ? (int)o->max_candidates
+                                             : (int)std::min<int64_t>(60, (int64_t)((float)k * 1.5f));
+    P.n_trees = (int)default_n_trees(o);
+    P.leaf_size = std::max<int64_t>(o->leaf_size > 0 ? o->leaf_size : std::max<int64_t>(10, k), std::max<int64_t>(10, k));
+    P.max_depth = o->max_rptree_depth;
+    P.delta = o->delta;
+    P.delta_improv = o->delta_improv;
+    P.use_delta_improv = true;
+    P.seed = ((uint64_t)tau_state[0] * 0x9e3779b97f4a7c15ull) ^ ((uint64_t)tau_state[1] << 21) ^
+             ((uint64_t)tau_state[2] << 42);
+    for (int t = 0; t < P.n_trees; t++) {
+      uint64_t ts = P.seed + 0x632be59bd9b4e019ull * (uint64_t)(t + 1);
+      if (tree_states && t < n_tree_rows)
+        ts = ((uint64_t)tree_states[3 * t] << 40) ^ ((uint64_t)tree_states[3 * t + 1] << 20) ^
+             (uint64_t)tree_states[3 * t + 2];
+      P.tree_seeds.push_back(ts);
+    }
+    const float* frows = dtype == EVOC_DTYPE_F32 ? (const float*)(bf16 ? xf32.ptr : rows.ptr) : nullptr;
+    NndStats ns;
+    rc = nndescent_search(ctx, kind, (const uint8_t*)rows.ptr, row_bytes, (const uint32_t*)pop.ptr, frows,
+                          f32_ld, (int)d, n, (int)K, P, (uint64_t*)keys.ptr, &ns);
+    if (rc) return rc;
+    out->iterations = ns.iterations;
+    out->offers_dropped = ns.dropped;
+  } else {
+    ExactStats xs;
+    rc = exact_search(ctx, kind, (const uint8_t*)rows.ptr, row_bytes, (const uint32_t*)pop.ptr, n, (int)K,
+                      seed_self, (uint64_t*)keys.ptr, &xs);
+    if (rc) return rc;
+    out->rescore_rounds = xs.extra_rounds;
+    out->score_seconds = xs.pass2_ms * 1e-3;
+  }
 
   DevBuf fkeys;
   const uint64_t* final_keys = (const uint64_t*)keys.ptr;
-  if (bf16) {
+  if (bf16 || K != k) {
     if ((rc = fkeys.alloc(ctx, (size_t)(n * k * 8), "rescored keys"))) return rc;
-    e = launch_rescore((const float*)xf32.ptr, f32_ld, (int)d, (int)n, 0, (const uint64_t*)keys.ptr, (int)K,
-                       (int)k, (uint64_t*)fkeys.ptr, st);
+    if (dtype == EVOC_DTYPE_F32) {
+      e = launch_rescore((const float*)(bf16 ? xf32.ptr : rows.ptr), f32_ld, (int)d, (int)n, 0,
+                         (const uint64_t*)keys.ptr, (int)K, (int)k, (uint64_t*)fkeys.ptr, st);
+    } else {
+      e = hipMemcpy2DAsync(fkeys.ptr, (size_t)k * 8, keys.ptr, (size_t)K * 8, (size_t)k * 8, (size_t)n,
+                           hipMemcpyDeviceToDevice, st);
+    }
     if (e != hipSuccess) return ctx->hip_fail(e, "rescore");
     final_keys = (const uint64_t*)fkeys.ptr;
   }

@@ -200,9 +200,8 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
           qlist.push_back((int32_t)q);
       int32_t* d_blocks = (int32_t*)list.ptr;
       int32_t* d_q = d_blocks + nqb;
-      e = hipMemcpyAsync(d_blocks, blocks.data(), blocks.size() * 4, hipMemcpyHostToDevice, s);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(d_q, qlist.data(), qlist.size() * 4, hipMemcpyHostToDevice, s);
+      e = upload_sync(d_blocks, blocks.data(), blocks.size() * 4, s);
+      if (e == hipSuccess) e = upload_sync(d_q, qlist.data(), qlist.size() * 4, s);
       TileArgs ar = a2;
       ar.block_list = d_blocks;
       ar.tau = nullptr;

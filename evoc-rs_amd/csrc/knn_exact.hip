@@ -23,13 +23,9 @@
 
 #include "evoc_device.h"
 #include "evoc_kernels.h"
+#include "mfma_traits.h"
 
 namespace evoc {
-
-typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
-typedef __attribute__((ext_vector_type(16))) float f32x16;
-typedef __attribute__((ext_vector_type(4))) int i32x4;
-typedef __attribute__((ext_vector_type(16))) int i32x16;
 
 // ---------------------------------------------------------------------------
 // Tile geometry.  256 threads = 4 waves in a 2 (candidates) x 2 (queries)
@@ -42,44 +38,6 @@ constexpr int TILE = 128;            // rows per tile (both operands)
 constexpr int STAGE_BYTES = 128;     // bytes of each row staged per K step
 constexpr int ROW_PITCH = 144;       // padded LDS row pitch (bytes)
 constexpr int OPERAND_LDS = TILE * ROW_PITCH;
-
-template <int KIND> struct Mfma;
-
-template <> struct Mfma<EVOC_ELEM_BF16> {
-  static constexpr int KSTEP_BYTES = 32;  // 16 bf16 per MFMA K step
-  static constexpr int LANE_BYTES = 16;   // 8 bf16 per lane
-  typedef f32x16 acc_t;
-  __device__ static acc_t zero() { return acc_t{}; }
-  __device__ static void step(acc_t& c, const uint8_t* a, const uint8_t* b) {
-    bf16x8 av = __builtin_bit_cast(bf16x8, *(const i32x4*)a);
-    bf16x8 bv = __builtin_bit_cast(bf16x8, *(const i32x4*)b);
-    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, c, 0, 0, 0);
-  }
-  __device__ static float value(const acc_t& c, int j) { return f32_score(c[j]); }
-};
-
-template <> struct Mfma<EVOC_ELEM_F32> {
-  static constexpr int KSTEP_BYTES = 8;  // 2 f32 per MFMA K step
-  static constexpr int LANE_BYTES = 4;
-  typedef f32x16 acc_t;
-  __device__ static acc_t zero() { return acc_t{}; }
-  __device__ static void step(acc_t& c, const uint8_t* a, const uint8_t* b) {
-    c = __builtin_amdgcn_mfma_f32_32x32x2f32(*(const float*)a, *(const float*)b, c, 0, 0, 0);
-  }
-  __device__ static float value(const acc_t& c, int j) { return f32_score(c[j]); }
-};
-
-template <> struct Mfma<EVOC_ELEM_I8> {
-  static constexpr int KSTEP_BYTES = 32;  // 32 i8 per MFMA K step
-  static constexpr int LANE_BYTES = 16;
-  typedef i32x16 acc_t;
-  __device__ static acc_t zero() { return acc_t{}; }
-  __device__ static void step(acc_t& c, const uint8_t* a, const uint8_t* b) {
-    c = __builtin_amdgcn_mfma_i32_32x32x32_i8(*(const i32x4*)a, *(const i32x4*)b, c, 0, 0, 0);
-  }
-  // int8.rs:15-21: -(dot as f32); the score is dot as f32 (exact below 2^24)
-  __device__ static float value(const acc_t& c, int j) { return (float)c[j]; }
-};
 
 // Cooperative 16-byte loads of one 128-row x 128-byte stage of an operand into
 // registers (4 chunks per thread), then into LDS.

@@ -1,0 +1,238 @@
+// NN-descent driver (host side of nndescent.hip): forest initialisation,
+// random fill, and the candidate / local-join / merge iterations with the
+// reference's stopping rules (src/nndescent/float.rs:775-879).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <utility>
+#include <vector>
+
+#include "evoc_host.h"
+#include "evoc_kernels.h"
+#include "evoc_mi355x.h"
+#include "nndescent.h"
+
+using namespace evoc;
+
+namespace {
+
+constexpr int UCAP = 192;  // offers buffered per vertex between merges
+
+uint64_t mix(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+struct Check {
+  Ctx* ctx;
+  hipError_t e = hipSuccess;
+  const char* what = "";
+  bool operator()(hipError_t r, const char* w) {
+    if (r != hipSuccess && e == hipSuccess) {
+      e = r;
+      what = w;
+    }
+    return e == hipSuccess;
+  }
+  int fail() { return ctx->hip_fail(e, what); }
+};
+
+}  // namespace
+
+int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, const uint32_t* pop,
+                     const float* frows, int64_t frow_ld, int dim, int64_t n64, int Kg,
+                     const NndParams& P, uint64_t* out_keys, NndStats* st) {
+  hipStream_t s = ctx->stream;
+  const int n = (int)n64;
+  if (Kg < 1 || Kg > 64) return ctx->fail(EVOC_ERR_UNSUPPORTED, "NN-descent keeps at most 64 neighbours per vertex");
+  const int mc = std::min(P.max_candidates, 64);
+  if (mc < 1) return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "max_candidates must be positive");
+  const int leaf_size = std::min<int64_t>(std::max<int64_t>(P.leaf_size, 2), 64);
+  const int nb = (n + 1023) / 1024;
+
+  DevBuf keys, thr, ucnt, ubuf, newc, oldc, revcnt, revoff, revcur, rev, sums, counters, sumd;
+  int rc;
+  if ((rc = keys.alloc(ctx, (size_t)n * Kg * 8, "neighbour lists")) || (rc = thr.alloc(ctx, (size_t)n * 4, "list bounds")) ||
+      (rc = ucnt.alloc(ctx, (size_t)n * 4, "offer counters")) ||
+      (rc = ubuf.alloc(ctx, (size_t)n * UCAP * 8, "offer buffers")) ||
+      (rc = newc.alloc(ctx, (size_t)n * mc * 4, "new candidates")) ||
+      (rc = oldc.alloc(ctx, (size_t)n * mc * 4, "old candidates")) ||
+      (rc = revcnt.alloc(ctx, (size_t)n * 4, "reverse counts")) ||
+      (rc = revoff.alloc(ctx, (size_t)(n + 1) * 4, "reverse offsets")) ||
+      (rc = revcur.alloc(ctx, (size_t)n * 4, "reverse cursors")) ||
+      (rc = rev.alloc(ctx, (size_t)n * Kg * 4, "reverse edges")) ||
+      (rc = sums.alloc(ctx, (size_t)(nb + 2) * 4, "scan sums")) ||
+      (rc = counters.alloc(ctx, 16, "counters")) || (rc = sumd.alloc(ctx, 8, "distance sum")))
+    return rc;
+  uint32_t* d_changed = (uint32_t*)counters.ptr;
+  uint32_t* d_dropped = d_changed + 1;
+  Check ck{ctx};
+  ck(launch_fill_i32((int32_t*)counters.ptr, 4, 0, s), "counters");
+  ck(nnd_launch_init((uint64_t*)keys.ptr, (float*)thr.ptr, (int32_t*)ucnt.ptr, n, Kg, s), "list init");
+
+  JoinArgs ja = {};
+  ja.rows = rows;
+  ja.row_bytes = row_bytes;
+  ja.row_pop = pop;
+  ja.thr = (const float*)thr.ptr;
+  ja.ucnt = (int32_t*)ucnt.ptr;
+  ja.ubuf = (uint64_t*)ubuf.ptr;
+  ja.ucap = UCAP;
+  ja.n = n;
+  ja.dropped = d_dropped;
+  ja.newc = (const int32_t*)newc.ptr;
+  ja.oldc = (const int32_t*)oldc.ptr;
+  ja.mc = mc;
+
+  auto merge_all = [&]() {
+    return ck(nnd_launch_merge((uint64_t*)keys.ptr, (float*)thr.ptr, (int32_t*)ucnt.ptr,
+                               (const uint64_t*)ubuf.ptr, UCAP, n, Kg, 0, n, d_changed, s),
+              "merge");
+  };
+
+  // ---- random-projection forest: leaves seed the lists (float.rs:243-348) ----
+  if (P.n_trees > 0 && n > 1 && ck.e == hipSuccess) {
+    const int max_seg = n / (leaf_size + 1) + 2;
+    DevBuf perm, perm2, seg_of, side, segs, nsegs, leaves, fcnt, hyper;
+    const int64_t hyper_ld = kind == EVOC_ELEM_U8BITS ? (row_bytes * 2 + 3) / 4 : dim;
+    if ((rc = perm.alloc(ctx, (size_t)n * 4, "forest perm")) || (rc = perm2.alloc(ctx, (size_t)n * 4, "forest perm")) ||
+        (rc = seg_of.alloc(ctx, (size_t)n * 4, "forest segments")) ||
+        (rc = side.alloc(ctx, (size_t)n, "forest sides")) ||
+        (rc = segs.alloc(ctx, (size_t)max_seg * 16, "forest segments")) ||
+        (rc = nsegs.alloc(ctx, (size_t)max_seg * 16, "forest segments")) ||
+        (rc = leaves.alloc(ctx, (size_t)(2 * (int64_t)n + 2) * 8, "forest leaves")) ||
+        (rc = fcnt.alloc(ctx, 16, "forest counters")) ||
+        (rc = hyper.alloc(ctx, (size_t)max_seg * hyper_ld * 4, "hyperplanes")))
+      return rc;
+    ForestArgs fa = {};
+    fa.kind = kind;
+    fa.n = n;
+    fa.dim = dim;
+    fa.leaf_size = leaf_size;
+    fa.rows = rows;
+    fa.row_bytes = row_bytes;
+    fa.frows = frows;
+    fa.frow_ld = frow_ld;
+    fa.hyper = (float*)hyper.ptr;
+    fa.hyper_ld = hyper_ld;
+    std::vector<int32_t> iota((size_t)n);
+    for (int i = 0; i < n; i++) iota[(size_t)i] = i;
+    for (int t = 0; t < P.n_trees && ck.e == hipSuccess; t++) {
+      fa.seed = mix(P.tree_seeds[(size_t)t]);
+      int32_t* d_perm = (int32_t*)perm.ptr;
+      int32_t* d_perm2 = (int32_t*)perm2.ptr;
+      int4* d_segs = (int4*)segs.ptr;
+      int4* d_next = (int4*)nsegs.ptr;
+      int32_t* d_fc = (int32_t*)fcnt.ptr;  // [0] next segments, [1] leaves
+      ck(upload_sync(d_perm, iota.data(), (size_t)n * 4, s), "forest perm");
+      ck(launch_fill_i32(d_fc, 4, 0, s), "forest counters");
+      int nseg;
+      if (n > leaf_size) {
+        const int4 root = make_int4(0, n, 0, 0);
+        ck(upload_sync(d_segs, &root, sizeof(root), s), "forest root");
+        nseg = 1;
+      } else {
+        const int2 leaf = make_int2(0, n);
+        const int32_t one = 1;
+        ck(upload_sync(leaves.ptr, &leaf, sizeof(leaf), s), "forest leaf");
+        ck(upload_sync(d_fc + 1, &one, 4, s), "forest leaf");
+        nseg = 0;
+      }
+      for (int level = 0; nseg > 0 && ck.e == hipSuccess; level++) {
+        const int last = level + 1 >= P.max_depth ? 1 : 0;
+        ck(launch_fill_i32(d_fc, 1, 0, s), "forest counters");
+        ck(rp_launch_level(d_segs, nseg, d_perm, d_perm2, (int32_t*)seg_of.ptr, (int8_t*)side.ptr, fa, level, d_next,
+                           d_fc, (int2*)leaves.ptr, d_fc + 1, last, s),
+           "forest level");
+        std::swap(d_perm, d_perm2);
+        std::swap(d_segs, d_next);
+        int32_t nn = 0;
+        ck(hipMemcpyAsync(&nn, d_fc, 4, hipMemcpyDeviceToHost, s), "forest counters");
+        ck(hipStreamSynchronize(s), "forest level");
+        nseg = nn;
+      }
+      int32_t nl = 0;
+      ck(hipMemcpyAsync(&nl, d_fc + 1, 4, hipMemcpyDeviceToHost, s), "forest leaves");
+      ck(hipStreamSynchronize(s), "forest");
+      JoinArgs jl = ja;
+      jl.mode = 1;
+      jl.n_items = nl;
+      jl.leaves = (const int2*)leaves.ptr;
+      jl.perm = d_perm;
+      if (nl > 0) ck(nnd_launch_join(kind, jl, s), "leaf join");
+      merge_all();
+      if (st) st->leaves += nl;
+    }
+  }
+
+  // ---- random fill of rows with empty slots (float.rs:350-390) ----
+  ck(nnd_launch_random_lists((const uint64_t*)keys.ptr, n, Kg, mc, mix(P.seed ^ 0x5851f42d4c957f2dull),
+                             (int32_t*)newc.ptr, (int32_t*)oldc.ptr, s),
+     "random lists");
+  {
+    JoinArgs jr = ja;
+    jr.mode = 0;
+    jr.v0 = 0;
+    jr.n_items = n;
+    ck(nnd_launch_join(kind, jr, s), "random join");
+    merge_all();
+  }
+
+  // ---- iterations (float.rs:807-874) ----
+  double prev_sum = 0.0;
+  bool have_prev = false;
+  int64_t block = std::max<int64_t>(std::min<int64_t>(n, 16384), n / 8);
+  int it = 0;
+  for (; it < P.n_iters && ck.e == hipSuccess; it++) {
+    ck(launch_fill_i32((int32_t*)d_changed, 1, 0, s), "counters");
+    ck(nnd_launch_candidates((uint64_t*)keys.ptr, n, Kg, (int32_t*)revcnt.ptr, (int32_t*)revoff.ptr,
+                             (int32_t*)revcur.ptr, (int32_t*)rev.ptr, (int32_t*)sums.ptr, mc, P.seed, it,
+                             (int32_t*)newc.ptr, (int32_t*)oldc.ptr, s),
+       "candidates");
+    for (int64_t v0 = 0; v0 < n && ck.e == hipSuccess; v0 += block) {
+      JoinArgs jj = ja;
+      jj.mode = 0;
+      jj.v0 = (int32_t)v0;
+      jj.n_items = (int32_t)std::min<int64_t>(block, n - v0);
+      ck(nnd_launch_join(kind, jj, s), "local join");
+      merge_all();
+    }
+    block = std::min<int64_t>(block, n) * 2;  // float.rs:872-873
+    uint32_t c = 0;
+    ck(hipMemcpyAsync(&c, d_changed, 4, hipMemcpyDeviceToHost, s), "counters");
+    if (P.use_delta_improv) {
+      const double zero = 0.0;
+      ck(upload_sync(sumd.ptr, &zero, 8, s), "distance sum");
+      ck(nnd_launch_sum_dist((const uint64_t*)keys.ptr, (int64_t)n * Kg, (double*)sumd.ptr, s), "distance sum");
+    }
+    double sum = 0.0;
+    if (P.use_delta_improv) ck(hipMemcpyAsync(&sum, sumd.ptr, 8, hipMemcpyDeviceToHost, s), "distance sum");
+    ck(hipStreamSynchronize(s), "iteration");
+    if (st) st->changed.push_back(c);
+    // float.rs:847: c <= (delta * k * n) as u32
+    const float lim = P.delta * (float)P.k * (float)n;
+    if ((double)c <= (lim <= 0.0f ? 0.0 : (double)(uint32_t)lim)) {
+      it++;
+      break;
+    }
+    if (P.use_delta_improv) {  // float.rs:853-870
+      if (have_prev && std::abs(sum - prev_sum) / std::abs(prev_sum) < (double)P.delta_improv) {
+        it++;
+        break;
+      }
+      prev_sum = sum;
+      have_prev = true;
+    }
+  }
+  if (st) st->iterations = it;
+  ck(nnd_launch_export((const uint64_t*)keys.ptr, n, Kg, out_keys, s), "export");
+  uint32_t dropped = 0;
+  ck(hipMemcpyAsync(&dropped, d_dropped, 4, hipMemcpyDeviceToHost, s), "counters");
+  ck(hipStreamSynchronize(s), "NN-descent");
+  if (st) st->dropped = dropped;
+  if (ck.e != hipSuccess) return ck.fail();
+  return EVOC_OK;
+}

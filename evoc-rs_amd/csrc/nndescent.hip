@@ -1,0 +1,761 @@
+// NN-descent on the device (gfx950).
+//
+// The reference's loop (src/nndescent/float.rs:775-879, int8.rs:575-678,
+// uint8.rs:563-666) re-expressed for one GPU:
+//   lists      per-vertex neighbour lists of Kg 64-bit keys kept sorted best
+//              first (the reference's bounded max-heaps, src/heap.rs:27-200)
+//   candidates forward edges + a CSR of reverse edges; each vertex keeps the
+//              mc smallest priorities per new/old class (src/heap.rs:203-343)
+//   local join per vertex, MFMA Gram of its new list against new + old lists
+//              (float.rs:517-669); a pair is offered to a vertex when it beats
+//              that vertex's current worst entry
+//   apply      offers land in bounded per-vertex buffers; one wave per vertex
+//              merges buffer and list (sort, de-duplicate, keep Kg)
+//              (src/heap.rs:425-502)
+//   forest     random-projection trees split level by level (float.rs:13-177)
+//              whose leaves seed the lists (float.rs:179-348)
+// Priorities and pivots come from a counter-based hash (splitmix64 finalizer,
+// Steele, Lea & Flood 2014) instead of the sequential tau RNG, so every vertex
+// and every split can run in parallel.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "evoc_device.h"
+#include "evoc_kernels.h"
+#include "mfma_traits.h"
+#include "nndescent.h"
+
+namespace evoc {
+
+// ---- graph keys: [ord(score):32][0x3fffffff - idx:30][from_list:1][new:1] ----
+__device__ __forceinline__ uint64_t gkey(float score, int32_t idx, uint32_t from_list, uint32_t flag) {
+  return ((uint64_t)ord_f32(score) << 32) | ((uint64_t)(0x3fffffffu - (uint32_t)idx) << 2) |
+         ((uint64_t)from_list << 1) | (uint64_t)flag;
+}
+__device__ __forceinline__ int32_t gkey_idx(uint64_t k) {
+  return (int32_t)(0x3fffffffu - (uint32_t)((k >> 2) & 0x3fffffffu));
+}
+__device__ __forceinline__ float gkey_score(uint64_t k) { return unord_f32((uint32_t)(k >> 32)); }
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// ---------------------------------------------------------------------------
+// List initialisation: self seeded with the reference's -1.0 (score 1.0,
+// float.rs:210-214), the rest empty.
+// ---------------------------------------------------------------------------
+__global__ void nnd_init_lists_kernel(uint64_t* __restrict__ keys, float* __restrict__ thr,
+                                      int32_t* __restrict__ ucnt, int n, int Kg) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n * Kg) return;
+  const int v = (int)(i / Kg), s = (int)(i % Kg);
+  keys[i] = s == 0 ? gkey(1.0f, v, 0, 1) : EVOC_KEY_NONE;
+  if (s == 0) {
+    thr[v] = Kg == 1 ? 1.0f : -INFINITY;
+    ucnt[v] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Local join.  One wave per work item: a "new" list N (<= 64 vertices) and an
+// "old" list O; C = N followed by O (<= 128).  Scores of N x C on MFMA from
+// rows gathered straight from HBM, then every pair (a < c inside N, all of
+// N x O) is offered to both endpoints.
+// ---------------------------------------------------------------------------
+struct JoinShared {
+  int32_t nidx[64];
+  int32_t cidx[128];
+  float cthr[128];
+};
+
+__device__ __forceinline__ void offer(const JoinArgs& a, int32_t target, int32_t cand, float s) {
+  const int slot = atomicAdd(&a.ucnt[target], 1);
+  if (slot < a.ucap)
+    a.ubuf[(int64_t)target * a.ucap + slot] = gkey(s, cand, 0, 1);
+  else
+    atomicAdd(a.dropped, 1u);
+}
+
+// gather the item's lists into LDS; returns (nN, nC) through refs
+__device__ __forceinline__ void load_item(const JoinArgs& a, int item, JoinShared& sh, int& nN,
+                                          int& nC) {
+  const int lane = threadIdx.x & 63;
+  int32_t nv = -1, ov = -1;
+  if (a.mode == 0) {
+    // iteration: new/old candidate lists of vertex a.v0 + item (mc <= 64 each)
+    const int v = a.v0 + item;
+    if (lane < a.mc) {
+      nv = a.newc[(int64_t)v * a.mc + lane];
+      ov = a.oldc[(int64_t)v * a.mc + lane];
+    }
+  } else {
+    // forest leaf: members perm[start .. start + len), len <= 64
+    const int2 lf = a.leaves[item];
+    if (lane < lf.y) nv = a.perm[lf.x + lane];
+  }
+  const uint64_t mn = __ballot(nv >= 0), mo = __ballot(ov >= 0);
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  nN = __popcll(mn);
+  nC = nN + __popcll(mo);
+  if (nv >= 0) {
+    const int p = __popcll(mn & below);
+    sh.nidx[p] = nv;
+    sh.cidx[p] = nv;
+    sh.cthr[p] = a.thr[nv];
+  }
+  if (ov >= 0) {
+    const int p = nN + __popcll(mo & below);
+    sh.cidx[p] = ov;
+    sh.cthr[p] = a.thr[ov];
+  }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void nnd_join_mfma_kernel(JoinArgs a) {
+  using M = Mfma<KIND>;
+  __shared__ JoinShared shm[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int item = blockIdx.x * 4 + w;
+  if (item >= a.n_items) return;
+  JoinShared& sh = shm[w];
+  int nN, nC;
+  load_item(a, item, sh, nN, nC);
+  __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
+  if (nN == 0 || nC < 2) return;
+  const int nbm = (nN + 31) >> 5, nbn = (nC + 31) >> 5;
+  // fragment row pointers (invalid rows read a valid row; masked below)
+  const uint8_t* arow[2];
+  const uint8_t* brow[4];
+#pragma unroll
+  for (int bm = 0; bm < 2; bm++) {
+    const int r = bm * 32 + (lane & 31);
+    arow[bm] = a.rows + (int64_t)sh.nidx[r < nN ? r : 0] * a.row_bytes + (lane >> 5) * M::LANE_BYTES;
+  }
+#pragma unroll
+  for (int bn = 0; bn < 4; bn++) {
+    const int c = bn * 32 + (lane & 31);
+    brow[bn] = a.rows + (int64_t)sh.cidx[c < nC ? c : 0] * a.row_bytes + (lane >> 5) * M::LANE_BYTES;
+  }
+  typename M::acc_t acc[2][4];
+#pragma unroll
+  for (int bm = 0; bm < 2; bm++)
+#pragma unroll
+    for (int bn = 0; bn < 4; bn++) acc[bm][bn] = M::zero();
+  const int ksteps = (int)(a.row_bytes / M::KSTEP_BYTES);
+  for (int ks = 0; ks < ksteps; ks++) {
+    const int off = ks * M::KSTEP_BYTES;
+#pragma unroll
+    for (int bm = 0; bm < 2; bm++) {
+      if (bm < nbm) {
+#pragma unroll
+        for (int bn = 0; bn < 4; bn++)
+          if (bn < nbn) M::step(acc[bm][bn], arow[bm] + off, brow[bn] + off);
+      }
+    }
+  }
+#pragma unroll
+  for (int bm = 0; bm < 2; bm++) {
+    if (bm >= nbm) continue;
+#pragma unroll
+    for (int bn = 0; bn < 4; bn++) {
+      if (bn >= nbn) continue;
+      const int c = bn * 32 + (lane & 31);
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        const int r = bm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+        if (r >= nN || c >= nC) continue;
+        if (c < nN && r >= c) continue;  // each new-new pair once
+        const int32_t p = sh.nidx[r], q = sh.cidx[c];
+        if (p == q) continue;
+        const float s = M::value(acc[bm][bn], j);
+        if (s > sh.cthr[r]) offer(a, p, q, s);
+        if (s > sh.cthr[c]) offer(a, q, p, s);
+      }
+    }
+  }
+}
+
+// Binary rows: Jaccard on the VALU, one wave per item, pairs spread over lanes.
+__global__ __launch_bounds__(256) void nnd_join_jaccard_kernel(JoinArgs a) {
+  __shared__ JoinShared shm[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int item = blockIdx.x * 4 + w;
+  if (item >= a.n_items) return;
+  JoinShared& sh = shm[w];
+  int nN, nC;
+  load_item(a, item, sh, nN, nC);
+  __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
+  const int words = (int)(a.row_bytes / 4);
+  for (int pi = lane; pi < nN * nC; pi += 64) {
+    const int r = pi / nC, c = pi % nC;
+    if (c < nN && r >= c) continue;
+    const int32_t p = sh.nidx[r], q = sh.cidx[c];
+    if (p == q) continue;
+    const uint32_t* xp = (const uint32_t*)(a.rows + (int64_t)p * a.row_bytes);
+    const uint32_t* xq = (const uint32_t*)(a.rows + (int64_t)q * a.row_bytes);
+    uint32_t inter = 0;
+    for (int t = 0; t < words; t++) inter += __popc(xp[t] & xq[t]);
+    const uint32_t uni = a.row_pop[p] + a.row_pop[q] - inter;
+    const float s = uni > 0 ? __fdiv_rn((float)inter, (float)uni) : 0.0f;
+    if (s > a.thr[p]) offer(a, p, q, s);
+    if (s > a.thr[q]) offer(a, q, p, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Merge offers into the lists: one wave per vertex, list (<= 64) + offers
+// (<= 192) sorted best first, duplicates of one neighbour collapse onto the
+// copy already in the list (flagged_heap_push rejects a present index,
+// src/heap.rs:156-160), the first Kg survive.  New entries carry flag 1.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void nnd_merge_kernel(uint64_t* __restrict__ keys,
+                                                        float* __restrict__ thr,
+                                                        int32_t* __restrict__ ucnt,
+                                                        const uint64_t* __restrict__ ubuf, int ucap,
+                                                        int n, int Kg, int v0, int nv,
+                                                        uint32_t* __restrict__ changed) {
+  constexpr int E = 4;
+  const int lane = threadIdx.x & 63;
+  const int vi = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (vi >= nv) return;
+  const int v = v0 + vi;
+  const int cnt = min(ucnt[v], ucap);
+  if (cnt == 0) return;
+  uint64_t x[E];
+  x[0] = lane < Kg ? (keys[(int64_t)v * Kg + lane] | (keys[(int64_t)v * Kg + lane] ? 2ull : 0ull))
+                   : EVOC_KEY_NONE;
+#pragma unroll
+  for (int e = 1; e < E; e++) {
+    const int u = (e - 1) * 64 + lane;
+    x[e] = u < cnt ? ubuf[(int64_t)v * ucap + u] : EVOC_KEY_NONE;
+  }
+  wave_sort_desc<E>(x);
+  // keep[i]: non-empty and not a repeat of element i - 1 (same score and index)
+  bool keep[E];
+  uint64_t last_prev = 0;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const uint64_t up = (uint64_t)__shfl_up((long long)x[e], 1, 64);
+    const uint64_t prev = lane == 0 ? last_prev : up;
+    keep[e] = x[e] != EVOC_KEY_NONE && (e == 0 && lane == 0 ? true : (x[e] >> 2) != (prev >> 2));
+    last_prev = (uint64_t)__shfl((long long)x[e], 63, 64);
+  }
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int base = 0;
+  int inserted = 0;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const uint64_t m = __ballot(keep[e]);
+    const int pos = base + __popcll(m & below);
+    if (keep[e] && pos < Kg) {
+      keys[(int64_t)v * Kg + pos] = x[e] & ~2ull;
+      if (!(x[e] & 2ull)) inserted++;
+      if (pos == Kg - 1) thr[v] = gkey_score(x[e]);
+    }
+    base += __popcll(m);
+  }
+  // slots beyond the kept entries stay empty
+  for (int pos = base + lane; pos < Kg; pos += 64) keys[(int64_t)v * Kg + pos] = EVOC_KEY_NONE;
+  if (base < Kg && lane == 0) thr[v] = -INFINITY;
+  inserted = wave_sum_i32(inserted);
+  if (lane == 0) {
+    atomicAdd(changed, (uint32_t)inserted);
+    ucnt[v] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Candidate sampling (src/heap.rs:203-343).
+// ---------------------------------------------------------------------------
+__global__ void nnd_rev_count_kernel(const uint64_t* __restrict__ keys, int n, int Kg,
+                                     int32_t* __restrict__ revcnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n * Kg) return;
+  const uint64_t k = keys[i];
+  if (k == EVOC_KEY_NONE) return;
+  const int v = (int)(i / Kg), t = gkey_idx(k);
+  if (t != v) atomicAdd(&revcnt[t], 1);
+}
+
+__global__ void nnd_rev_fill_kernel(const uint64_t* __restrict__ keys, int n, int Kg,
+                                    const int32_t* __restrict__ revoff, int32_t* __restrict__ revcur,
+                                    int32_t* __restrict__ rev) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n * Kg) return;
+  const uint64_t k = keys[i];
+  if (k == EVOC_KEY_NONE) return;
+  const int v = (int)(i / Kg), t = gkey_idx(k);
+  if (t == v) return;
+  const int pos = revoff[t] + atomicAdd(&revcur[t], 1);
+  rev[pos] = v | (int32_t)((uint32_t)(k & 1ull) << 31);
+}
+
+// priority of edge {a, b} in iteration `it`: the same for both directions
+__device__ __forceinline__ uint32_t edge_priority(int a, int b, uint64_t seed, int it) {
+  const uint32_t lo = (uint32_t)min(a, b), hi = (uint32_t)max(a, b);
+  return (uint32_t)(mix64(((uint64_t)lo << 32 | hi) ^ (seed + 0x632be59bd9b4e019ull * (uint64_t)(it + 1))) >> 32);
+}
+
+// sort key: larger = smaller priority, then smaller candidate
+__device__ __forceinline__ uint64_t cand_key(uint32_t prio, int32_t c) {
+  return ((uint64_t)(~prio) << 32) | (uint64_t)(0xffffffffu - (uint32_t)c);
+}
+__device__ __forceinline__ int32_t cand_of(uint64_t k) { return (int32_t)(0xffffffffu - (uint32_t)k); }
+
+// keep the mc best of (best ∪ chunk): best holds one key per lane (lane < mc)
+__device__ __forceinline__ uint64_t keep_best(uint64_t best, uint64_t chunk, int mc) {
+  uint64_t x[2] = {best, chunk};
+  wave_sort_desc<2>(x);
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  bool keep[2];
+  uint64_t last_prev = 0;
+#pragma unroll
+  for (int e = 0; e < 2; e++) {
+    const uint64_t up = (uint64_t)__shfl_up((long long)x[e], 1, 64);
+    const uint64_t prev = lane == 0 ? last_prev : up;
+    keep[e] = x[e] != EVOC_KEY_NONE && ((e == 0 && lane == 0) || x[e] != prev);
+    last_prev = (uint64_t)__shfl((long long)x[e], 63, 64);
+  }
+  // scatter the first mc kept keys to lanes 0..mc-1 through a register shuffle
+  uint64_t out = EVOC_KEY_NONE;
+  int base = 0;
+#pragma unroll
+  for (int e = 0; e < 2; e++) {
+    const uint64_t m = __ballot(keep[e]);
+    // lane L wants the (L - base)-th kept element of register e
+    const int want = lane - base;
+    const int total = __popcll(m);
+    int src = 0;
+    if (want >= 0 && want < total) {
+      // position of the want-th set bit of m
+      uint64_t mm = m;
+      for (int t = 0; t < want; t++) mm &= mm - 1;
+      src = __ffsll((long long)mm) - 1;
+    }
+    const uint64_t got = (uint64_t)__shfl((long long)x[e], src, 64);
+    if (want >= 0 && want < total && lane < mc) out = got;
+    base += total;
+  }
+  return out;
+}
+
+__global__ __launch_bounds__(256) void nnd_build_candidates_kernel(
+    uint64_t* __restrict__ keys, int n, int Kg, const int32_t* __restrict__ revoff,
+    const int32_t* __restrict__ rev, int mc, uint64_t seed, int it, int32_t* __restrict__ newc,
+    int32_t* __restrict__ oldc) {
+  const int lane = threadIdx.x & 63;
+  const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (v >= n) return;
+  const uint64_t fk = lane < Kg ? keys[(int64_t)v * Kg + lane] : EVOC_KEY_NONE;
+  const int32_t fc = fk != EVOC_KEY_NONE ? gkey_idx(fk) : -1;
+  const int fflag = (int)(fk & 1ull);
+  const int r0 = revoff[v], r1 = revoff[v + 1];
+  uint64_t best[2];
+#pragma unroll
+  for (int cls = 0; cls < 2; cls++) {  // cls 1 = new, 0 = old
+    uint64_t b = EVOC_KEY_NONE;
+    uint64_t ck = (fc >= 0 && fc != v && fflag == cls) ? cand_key(edge_priority(v, fc, seed, it), fc)
+                                                       : EVOC_KEY_NONE;
+    b = keep_best(b, ck, mc);
+    for (int base = r0; base < r1; base += 64) {
+      const int i = base + lane;
+      uint64_t rk = EVOC_KEY_NONE;
+      if (i < r1) {
+        const int32_t e = rev[i];
+        const int c = e & 0x7fffffff, f = (int)((uint32_t)e >> 31);
+        if (f == cls) rk = cand_key(edge_priority(v, c, seed, it), c);
+      }
+      b = keep_best(b, rk, mc);
+    }
+    best[cls] = b;
+  }
+  if (lane < mc) {
+    newc[(int64_t)v * mc + lane] = best[1] != EVOC_KEY_NONE ? cand_of(best[1]) : -1;
+    oldc[(int64_t)v * mc + lane] = best[0] != EVOC_KEY_NONE ? cand_of(best[0]) : -1;
+  }
+  // clear the flag of forward entries sampled into the new list (heap.rs:319-340)
+  bool sampled = false;
+  for (int t = 0; t < mc; t++) {
+    const uint64_t bk = (uint64_t)__shfl((long long)best[1], t, 64);
+    if (bk != EVOC_KEY_NONE && cand_of(bk) == fc) sampled = true;
+  }
+  if (lane < Kg && fc >= 0 && fflag && sampled) keys[(int64_t)v * Kg + lane] = fk & ~1ull;
+}
+
+// ---------------------------------------------------------------------------
+// Random fill (float.rs:350-390): a vertex whose list still has empty slots
+// gets the item N = {v}, O = random vertices; the join kernel scores them with
+// the same MFMA path as every other pair, so repeated offers de-duplicate.
+// ---------------------------------------------------------------------------
+__global__ void nnd_random_lists_kernel(const uint64_t* __restrict__ keys, int n, int Kg, int mc,
+                                        uint64_t seed, int32_t* __restrict__ newc,
+                                        int32_t* __restrict__ oldc) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int filled = 0;
+  for (int s = 0; s < Kg; s++) filled += keys[(int64_t)v * Kg + s] != EVOC_KEY_NONE;
+  int32_t* nc = newc + (int64_t)v * mc;
+  int32_t* oc = oldc + (int64_t)v * mc;
+  for (int s = 0; s < mc; s++) {
+    nc[s] = -1;
+    oc[s] = -1;
+  }
+  if (filled >= Kg || n < 2) return;
+  nc[0] = v;
+  int j = 0;
+  for (int t = 0; t < mc && j < Kg - filled + 1; t++) {
+    const int c = (int)(mix64(seed ^ ((uint64_t)(uint32_t)v << 24) ^ (uint64_t)t) % (uint64_t)n);
+    if (c != v) oc[j++] = c;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Level-synchronous random-projection forest (float.rs:13-177 semantics:
+// hyperplane between two random members, margin sign, random side when
+// |margin| < 1e-8, random halves when one side is empty).
+// ---------------------------------------------------------------------------
+__global__ void rp_seg_fill_kernel(const int4* __restrict__ segs, int nseg, int32_t* __restrict__ seg_of) {
+  const int s = blockIdx.x;
+  if (s >= nseg) return;
+  const int4 g = segs[s];
+  for (int i = threadIdx.x; i < g.y; i += blockDim.x) seg_of[g.x + i] = s;
+}
+
+// one wave per active segment: hyperplane (float) or masks (binary) into hyper[s]
+__global__ __launch_bounds__(256) void rp_hyperplane_kernel(const int4* __restrict__ segs, int nseg,
+                                                            const int32_t* __restrict__ perm,
+                                                            ForestArgs f, int level) {
+  const int lane = threadIdx.x & 63;
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= nseg) return;
+  const int4 g = segs[s];
+  const uint64_t h = mix64(f.seed ^ ((uint64_t)level << 40) ^ (uint64_t)(uint32_t)g.x);
+  const int li = (int)(h % (uint64_t)g.y);
+  int ri = (int)((h >> 32) % (uint64_t)g.y);
+  ri += (li == ri);
+  ri %= g.y;
+  const int L = perm[g.x + li], R = perm[g.x + ri];
+  float* hv = f.hyper + (int64_t)s * f.hyper_ld;
+  if (f.kind == EVOC_ELEM_U8BITS) {
+    const uint8_t* xl = f.rows + (int64_t)L * f.row_bytes;
+    const uint8_t* xr = f.rows + (int64_t)R * f.row_bytes;
+    uint8_t* pm = (uint8_t*)hv;  // positive mask then negative mask
+    for (int i = lane; i < f.row_bytes; i += 64) {
+      const uint8_t xo = xl[i] ^ xr[i];
+      pm[i] = xo & xl[i];
+      pm[f.row_bytes + i] = xo & xr[i];
+    }
+    return;
+  }
+  // per-row L2 norms so the int8 split normalises both pivots (int8.rs:43-76);
+  // f32 rows are unit already and their norms are ~1
+  float nl = 0.0f, nr = 0.0f;
+  for (int i = lane; i < f.dim; i += 64) {
+    const float a = f.kind == EVOC_ELEM_I8 ? (float)((const int8_t*)(f.rows + (int64_t)L * f.row_bytes))[i]
+                                           : ((const float*)(f.frows + (int64_t)L * f.frow_ld))[i];
+    const float b = f.kind == EVOC_ELEM_I8 ? (float)((const int8_t*)(f.rows + (int64_t)R * f.row_bytes))[i]
+                                           : ((const float*)(f.frows + (int64_t)R * f.frow_ld))[i];
+    nl = fmaf(a, a, nl);
+    nr = fmaf(b, b, nr);
+  }
+  nl = sqrtf(wave_sum_f32(nl));
+  nr = sqrtf(wave_sum_f32(nr));
+  float hn = 0.0f;
+  for (int i = lane; i < f.dim; i += 64) {
+    const float a = f.kind == EVOC_ELEM_I8 ? (float)((const int8_t*)(f.rows + (int64_t)L * f.row_bytes))[i]
+                                           : ((const float*)(f.frows + (int64_t)L * f.frow_ld))[i];
+    const float b = f.kind == EVOC_ELEM_I8 ? (float)((const int8_t*)(f.rows + (int64_t)R * f.row_bytes))[i]
+                                           : ((const float*)(f.frows + (int64_t)R * f.frow_ld))[i];
+    const float d = (nl > 0.0f ? a / nl : 0.0f) - (nr > 0.0f ? b / nr : 0.0f);
+    hv[i] = d;
+    hn = fmaf(d, d, hn);
+  }
+  hn = sqrtf(wave_sum_f32(hn));
+  if (hn < 1e-8f) hn = 1.0f;
+  for (int i = lane; i < f.dim; i += 64) hv[i] = hv[i] / hn;
+}
+
+// one wave per position of an active segment: side (0 left, 1 right)
+__global__ __launch_bounds__(256) void rp_side_kernel(const int32_t* __restrict__ perm,
+                                                      const int32_t* __restrict__ seg_of,
+                                                      const int4* __restrict__ segs, int n_pos,
+                                                      ForestArgs f, int level, int8_t* __restrict__ side) {
+  const int lane = threadIdx.x & 63;
+  const int pos = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (pos >= n_pos) return;
+  const int s = seg_of[pos];
+  if (s < 0) return;
+  const int p = perm[pos];
+  const float* hv = f.hyper + (int64_t)s * f.hyper_ld;
+  float m = 0.0f;
+  if (f.kind == EVOC_ELEM_U8BITS) {
+    const uint8_t* pm = (const uint8_t*)hv;
+    const uint8_t* x = f.rows + (int64_t)p * f.row_bytes;
+    int c = 0;
+    for (int i = lane; i < f.row_bytes; i += 64) c += __popc(pm[i] & x[i]) - __popc(pm[f.row_bytes + i] & x[i]);
+    m = (float)wave_sum_i32(c);
+  } else if (f.kind == EVOC_ELEM_I8) {
+    const int8_t* x = (const int8_t*)(f.rows + (int64_t)p * f.row_bytes);
+    for (int i = lane; i < f.dim; i += 64) m = fmaf(hv[i], (float)x[i], m);
+    m = wave_sum_f32(m);
+  } else {
+    const float* x = (const float*)(f.frows + (int64_t)p * f.frow_ld);
+    for (int i = lane; i < f.dim; i += 64) m = fmaf(hv[i], x[i], m);
+    m = wave_sum_f32(m);
+  }
+  if (lane == 0) {
+    int8_t sd;
+    if (fabsf(m) < 1e-8f)
+      sd = (int8_t)(mix64(f.seed ^ ((uint64_t)level << 40) ^ ((uint64_t)(uint32_t)p << 1) ^ 0x5bd1e995ull) & 1ull);
+    else
+      sd = m > 0.0f ? 0 : 1;
+    side[pos] = sd;
+  }
+}
+
+// one workgroup per active segment: stable partition (left first) into perm2,
+// random halves when a side is empty, children appended to the next level or
+// to the leaf list
+__global__ __launch_bounds__(256) void rp_partition_kernel(const int4* __restrict__ segs, int nseg,
+                                                           const int32_t* __restrict__ perm,
+                                                           int32_t* __restrict__ perm2,
+                                                           int8_t* __restrict__ side, ForestArgs f,
+                                                           int level, int4* __restrict__ next_segs,
+                                                           int32_t* __restrict__ n_next,
+                                                           int2* __restrict__ leaves,
+                                                           int32_t* __restrict__ n_leaves,
+                                                           int last_level) {
+  __shared__ int wsum[4];
+  __shared__ int total_left;
+  const int s = blockIdx.x;
+  if (s >= nseg) return;
+  const int4 g = segs[s];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    // count the left side
+    int c = 0;
+    for (int i = t; i < g.y; i += 256) c += side[g.x + i] == 0;
+    c = wave_sum_i32(c);
+    if (lane == 0) wsum[w] = c;
+    __syncthreads();
+    if (t == 0) total_left = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    const int nl = total_left;
+    if (nl > 0 && nl < g.y) break;
+    // one side empty: random halves (float.rs:77-90)
+    for (int i = t; i < g.y; i += 256)
+      side[g.x + i] = (int8_t)(mix64(f.seed ^ ((uint64_t)level << 40) ^ ((uint64_t)(uint32_t)(g.x + i) << 1) ^
+                                     0x2545f4914f6cdd1dull) & 1ull);
+    __syncthreads();
+  }
+  const int nl = total_left;
+  // stable scatter: chunks of 256 positions with a block prefix count
+  int left_base = 0, right_base = nl;
+  for (int c0 = 0; c0 < g.y; c0 += 256) {
+    const int i = c0 + t;
+    const int sd = i < g.y ? side[g.x + i] : 2;
+    const uint64_t ml = __ballot(sd == 0), mr = __ballot(sd == 1);
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    __shared__ int wl[4], wr[4];
+    if (lane == 0) {
+      wl[w] = __popcll(ml);
+      wr[w] = __popcll(mr);
+    }
+    __syncthreads();
+    int ol = 0, orr = 0;
+    for (int k = 0; k < w; k++) {
+      ol += wl[k];
+      orr += wr[k];
+    }
+    if (sd == 0) perm2[g.x + left_base + ol + __popcll(ml & below)] = perm[g.x + i];
+    if (sd == 1) perm2[g.x + right_base + orr + __popcll(mr & below)] = perm[g.x + i];
+    left_base += wl[0] + wl[1] + wl[2] + wl[3];
+    right_base += wr[0] + wr[1] + wr[2] + wr[3];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const int2 kids[2] = {make_int2(g.x, nl), make_int2(g.x + nl, g.y - nl)};
+    for (int k = 0; k < 2; k++) {
+      if (kids[k].y > f.leaf_size && !last_level) {
+        const int slot = atomicAdd(n_next, 1);
+        next_segs[slot] = make_int4(kids[k].x, kids[k].y, 0, 0);
+      } else {
+        const int slot = atomicAdd(n_leaves, 1);
+        leaves[slot] = kids[k];
+      }
+    }
+  }
+}
+
+// positions outside active segments keep their order
+__global__ void rp_copy_kernel(const int32_t* __restrict__ perm, int32_t* __restrict__ perm2,
+                               const int32_t* __restrict__ seg_of, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && seg_of[i] < 0) perm2[i] = perm[i];
+}
+
+// graph keys (sorted best first) -> exact-format keys for the re-score/finalise kernels
+__global__ void nnd_export_keys_kernel(const uint64_t* __restrict__ keys, int n, int Kg,
+                                       uint64_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n * Kg) return;
+  const uint64_t k = keys[i];
+  const int v = (int)(i / Kg);
+  if (k == EVOC_KEY_NONE) {
+    out[i] = EVOC_KEY_NONE;
+    return;
+  }
+  const int c = gkey_idx(k);
+  out[i] = make_key(gkey_score(k), c, c == v);
+}
+
+// sum over all filled slots of the raw distance -score, in f64 (float.rs:854-859)
+__global__ void nnd_sum_dist_kernel(const uint64_t* __restrict__ keys, int64_t m,
+                                    double* __restrict__ out) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = keys[i];
+    if (k != EVOC_KEY_NONE) s += -(double)gkey_score(k);
+  }
+  for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
+// ---------------------------------------------------------------------------
+// scans (exclusive, int32) for the reverse-edge CSR
+// ---------------------------------------------------------------------------
+__global__ void scan_block_kernel(const int32_t* __restrict__ in, int n, int32_t* __restrict__ out,
+                                  int32_t* __restrict__ block_sums) {
+  __shared__ int ws[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int i = blockIdx.x * 1024 + t;
+  const int x = i < n ? in[i] : 0;
+  int incl = x;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) ws[w] = incl;
+  __syncthreads();
+  int off = 0;
+  for (int k = 0; k < w; k++) off += ws[k];
+  if (i < n) out[i] = off + incl - x;
+  if (t == 1023) block_sums[blockIdx.x] = off + incl;
+}
+
+__global__ void scan_sums_kernel(int32_t* __restrict__ sums, int nb, int32_t* __restrict__ total) {
+  if (threadIdx.x != 0) return;
+  int run = 0;
+  for (int b = 0; b < nb; b++) {
+    const int v = sums[b];
+    sums[b] = run;
+    run += v;
+  }
+  *total = run;
+}
+
+__global__ void scan_add_kernel(int32_t* __restrict__ out, int n, const int32_t* __restrict__ sums,
+                                const int32_t* __restrict__ total) {
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  if (i < n) out[i] += sums[blockIdx.x];
+  if (i == 0) out[n] = *total;
+}
+
+__global__ void fill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+hipError_t nnd_launch_init(uint64_t* keys, float* thr, int32_t* ucnt, int n, int Kg, hipStream_t s) {
+  hipLaunchKernelGGL(nnd_init_lists_kernel, dim3(blocks_for((int64_t)n * Kg, 256)), dim3(256), 0, s, keys,
+                     thr, ucnt, n, Kg);
+  return hipGetLastError();
+}
+
+hipError_t nnd_launch_join(int kind, const JoinArgs& a, hipStream_t s) {
+  const dim3 g(blocks_for(a.n_items, 4)), b(256);
+  switch (kind) {
+    case EVOC_ELEM_BF16: hipLaunchKernelGGL(nnd_join_mfma_kernel<EVOC_ELEM_BF16>, g, b, 0, s, a); break;
+    case EVOC_ELEM_F32: hipLaunchKernelGGL(nnd_join_mfma_kernel<EVOC_ELEM_F32>, g, b, 0, s, a); break;
+    case EVOC_ELEM_I8: hipLaunchKernelGGL(nnd_join_mfma_kernel<EVOC_ELEM_I8>, g, b, 0, s, a); break;
+    case EVOC_ELEM_U8BITS: hipLaunchKernelGGL(nnd_join_jaccard_kernel, g, b, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t nnd_launch_random_lists(const uint64_t* keys, int n, int Kg, int mc, uint64_t seed, int32_t* newc,
+                                   int32_t* oldc, hipStream_t s) {
+  hipLaunchKernelGGL(nnd_random_lists_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, Kg, mc, seed,
+                     newc, oldc);
+  return hipGetLastError();
+}
+
+hipError_t nnd_launch_merge(uint64_t* keys, float* thr, int32_t* ucnt, const uint64_t* ubuf, int ucap, int n,
+                            int Kg, int v0, int nv, uint32_t* changed, hipStream_t s) {
+  hipLaunchKernelGGL(nnd_merge_kernel, dim3(blocks_for(nv, 4)), dim3(256), 0, s, keys, thr, ucnt, ubuf, ucap,
+                     n, Kg, v0, nv, changed);
+  return hipGetLastError();
+}
+
+hipError_t nnd_launch_candidates(uint64_t* keys, int n, int Kg, int32_t* revcnt, int32_t* revoff,
+                                 int32_t* revcur, int32_t* rev, int32_t* scan_sums, int mc, uint64_t seed,
+                                 int it, int32_t* newc, int32_t* oldc, hipStream_t s) {
+  const int64_t nk = (int64_t)n * Kg;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, revcnt, (int64_t)n, 0);
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, revcur, (int64_t)n, 0);
+  hipLaunchKernelGGL(nnd_rev_count_kernel, dim3(blocks_for(nk, 256)), dim3(256), 0, s, keys, n, Kg, revcnt);
+  const int nb = (int)blocks_for(n, 1024);
+  hipLaunchKernelGGL(scan_block_kernel, dim3(nb), dim3(1024), 0, s, revcnt, n, revoff, scan_sums);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(64), 0, s, scan_sums, nb, scan_sums + nb);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(1024), 0, s, revoff, n, scan_sums, scan_sums + nb);
+  hipLaunchKernelGGL(nnd_rev_fill_kernel, dim3(blocks_for(nk, 256)), dim3(256), 0, s, keys, n, Kg, revoff,
+                     revcur, rev);
+  hipLaunchKernelGGL(nnd_build_candidates_kernel, dim3(blocks_for(n, 4)), dim3(256), 0, s, keys, n, Kg, revoff,
+                     rev, mc, seed, it, newc, oldc);
+  return hipGetLastError();
+}
+
+hipError_t nnd_launch_export(const uint64_t* keys, int n, int Kg, uint64_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(nnd_export_keys_kernel, dim3(blocks_for((int64_t)n * Kg, 256)), dim3(256), 0, s, keys, n,
+                     Kg, out);
+  return hipGetLastError();
+}
+
+hipError_t nnd_launch_sum_dist(const uint64_t* keys, int64_t m, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(nnd_sum_dist_kernel, dim3(1024), dim3(256), 0, s, keys, m, out);
+  return hipGetLastError();
+}
+
+hipError_t rp_launch_level(const int4* segs, int nseg, const int32_t* perm, int32_t* perm2, int32_t* seg_of,
+                           int8_t* side, const ForestArgs& f, int level, int4* next_segs, int32_t* n_next,
+                           int2* leaves, int32_t* n_leaves, int last_level, hipStream_t s) {
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(f.n, 256)), dim3(256), 0, s, seg_of, (int64_t)f.n, -1);
+  hipLaunchKernelGGL(rp_seg_fill_kernel, dim3(nseg), dim3(256), 0, s, segs, nseg, seg_of);
+  hipLaunchKernelGGL(rp_hyperplane_kernel, dim3(blocks_for(nseg, 4)), dim3(256), 0, s, segs, nseg, perm, f, level);
+  hipLaunchKernelGGL(rp_side_kernel, dim3(blocks_for(f.n, 4)), dim3(256), 0, s, perm, seg_of, segs, f.n, f, level,
+                     side);
+  hipLaunchKernelGGL(rp_copy_kernel, dim3(blocks_for(f.n, 256)), dim3(256), 0, s, perm, perm2, seg_of, f.n);
+  hipLaunchKernelGGL(rp_partition_kernel, dim3(nseg), dim3(256), 0, s, segs, nseg, perm, perm2, side, f, level,
+                     next_segs, n_next, leaves, n_leaves, last_level);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace evoc

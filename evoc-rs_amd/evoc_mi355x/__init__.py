@@ -83,7 +83,7 @@ class _Output(C.Structure):
     _fields_ = [("location", C.c_int32), ("reserved0", C.c_int32), ("indices", C.c_void_p),
                 ("distances", C.c_void_p), ("capacity_k", C.c_int64), ("k_out", C.c_int64),
                 ("seconds", C.c_double), ("device_seconds", C.c_double), ("score_seconds", C.c_double),
-                ("iterations", C.c_int64), ("rescore_rounds", C.c_int64)]
+                ("iterations", C.c_int64), ("rescore_rounds", C.c_int64), ("offers_dropped", C.c_int64)]
 
 
 _lib = None
@@ -299,6 +299,7 @@ class CallStats:
     score_seconds: float = 0.0
     iterations: int = 0
     rescore_rounds: int = 0
+    offers_dropped: int = 0
     extra: dict = field(default_factory=dict)
 
 
@@ -373,6 +374,7 @@ def knn_graph(data, options: KnnGraphOptions, rng: NumpyRandomState,
         stats.score_seconds = res.score_seconds
         stats.iterations = res.iterations
         stats.rescore_rounds = res.rescore_rounds
+        stats.offers_dropped = res.offers_dropped
     if out is not None:
         return out
     return idx, dist
@@ -415,6 +417,7 @@ def knn_graph_seeded(data, options: KnnGraphOptions, tau_state, tree_states=None
         stats.score_seconds = res.score_seconds
         stats.iterations = res.iterations
         stats.rescore_rounds = res.rescore_rounds
+        stats.offers_dropped = res.offers_dropped
     if out is not None:
         return out
     return idx, dist

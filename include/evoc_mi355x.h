@@ -100,6 +100,7 @@ typedef struct evoc_knn_output {
   double score_seconds;    /* device time of the all-pairs scoring kernel (HIP events) */
   int64_t iterations;      /* NN-descent iterations run (0 for exact) */
   int64_t rescore_rounds;  /* exact: extra scoring rounds for overflowing buffers */
+  int64_t offers_dropped;  /* NN-descent: candidate offers lost to full per-vertex buffers */
 } evoc_knn_output;
 
 typedef struct evoc_mi355x_ctx evoc_mi355x_ctx;

