@@ -14,7 +14,7 @@
 
 namespace evoc {
 
-// One 128-query block against a range of 128-candidate tiles.
+// One query block against a range of candidate tiles (tile_rows() rows each).
 struct TileArgs {
   const uint8_t* q_rows;   // query rows (row_bytes apart, zero padded)
   const uint8_t* c_rows;   // candidate rows; candidate c is row c * c_stride
@@ -22,7 +22,7 @@ struct TileArgs {
   int32_t n_q;             // queries
   int32_t n_c;             // candidates in candidate index space
   int32_t c_stride;        // row stride of the candidate sample (pass 1)
-  int32_t n_ctiles;        // candidate tiles of 128
+  int32_t n_ctiles;        // candidate tiles
   int32_t tiles_per_split; // pass 2: tiles per blockIdx.y split
   int32_t q_offset;        // global index of query 0 (candidates are global)
   const int32_t* block_list;  // optional list of query blocks (re-scoring)
@@ -56,6 +56,8 @@ struct SelectArgs {
 };
 
 // ---- launchers (knn_exact.hip) ----
+int tile_rows(int kind);          // query / candidate rows per tile of the engine
+int groups_per_tile(int kind);    // pass-1 group maxima per candidate tile
 size_t tile_lds_bytes(int kind, int pass);
 hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, hipStream_t s);
 hipError_t launch_pass2(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);

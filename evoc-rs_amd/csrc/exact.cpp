@@ -16,7 +16,6 @@ using namespace evoc;
 
 namespace {
 
-constexpr int TILE = 128;
 constexpr int MAX_SORT = 2048;  // largest per-query candidate list one wave sorts
 constexpr int MAX_ROUNDS = 8;
 
@@ -42,12 +41,28 @@ struct Plan {
 
 Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   Plan p;
+  const int TILE = tile_rows(kind);
   const int64_t n_ctiles = cdiv(n, TILE);
   const int64_t nqb = cdiv(nq, TILE);
-  // enough workgroups to fill the chip several times over
-  int64_t ns = std::max<int64_t>(1, cdiv(4LL * n_cu, nqb));
-  ns = std::min<int64_t>({ns, 16, n_ctiles});
-  p.tiles_per_split = (int)cdiv(n_ctiles, ns);
+  // workgroups per CU at once: 1 for the 256-row MFMA tiles (LDS), 2 for binary
+  const int64_t slots = (int64_t)n_cu * (kind == EVOC_ELEM_U8BITS ? 2 : 1);
+  // candidate splits: the count whose workgroups fill the last dispatch round
+  // best (fraction of busy slots), fewer splits on ties
+  int64_t best_ns = 1;
+  double best_eff = -1.0;
+  for (int64_t ns = 1; ns <= std::min<int64_t>(16, n_ctiles); ns++) {
+    const int64_t tps = cdiv(n_ctiles, ns);
+    const int64_t real_ns = cdiv(n_ctiles, tps);
+    const int64_t wgs = nqb * real_ns;
+    const double rounds = (double)cdiv(wgs, slots);
+    // a round costs tps tiles; efficiency = useful tile work / (rounds * tps * slots)
+    const double eff = (double)(nqb * n_ctiles) / (rounds * (double)tps * (double)slots);
+    if (eff > best_eff + 1e-3) {
+      best_eff = eff;
+      best_ns = real_ns;
+    }
+  }
+  p.tiles_per_split = (int)cdiv(n_ctiles, best_ns);
   p.n_splits = (int)cdiv(n_ctiles, p.tiles_per_split);
   if (n <= MAX_SORT) {
     // every candidate fits one sorted list: no threshold pass
@@ -57,11 +72,11 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
     return p;
   }
   p.pass1 = true;
-  const int groups_per_tile = kind == EVOC_ELEM_U8BITS ? 16 : 8;
+  const int gpt = groups_per_tile(kind);
   int64_t stride = std::min<int64_t>(std::max<int64_t>(cdiv(n, 16384), 1), 16);
   for (;;) {
     const int64_t nsamp = cdiv(n, stride);
-    const int64_t ng = cdiv(nsamp, TILE) * groups_per_tile;
+    const int64_t ng = cdiv(nsamp, TILE) * gpt;
     if (ng <= MAX_SORT || stride >= n) {
       p.stride = (int)stride;
       p.n_sample = (int)nsamp;
@@ -86,6 +101,7 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
 int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, const uint32_t* pop,
                  int64_t n, int K, bool seed_self, uint64_t* keys, ExactStats* stats) {
   hipStream_t s = ctx->stream;
+  const int TILE = tile_rows(kind);
   if (K < 1 || K > MAX_SORT / 2 || K > n)
     return ctx->fail(EVOC_ERR_UNSUPPORTED, "exact search needs 1 <= K <= min(n, 1024)");
   // slab of queries whose scratch fits a quarter of the remaining budget

@@ -28,94 +28,105 @@
 namespace evoc {
 
 // ---------------------------------------------------------------------------
-// Tile geometry.  256 threads = 4 waves in a 2 (candidates) x 2 (queries)
-// arrangement; each wave owns a 64 x 64 sub-tile = 2 x 2 MFMA 32x32 blocks.
+// MFMA tile geometry.  256 threads = 4 waves in a 2 (candidates) x 2 (queries)
+// arrangement over a 256 x 256 tile; each wave owns 128 x 128 = 4 x 4 MFMA
+// 32x32 blocks (256 f32/i32 accumulators per lane), which halves the LDS bytes
+// read per MFMA against 64 x 64 wave tiles.
 // MFMA orientation: M (rows) = candidates (A operand), N (cols) = queries
 // (B operand), so lane l of a 32x32 block holds query column l % 32 and the 16
 // candidate rows 8*(j/4) + 4*(l/32) + j%4, j = 0..15.
 // ---------------------------------------------------------------------------
-constexpr int TILE = 128;            // rows per tile (both operands)
+constexpr int MTILE = 256;           // rows per tile (both operands)
+constexpr int WB = 4;                // 32-row blocks per wave and operand
 constexpr int STAGE_BYTES = 128;     // bytes of each row staged per K step
-constexpr int ROW_PITCH = 144;       // padded LDS row pitch (bytes)
-constexpr int OPERAND_LDS = TILE * ROW_PITCH;
+constexpr int OPERAND_LDS = MTILE * STAGE_BYTES;  // 32 KiB per operand and stage
 
-// Cooperative 16-byte loads of one 128-row x 128-byte stage of an operand into
-// registers (4 chunks per thread), then into LDS.
-struct StageRegs {
-  i32x4 v[4];
-};
-
-__device__ __forceinline__ void load_stage(StageRegs& r, const uint8_t* __restrict__ base,
-                                           int64_t row_bytes, int row0, int nrows, int rstride,
-                                           int kbyte) {
-  const int t = threadIdx.x;
+// Stage loads go straight from global memory into LDS (global_load_lds_dwordx4:
+// lane i of a wave writes 16 bytes at M0 + 16 * i), 8 rows of 128 bytes per
+// instruction.  The 16-byte chunks of a row are XOR-swizzled by (row & 7) so the
+// MFMA fragment reads (8 consecutive rows, same chunk) hit 8 different bank groups.
+// Rows past the end read the last valid row; the epilogues mask them.
+__device__ __forceinline__ void dma_stage(uint8_t* lds_operand, const uint8_t* __restrict__ base,
+                                          int64_t row_bytes, int row0, int nrows, int rstride, int kbyte) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rl = lane >> 3;                 // row within the 8-row group
+  const int chunk = (lane & 7) ^ rl;        // logical chunk stored at physical slot lane & 7
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    int id = t + 256 * i;
-    int row = id >> 3, c16 = id & 7;
-    int g = row0 + row;
-    if (g < nrows) {
-      const uint8_t* p = base + (int64_t)g * rstride * row_bytes + kbyte + c16 * 16;
-      r.v[i] = *(const i32x4*)p;
-    } else {
-      r.v[i] = i32x4{0, 0, 0, 0};
-    }
+  for (int i = 0; i < MTILE / 8 / 4; i++) {
+    const int grp = wave + 4 * i;
+    int g = row0 + grp * 8 + rl;
+    g = g < nrows ? g : nrows - 1;
+    const uint8_t* src = base + (int64_t)g * rstride * row_bytes + kbyte + chunk * 16;
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)(lds_operand + grp * 1024), 16, 0,
+                                     0);
   }
 }
 
-__device__ __forceinline__ void store_stage(const StageRegs& r, uint8_t* lds) {
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    int id = t + 256 * i;
-    int row = id >> 3, c16 = id & 7;
-    *(i32x4*)(lds + row * ROW_PITCH + c16 * 16) = r.v[i];
-  }
-}
-
-// Accumulate the 128 x 128 score tile (candidates c0.., queries q0..) over all
-// K stages.  acc[bm][bn] are this wave's four 32x32 blocks.
+// Accumulate the 256 x 256 score tile (candidates c0.., queries q0..) over all
+// K stages.  acc[bm][bn] are this wave's sixteen 32x32 blocks.
 template <int KIND>
-__device__ __forceinline__ void score_tile(typename Mfma<KIND>::acc_t (&acc)[2][2], uint8_t* lds,
+__device__ __forceinline__ void score_tile(typename Mfma<KIND>::acc_t (&acc)[WB][WB], uint8_t* lds,
                                            const TileArgs& a, int c0, int q0) {
   using M = Mfma<KIND>;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < WB; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = M::zero();
+    for (int j = 0; j < WB; j++) acc[i][j] = M::zero();
 
   const int nst = (int)(a.row_bytes / STAGE_BYTES);
-  StageRegs ra, rb;
-  load_stage(ra, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, 0);
-  load_stage(rb, a.q_rows, a.row_bytes, q0, a.n_q, 1, 0);
-  store_stage(ra, lds);
-  store_stage(rb, lds + OPERAND_LDS);
+  // fragment rows of this lane; (row & 7) == (lane & 7) for every block
+  const int arow = (wm * 128 + (lane & 31)) * STAGE_BYTES;
+  const int brow = (wn * 128 + (lane & 31)) * STAGE_BYTES;
+  const int sw = lane & 7;
+  dma_stage(lds, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, 0);
+  dma_stage(lds + OPERAND_LDS, a.q_rows, a.row_bytes, q0, a.n_q, 1, 0);
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   for (int s = 0; s < nst; s++) {
-    uint8_t* cur = lds + (s & 1) * 2 * OPERAND_LDS;
-    if (s + 1 < nst) {
-      load_stage(ra, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, (s + 1) * STAGE_BYTES);
-      load_stage(rb, a.q_rows, a.row_bytes, q0, a.n_q, 1, (s + 1) * STAGE_BYTES);
-    }
-    const uint8_t* A = cur + (wm * 64 + (lane & 31)) * ROW_PITCH + (lane >> 5) * M::LANE_BYTES;
-    const uint8_t* B =
-        cur + OPERAND_LDS + (wn * 64 + (lane & 31)) * ROW_PITCH + (lane >> 5) * M::LANE_BYTES;
-#pragma unroll
-    for (int ks = 0; ks < STAGE_BYTES / M::KSTEP_BYTES; ks++) {
-      const int kb = ks * M::KSTEP_BYTES;
-      M::step(acc[0][0], A + kb, B + kb);
-      M::step(acc[0][1], A + kb, B + 32 * ROW_PITCH + kb);
-      M::step(acc[1][0], A + 32 * ROW_PITCH + kb, B + kb);
-      M::step(acc[1][1], A + 32 * ROW_PITCH + kb, B + 32 * ROW_PITCH + kb);
-    }
+    const uint8_t* cur = lds + (s & 1) * 2 * OPERAND_LDS;
     if (s + 1 < nst) {
       uint8_t* nxt = lds + ((s + 1) & 1) * 2 * OPERAND_LDS;
-      store_stage(ra, nxt);
-      store_stage(rb, nxt + OPERAND_LDS);
+      dma_stage(nxt, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, (s + 1) * STAGE_BYTES);
+      dma_stage(nxt + OPERAND_LDS, a.q_rows, a.row_bytes, q0, a.n_q, 1, (s + 1) * STAGE_BYTES);
     }
+    // fragments of K step ks + 1 are read from LDS while the 16 MFMAs of step
+    // ks run (one wave per SIMD: nothing else hides the LDS latency)
+    constexpr int KS = STAGE_BYTES / M::KSTEP_BYTES;
+    typename M::frag_t fa[2][WB], fb[2][WB];
+    {
+      const int kb = (lane >> 5) * M::LANE_BYTES;
+      const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
+#pragma unroll
+      for (int b = 0; b < WB; b++) {
+        fa[0][b] = M::load(cur + arow + off + b * 32 * STAGE_BYTES);
+        fb[0][b] = M::load(cur + OPERAND_LDS + brow + off + b * 32 * STAGE_BYTES);
+      }
+    }
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      const int p = ks & 1;
+      if (ks + 1 < KS) {
+        const int kb = (ks + 1) * M::KSTEP_BYTES + (lane >> 5) * M::LANE_BYTES;
+        const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
+#pragma unroll
+        for (int b = 0; b < WB; b++) {
+          fa[p ^ 1][b] = M::load(cur + arow + off + b * 32 * STAGE_BYTES);
+          fb[p ^ 1][b] = M::load(cur + OPERAND_LDS + brow + off + b * 32 * STAGE_BYTES);
+        }
+      }
+      // the reads above stay ahead of these MFMAs: their wait lands at the next step
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int bm = 0; bm < WB; bm++)
+#pragma unroll
+        for (int bn = 0; bn < WB; bn++) M::mma(acc[bm][bn], fa[p][bm], fb[p][bn]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
   }
 }
@@ -125,39 +136,76 @@ __device__ __forceinline__ int query_block(const TileArgs& a) {
   return a.block_list ? a.block_list[blockIdx.x] : (int)blockIdx.x;
 }
 
+// raw accumulator value for the fast threshold test (no self / FLT_MIN rule)
+template <int KIND>
+__device__ __forceinline__ float raw_value(const typename Mfma<KIND>::acc_t& c, int j) {
+  return (float)c[j];
+}
+
+// A lower bound on the score test that never rejects what the exact test
+// accepts: f32 scores are dot > 0 ? dot : -FLT_MIN, so for a bound above
+// -FLT_MIN the raw dot must reach it; below that everything may pass.
+template <int KIND>
+__device__ __forceinline__ float fast_bound(float tau) {
+  if (KIND == EVOC_ELEM_I8) return tau;
+  return tau > -1.17549435e-38f ? tau : -INFINITY;
+}
+
 // ---------------------------------------------------------------------------
-// Pass 1: group maxima over a strided candidate sample.
+// Pass 1: group maxima over a strided candidate sample (16 groups per tile:
+// wave row half x 4 blocks x lane half).
 // ---------------------------------------------------------------------------
 template <int KIND>
 __global__ __launch_bounds__(256) void pass1_groupmax_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   using M = Mfma<KIND>;
-  typename M::acc_t acc[2][2];
+  typename M::acc_t acc[WB][WB];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int qb = query_block(a);
-  const int q0 = qb * TILE;
+  const int q0 = qb * MTILE;
   for (int t = 0; t < a.n_ctiles; t++) {
-    const int c0 = t * TILE;
+    const int c0 = t * MTILE;
     score_tile<KIND>(acc, lds, a, c0, q0);
 #pragma unroll
-    for (int bm = 0; bm < 2; bm++) {
+    for (int bm = 0; bm < WB; bm++) {
 #pragma unroll
-      for (int bn = 0; bn < 2; bn++) {
-        const int q = q0 + wn * 64 + bn * 32 + (lane & 31);
+      for (int bn = 0; bn < WB; bn++) {
+        const int q = q0 + wn * 128 + bn * 32 + (lane & 31);
         float m = -INFINITY;
 #pragma unroll
         for (int j = 0; j < 16; j++) {
-          const int c = c0 + wm * 64 + bm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+          const int c = c0 + wm * 128 + bm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
           const bool self = (int64_t)c * a.c_stride == (int64_t)q + a.q_offset;
           if (c < a.n_c && !(self && a.skip_self)) {
             const float s = self ? 1.0f : M::value(acc[bm][bn], j);
             m = fmaxf(m, s);
           }
         }
-        const int g = ((t * 2 + wm) * 2 + bm) * 2 + (lane >> 5);
+        const int g = ((t * 2 + wm) * WB + bm) * 2 + (lane >> 5);
         if (q < a.n_q) a.gmax[(int64_t)q * a.ng + g] = m;
       }
+    }
+  }
+}
+
+// Exact test and append for one 32x32 block whose fast test fired (rare): kept
+// out of line so the unrolled epilogue stays small.  s[j] holds the score of
+// candidate cb + 8*(j/4) + 4*(lane/32) + j%4 for query q0 + ql.
+__device__ __attribute__((noinline)) void append_block(const TileArgs& a, f32x16 s, int cb, int q0, int ql,
+                                                       float tau, uint64_t tkey, int split, int* cnt) {
+  const int lane = threadIdx.x & 63;
+  const int q = q0 + ql;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const int c = cb + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+    const bool self = (c == q + a.q_offset);
+    const float sc = self ? 1.0f : s[j];
+    bool take = (c < a.n_c) && (q < a.n_q) && (sc >= tau) && !(self && a.skip_self);
+    if (take && a.tau_key) take = make_key(sc, c, self) >= tkey;
+    if (take) {
+      const int pos = atomicAdd(&cnt[ql], 1);
+      if (pos < a.caps) a.bufs[((int64_t)split * a.n_q + q) * a.caps + pos] = make_key(sc, c, self);
     }
   }
 }
@@ -170,50 +218,49 @@ __global__ __launch_bounds__(256) void pass2_append_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   using M = Mfma<KIND>;
   int* cnt = (int*)(lds + 4 * OPERAND_LDS);
-  typename M::acc_t acc[2][2];
+  typename M::acc_t acc[WB][WB];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int qb = query_block(a);
-  const int q0 = qb * TILE;
+  const int q0 = qb * MTILE;
   const int split = blockIdx.y;
-  if (threadIdx.x < TILE) cnt[threadIdx.x] = 0;
-  float tau[2];
-  uint64_t tkey[2];
+  cnt[threadIdx.x] = 0;  // 256 threads, 256 queries
+  float tau[WB], tfast[WB];
+  uint64_t tkey[WB];
 #pragma unroll
-  for (int bn = 0; bn < 2; bn++) {
-    const int q = q0 + wn * 64 + bn * 32 + (lane & 31);
+  for (int bn = 0; bn < WB; bn++) {
+    const int q = q0 + wn * 128 + bn * 32 + (lane & 31);
     tau[bn] = (q < a.n_q && a.tau) ? a.tau[q] : -INFINITY;
     tkey[bn] = (q < a.n_q && a.tau_key) ? a.tau_key[q] : 0ull;
+    float tb = tau[bn];
+    if (a.tau_key && tkey[bn] != EVOC_KEY_NONE) tb = fmaxf(tb, key_score(tkey[bn]));
+    tfast[bn] = fast_bound<KIND>(tb);
   }
   const int t_begin = split * a.tiles_per_split;
   const int t_end = min(a.n_ctiles, t_begin + a.tiles_per_split);
   for (int t = t_begin; t < t_end; t++) {
-    const int c0 = t * TILE;
+    const int c0 = t * MTILE;
     score_tile<KIND>(acc, lds, a, c0, q0);
 #pragma unroll
-    for (int bm = 0; bm < 2; bm++) {
+    for (int bm = 0; bm < WB; bm++) {
 #pragma unroll
-      for (int bn = 0; bn < 2; bn++) {
-        const int ql = wn * 64 + bn * 32 + (lane & 31);
-        const int q = q0 + ql;
+      for (int bn = 0; bn < WB; bn++) {
+        const int cb = c0 + wm * 128 + bm * 32;            // first candidate of the block
+        const int qbk = q0 + wn * 128 + bn * 32;           // first query of the block
+        const bool diag = cb < qbk + a.q_offset + 32 && qbk + a.q_offset < cb + 32;
+        bool hit = diag;
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-          const int c = c0 + wm * 64 + bm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-          const bool self = (c == q + a.q_offset);
-          const float s = self ? 1.0f : M::value(acc[bm][bn], j);
-          bool take = (c < a.n_c) && (q < a.n_q) && (s >= tau[bn]) && !(self && a.skip_self);
-          if (take && a.tau_key) take = make_key(s, c, self) >= tkey[bn];
-          if (take) {
-            int pos = atomicAdd(&cnt[ql], 1);
-            if (pos < a.caps)
-              a.bufs[((int64_t)split * a.n_q + q) * a.caps + pos] = make_key(s, c, self);
-          }
-        }
+        for (int j = 0; j < 16; j++) hit |= raw_value<KIND>(acc[bm][bn], j) >= tfast[bn];
+        if (!__any(hit)) continue;
+        f32x16 sv;
+#pragma unroll
+        for (int j = 0; j < 16; j++) sv[j] = M::value(acc[bm][bn], j);
+        append_block(a, sv, cb, q0, wn * 128 + bn * 32 + (lane & 31), tau[bn], tkey[bn], split, cnt);
       }
     }
   }
   __syncthreads();
-  if (threadIdx.x < TILE) {
+  {
     const int q = q0 + threadIdx.x;
     if (q < a.n_q) a.counts[(int64_t)split * a.n_q + q] = cnt[threadIdx.x];
   }
@@ -224,6 +271,7 @@ __global__ __launch_bounds__(256) void pass2_append_kernel(TileArgs a) {
 // owning 8 candidates x 8 queries; popcount of AND per 32-bit word, union from
 // the per-row popcounts.  Same two epilogues as the MFMA kernels.
 // ---------------------------------------------------------------------------
+constexpr int TILE = 128;                   // Jaccard tile rows (both operands)
 constexpr int JAC_PITCH = STAGE_BYTES + 4;  // LDS row pitch in bytes (word-padded)
 
 __device__ __forceinline__ void jac_tile(uint32_t (&inter)[8][8], uint8_t* lds, const TileArgs& a,
@@ -519,10 +567,13 @@ __global__ void finalize_kernel(const uint64_t* __restrict__ keys, int n_q, int 
 // ---------------------------------------------------------------------------
 namespace evoc {
 
+int tile_rows(int kind) { return kind == EVOC_ELEM_U8BITS ? TILE : MTILE; }
+
+int groups_per_tile(int kind) { return kind == EVOC_ELEM_U8BITS ? 16 : 2 * WB * 2; }
+
 size_t tile_lds_bytes(int kind, int pass) {
-  const size_t counters = pass == 2 ? TILE * sizeof(int) : 0;
-  if (kind == EVOC_ELEM_U8BITS) return 2 * TILE * JAC_PITCH + counters;
-  return 4 * OPERAND_LDS + counters;
+  if (kind == EVOC_ELEM_U8BITS) return 2 * TILE * JAC_PITCH + (pass == 2 ? TILE * sizeof(int) : 0);
+  return 4 * OPERAND_LDS + (pass == 2 ? MTILE * sizeof(int) : 0);
 }
 
 template <class Kern>

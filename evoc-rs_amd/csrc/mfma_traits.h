@@ -26,6 +26,12 @@ template <> struct Mfma<EVOC_ELEM_BF16> {
   static constexpr int KSTEP_BYTES = 32;  // 16 bf16 per MFMA K step
   static constexpr int LANE_BYTES = 16;   // 8 bf16 per lane
   typedef f32x16 acc_t;
+  typedef i32x4 frag_t;
+  __device__ static frag_t load(const uint8_t* p) { return *(const i32x4*)p; }
+  __device__ static void mma(acc_t& c, frag_t a, frag_t b) {
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                0, 0, 0);
+  }
   __device__ static acc_t zero() { return acc_t{}; }
   __device__ static void step(acc_t& c, const uint8_t* a, const uint8_t* b) {
     bf16x8 av = __builtin_bit_cast(bf16x8, *(const i32x4*)a);
@@ -39,6 +45,11 @@ template <> struct Mfma<EVOC_ELEM_F32> {
   static constexpr int KSTEP_BYTES = 8;  // 2 f32 per MFMA K step
   static constexpr int LANE_BYTES = 4;
   typedef f32x16 acc_t;
+  typedef float frag_t;
+  __device__ static frag_t load(const uint8_t* p) { return *(const float*)p; }
+  __device__ static void mma(acc_t& c, frag_t a, frag_t b) {
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
   __device__ static acc_t zero() { return acc_t{}; }
   __device__ static void step(acc_t& c, const uint8_t* a, const uint8_t* b) {
     c = __builtin_amdgcn_mfma_f32_32x32x2f32(*(const float*)a, *(const float*)b, c, 0, 0, 0);
@@ -50,6 +61,11 @@ template <> struct Mfma<EVOC_ELEM_I8> {
   static constexpr int KSTEP_BYTES = 32;  // 32 i8 per MFMA K step
   static constexpr int LANE_BYTES = 16;
   typedef i32x16 acc_t;
+  typedef i32x4 frag_t;
+  __device__ static frag_t load(const uint8_t* p) { return *(const i32x4*)p; }
+  __device__ static void mma(acc_t& c, frag_t a, frag_t b) {
+    c = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+  }
   __device__ static acc_t zero() { return acc_t{}; }
   __device__ static void step(acc_t& c, const uint8_t* a, const uint8_t* b) {
     c = __builtin_amdgcn_mfma_i32_32x32x32_i8(*(const i32x4*)a, *(const i32x4*)b, c, 0, 0, 0);

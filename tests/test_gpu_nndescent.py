@@ -61,7 +61,8 @@ def test_f32_recall_at_equal_iterations(ctx, prec_name, prec):
     r_ref = recall(reference_nnd(x, k, iters), truth)
     print(f"[gpu] nnd f32 {prec_name}: recall gpu={r_gpu:.5f} reference={r_ref:.5f} "
           f"iters={st.iterations} dropped={st.offers_dropped}", flush=True)
-    assert st.iterations == iters
+    # the reference also stops early once an iteration changes nothing (float.rs:847)
+    assert 1 <= st.iterations <= iters
     assert r_gpu >= r_ref - 0.002
     # every reported distance is the FP32 transform of the listed pair
     xs = O.prepare_float(x)
@@ -76,7 +77,9 @@ def test_int8_recall_at_equal_iterations(ctx):
     n, d, k, iters = 5000, 128, 15, 6
     x = synth.blobs_i8(n, d, 25, 0.25, seed=6)
     idx, dist, st = nnd(ctx, x, k, iters)
-    assert (np.diff(dist, axis=1) >= 0).all() or True  # int8 distances are 1/dot, ascending
+    # int8 distances are 1/dot for positive dots and 0 otherwise (src/knn.rs:173-176), so they
+    # are not monotone over a whole row; check the indices only
+    assert (idx >= 0).all() and all(len(set(r.tolist())) == k for r in idx)
     truth, _ = O.exact_knn(x, k, include_self=False, workers=8)
     r_gpu = recall(idx, truth)
     r_ref = recall(reference_nnd(x, k, iters), truth)
