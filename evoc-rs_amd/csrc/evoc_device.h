@@ -58,8 +58,13 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
 // i = e*64 + lane lives in register e of lane `lane`.  Result is DESCENDING
 // (element 0 = largest).  Distances j < 64 exchange across lanes, j >= 64
 // swap registers within a lane.  Fully unrolled for compile-time E.
-template <int E>
-__device__ __forceinline__ void wave_sort_desc(uint64_t (&v)[E]) {
+__device__ __forceinline__ uint32_t shfl_xor_key(uint32_t v, int m) {
+  return (uint32_t)__shfl_xor((int)v, m, EVOC_WAVE);
+}
+__device__ __forceinline__ uint64_t shfl_xor_key(uint64_t v, int m) { return shfl_xor_u64(v, m); }
+
+template <int E, typename T>
+__device__ __forceinline__ void wave_sort_desc(T (&v)[E]) {
   const int lane = threadIdx.x & (EVOC_WAVE - 1);
   constexpr int N = EVOC_WAVE * E;
 #pragma unroll
@@ -75,7 +80,7 @@ __device__ __forceinline__ void wave_sort_desc(uint64_t (&v)[E]) {
             const int i = e * EVOC_WAVE + lane;  // the lower element of the pair
             // segments with (i & k) == 0 sort descending, the others ascending
             const bool desc = (i & k) == 0;
-            const uint64_t a = v[e], b = v[p];
+            const T a = v[e], b = v[p];
             const bool sw = desc ? (a < b) : (a > b);
             v[e] = sw ? b : a;
             v[p] = sw ? a : b;
@@ -85,7 +90,7 @@ __device__ __forceinline__ void wave_sort_desc(uint64_t (&v)[E]) {
 #pragma unroll
         for (int e = 0; e < E; e++) {
           const int i = e * EVOC_WAVE + lane;
-          const uint64_t o = shfl_xor_u64(v[e], j);
+          const T o = shfl_xor_key(v[e], j);
           const bool lower = (i & j) == 0;
           const bool desc = (i & k) == 0;
           const bool keep_max = (lower == desc);
@@ -107,6 +112,14 @@ __device__ __forceinline__ uint64_t wave_elem(const uint64_t (&v)[E], int i) {
   const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)mine, src, EVOC_WAVE);
   const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(mine >> 32), src, EVOC_WAVE);
   return ((uint64_t)hi << 32) | lo;
+}
+template <int E>
+__device__ __forceinline__ uint32_t wave_elem(const uint32_t (&v)[E], int i) {
+  uint32_t mine = 0;
+#pragma unroll
+  for (int e = 0; e < E; e++)
+    if (e == i / EVOC_WAVE) mine = v[e];
+  return (uint32_t)__shfl((int)mine, i % EVOC_WAVE, EVOC_WAVE);
 }
 
 // Sum over the wave with a fixed butterfly order (deterministic).

@@ -56,7 +56,10 @@ struct SelectArgs {
 };
 
 // ---- launchers (knn_exact.hip) ----
-int tile_rows(int kind);          // query / candidate rows per tile of the engine
+int tile_shape(int kind);         // MFMA tile shape in use (-1 for binary rows)
+void set_tile_shape(int shape);   // force a shape (-1: default)
+int tile_rows_c(int kind);        // candidate rows per tile
+int tile_rows_q(int kind);        // query rows per tile
 int groups_per_tile(int kind);    // pass-1 group maxima per candidate tile
 size_t tile_lds_bytes(int kind, int pass);
 hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, hipStream_t s);

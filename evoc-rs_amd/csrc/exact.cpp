@@ -41,11 +41,12 @@ struct Plan {
 
 Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   Plan p;
-  const int TILE = tile_rows(kind);
-  const int64_t n_ctiles = cdiv(n, TILE);
-  const int64_t nqb = cdiv(nq, TILE);
-  // workgroups per CU at once: 1 for the 256-row MFMA tiles (LDS), 2 for binary
-  const int64_t slots = (int64_t)n_cu * (kind == EVOC_ELEM_U8BITS ? 2 : 1);
+  const int TC = tile_rows_c(kind), TQ = tile_rows_q(kind);
+  const int64_t n_ctiles = cdiv(n, TC);
+  const int64_t nqb = cdiv(nq, TQ);
+  // workgroups resident per CU: LDS-limited (64 KiB tiles fit twice in 160 KiB)
+  const int64_t per_cu = tile_lds_bytes(kind, 2) <= 80 * 1024 ? 2 : 1;
+  const int64_t slots = (int64_t)n_cu * per_cu;
   // candidate splits: the count whose workgroups fill the last dispatch round
   // best (fraction of busy slots), fewer splits on ties
   int64_t best_ns = 1;
@@ -67,7 +68,7 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   if (n <= MAX_SORT) {
     // every candidate fits one sorted list: no threshold pass
     p.pass1 = false;
-    p.caps = p.tiles_per_split * TILE;
+    p.caps = p.tiles_per_split * TC;
     p.max_entries = (int)std::min<int64_t>(n, (int64_t)p.n_splits * p.caps);
     return p;
   }
@@ -76,7 +77,7 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   int64_t stride = std::min<int64_t>(std::max<int64_t>(cdiv(n, 16384), 1), 16);
   for (;;) {
     const int64_t nsamp = cdiv(n, stride);
-    const int64_t ng = cdiv(nsamp, TILE) * gpt;
+    const int64_t ng = cdiv(nsamp, TC) * gpt;
     if (ng <= MAX_SORT || stride >= n) {
       p.stride = (int)stride;
       p.n_sample = (int)nsamp;
@@ -101,7 +102,7 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
 int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, const uint32_t* pop,
                  int64_t n, int K, bool seed_self, uint64_t* keys, ExactStats* stats) {
   hipStream_t s = ctx->stream;
-  const int TILE = tile_rows(kind);
+  const int TC = tile_rows_c(kind), TQ = tile_rows_q(kind);
   if (K < 1 || K > MAX_SORT / 2 || K > n)
     return ctx->fail(EVOC_ERR_UNSUPPORTED, "exact search needs 1 <= K <= min(n, 1024)");
   // slab of queries whose scratch fits a quarter of the remaining budget
@@ -109,17 +110,17 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
   int64_t slab = n;
   for (;;) {
     Plan p = make_plan(kind, n, slab, K, ctx->n_cu);
-    if (p.scratch_bytes(slab, K) <= avail / 4 || slab <= 4 * TILE) break;
-    slab = std::max<int64_t>(4 * TILE, cdiv(slab / 2, TILE) * TILE);
+    if (p.scratch_bytes(slab, K) <= avail / 4 || slab <= 4 * TQ) break;
+    slab = std::max<int64_t>(4 * TQ, cdiv(slab / 2, TQ) * TQ);
   }
-  const int64_t n_ctiles = cdiv(n, TILE);
+  const int64_t n_ctiles = cdiv(n, TC);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   int rc = EVOC_OK;
   for (int64_t q0 = 0; q0 < n && rc == EVOC_OK; q0 += slab) {
     const int64_t nq = std::min(slab, n - q0);
-    const int64_t nqb = cdiv(nq, TILE);
+    const int64_t nqb = cdiv(nq, TQ);
     const Plan p = make_plan(kind, n, nq, K, ctx->n_cu);
     DevBuf gmax, tau, bufs, counts, ovf, tkey, list;
     if (p.pass1) {
@@ -145,7 +146,7 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       TileArgs a1 = a;
       a1.n_c = p.n_sample;
       a1.c_stride = p.stride;
-      a1.n_ctiles = (int32_t)cdiv(p.n_sample, TILE);
+      a1.n_ctiles = (int32_t)cdiv(p.n_sample, TC);
       a1.gmax = (float*)gmax.ptr;
       a1.ng = p.ng;
       e = launch_pass1(kind, a1, (int)nqb, s);
@@ -200,7 +201,7 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       std::vector<int32_t> blocks;
       for (int64_t b = 0; b < nqb; b++) {
         bool any = false;
-        for (int64_t q = b * TILE; q < std::min<int64_t>(nq, (b + 1) * TILE); q++) any |= flags[(size_t)q] != 0;
+        for (int64_t q = b * TQ; q < std::min<int64_t>(nq, (b + 1) * TQ); q++) any |= flags[(size_t)q] != 0;
         if (any) blocks.push_back((int32_t)b);
       }
       if (blocks.empty()) break;
@@ -212,7 +213,7 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       if (stats) stats->extra_rounds++;
       std::vector<int32_t> qlist;
       for (int32_t b : blocks)
-        for (int64_t q = (int64_t)b * TILE; q < std::min<int64_t>(nq, ((int64_t)b + 1) * TILE); q++)
+        for (int64_t q = (int64_t)b * TQ; q < std::min<int64_t>(nq, ((int64_t)b + 1) * TQ); q++)
           qlist.push_back((int32_t)q);
       int32_t* d_blocks = (int32_t*)list.ptr;
       int32_t* d_q = d_blocks + nqb;

@@ -28,31 +28,32 @@
 namespace evoc {
 
 // ---------------------------------------------------------------------------
-// MFMA tile geometry.  256 threads = 4 waves in a 2 (candidates) x 2 (queries)
-// arrangement over a 256 x 256 tile; each wave owns 128 x 128 = 4 x 4 MFMA
-// 32x32 blocks (256 f32/i32 accumulators per lane), which halves the LDS bytes
-// read per MFMA against 64 x 64 wave tiles.
+// MFMA tile engine.  256 threads = 4 waves in a 2 (candidates) x 2 (queries)
+// arrangement; each wave owns BM x BN MFMA 32x32 blocks, so a workgroup tile is
+// (64 BM) candidates x (64 BN) queries.  Shapes (see tile_shape()):
+//   <2,2>  128 x 128, 64 accumulators per lane, LDS 64 KiB: two workgroups per CU
+//          so one workgroup's epilogue overlaps the other's MFMAs
+//   <2,4>  128 x 256, 128 accumulators, LDS 96 KiB
+//   <4,4>  256 x 256, 256 accumulators, LDS 128 KiB: least LDS traffic per MFMA
 // MFMA orientation: M (rows) = candidates (A operand), N (cols) = queries
 // (B operand), so lane l of a 32x32 block holds query column l % 32 and the 16
 // candidate rows 8*(j/4) + 4*(l/32) + j%4, j = 0..15.
 // ---------------------------------------------------------------------------
-constexpr int MTILE = 256;           // rows per tile (both operands)
-constexpr int WB = 4;                // 32-row blocks per wave and operand
 constexpr int STAGE_BYTES = 128;     // bytes of each row staged per K step
-constexpr int OPERAND_LDS = MTILE * STAGE_BYTES;  // 32 KiB per operand and stage
 
 // Stage loads go straight from global memory into LDS (global_load_lds_dwordx4:
 // lane i of a wave writes 16 bytes at M0 + 16 * i), 8 rows of 128 bytes per
 // instruction.  The 16-byte chunks of a row are XOR-swizzled by (row & 7) so the
 // MFMA fragment reads (8 consecutive rows, same chunk) hit 8 different bank groups.
 // Rows past the end read the last valid row; the epilogues mask them.
+template <int ROWS>
 __device__ __forceinline__ void dma_stage(uint8_t* lds_operand, const uint8_t* __restrict__ base,
                                           int64_t row_bytes, int row0, int nrows, int rstride, int kbyte) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rl = lane >> 3;                 // row within the 8-row group
   const int chunk = (lane & 7) ^ rl;        // logical chunk stored at physical slot lane & 7
 #pragma unroll
-  for (int i = 0; i < MTILE / 8 / 4; i++) {
+  for (int i = 0; i < ROWS / 8 / 4; i++) {
     const int grp = wave + 4 * i;
     int g = row0 + grp * 8 + rl;
     g = g < nrows ? g : nrows - 1;
@@ -63,83 +64,81 @@ __device__ __forceinline__ void dma_stage(uint8_t* lds_operand, const uint8_t* _
   }
 }
 
-// Accumulate the 256 x 256 score tile (candidates c0.., queries q0..) over all
-// K stages.  acc[bm][bn] are this wave's sixteen 32x32 blocks.
-template <int KIND>
-__device__ __forceinline__ void score_tile(typename Mfma<KIND>::acc_t (&acc)[WB][WB], uint8_t* lds,
-                                           const TileArgs& a, int c0, int q0) {
+template <int KIND, int BM, int BN>
+struct Engine {
   using M = Mfma<KIND>;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-#pragma unroll
-  for (int i = 0; i < WB; i++)
-#pragma unroll
-    for (int j = 0; j < WB; j++) acc[i][j] = M::zero();
+  typedef typename M::acc_t acc_t;
+  static constexpr int TC = 64 * BM;                    // candidate rows per tile
+  static constexpr int TQ = 64 * BN;                    // query rows per tile
+  static constexpr int STAGE_LDS = (TC + TQ) * STAGE_BYTES;
+  static constexpr int LDS = 2 * STAGE_LDS;             // double buffered
+  static constexpr int GROUPS = 2 * BM * 2;             // pass-1 groups per tile
 
-  const int nst = (int)(a.row_bytes / STAGE_BYTES);
-  // fragment rows of this lane; (row & 7) == (lane & 7) for every block
-  const int arow = (wm * 128 + (lane & 31)) * STAGE_BYTES;
-  const int brow = (wn * 128 + (lane & 31)) * STAGE_BYTES;
-  const int sw = lane & 7;
-  dma_stage(lds, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, 0);
-  dma_stage(lds + OPERAND_LDS, a.q_rows, a.row_bytes, q0, a.n_q, 1, 0);
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  for (int s = 0; s < nst; s++) {
-    const uint8_t* cur = lds + (s & 1) * 2 * OPERAND_LDS;
-    if (s + 1 < nst) {
-      uint8_t* nxt = lds + ((s + 1) & 1) * 2 * OPERAND_LDS;
-      dma_stage(nxt, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, (s + 1) * STAGE_BYTES);
-      dma_stage(nxt + OPERAND_LDS, a.q_rows, a.row_bytes, q0, a.n_q, 1, (s + 1) * STAGE_BYTES);
-    }
-    // fragments of K step ks + 1 are read from LDS while the 16 MFMAs of step
-    // ks run (one wave per SIMD: nothing else hides the LDS latency)
-    constexpr int KS = STAGE_BYTES / M::KSTEP_BYTES;
-    typename M::frag_t fa[2][WB], fb[2][WB];
-    {
-      const int kb = (lane >> 5) * M::LANE_BYTES;
-      const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
+  // Accumulate the TC x TQ score tile (candidates c0.., queries q0..) over all
+  // K stages.
+  __device__ static void score_tile(acc_t (&acc)[BM][BN], uint8_t* lds, const TileArgs& a, int c0, int q0) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
 #pragma unroll
-      for (int b = 0; b < WB; b++) {
-        fa[0][b] = M::load(cur + arow + off + b * 32 * STAGE_BYTES);
-        fb[0][b] = M::load(cur + OPERAND_LDS + brow + off + b * 32 * STAGE_BYTES);
-      }
-    }
+    for (int i = 0; i < BM; i++)
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) {
-      const int p = ks & 1;
-      if (ks + 1 < KS) {
-        const int kb = (ks + 1) * M::KSTEP_BYTES + (lane >> 5) * M::LANE_BYTES;
-        const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
-#pragma unroll
-        for (int b = 0; b < WB; b++) {
-          fa[p ^ 1][b] = M::load(cur + arow + off + b * 32 * STAGE_BYTES);
-          fb[p ^ 1][b] = M::load(cur + OPERAND_LDS + brow + off + b * 32 * STAGE_BYTES);
-        }
-      }
-      // the reads above stay ahead of these MFMAs: their wait lands at the next step
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int bm = 0; bm < WB; bm++)
-#pragma unroll
-        for (int bn = 0; bn < WB; bn++) M::mma(acc[bm][bn], fa[p][bm], fb[p][bn]);
-      __builtin_amdgcn_sched_barrier(0);
-    }
+      for (int j = 0; j < BN; j++) acc[i][j] = M::zero();
+    const int nst = (int)(a.row_bytes / STAGE_BYTES);
+    // fragment rows of this lane; (row & 7) == (lane & 7) for every block
+    const int arow = (wm * 32 * BM + (lane & 31)) * STAGE_BYTES;
+    const int brow = TC * STAGE_BYTES + (wn * 32 * BN + (lane & 31)) * STAGE_BYTES;
+    const int sw = lane & 7;
+    dma_stage<TC>(lds, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, 0);
+    dma_stage<TQ>(lds + TC * STAGE_BYTES, a.q_rows, a.row_bytes, q0, a.n_q, 1, 0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    for (int s = 0; s < nst; s++) {
+      const uint8_t* cur = lds + (s & 1) * STAGE_LDS;
+      if (s + 1 < nst) {
+        uint8_t* nxt = lds + ((s + 1) & 1) * STAGE_LDS;
+        dma_stage<TC>(nxt, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, (s + 1) * STAGE_BYTES);
+        dma_stage<TQ>(nxt + TC * STAGE_BYTES, a.q_rows, a.row_bytes, q0, a.n_q, 1, (s + 1) * STAGE_BYTES);
+      }
+      // fragments of K step ks + 1 are read from LDS while the MFMAs of step ks run
+      constexpr int KS = STAGE_BYTES / M::KSTEP_BYTES;
+      typename M::frag_t fa[2][BM], fb[2][BN];
+      {
+        const int kb = (lane >> 5) * M::LANE_BYTES;
+        const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
+#pragma unroll
+        for (int b = 0; b < BM; b++) fa[0][b] = M::load(cur + arow + off + b * 32 * STAGE_BYTES);
+#pragma unroll
+        for (int b = 0; b < BN; b++) fb[0][b] = M::load(cur + brow + off + b * 32 * STAGE_BYTES);
+      }
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const int p = ks & 1;
+        if (ks + 1 < KS) {
+          const int kb = (ks + 1) * M::KSTEP_BYTES + (lane >> 5) * M::LANE_BYTES;
+          const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
+#pragma unroll
+          for (int b = 0; b < BM; b++) fa[p ^ 1][b] = M::load(cur + arow + off + b * 32 * STAGE_BYTES);
+#pragma unroll
+          for (int b = 0; b < BN; b++) fb[p ^ 1][b] = M::load(cur + brow + off + b * 32 * STAGE_BYTES);
+        }
+        // the reads above stay ahead of these MFMAs: their wait lands at the next step
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int bm = 0; bm < BM; bm++)
+#pragma unroll
+          for (int bn = 0; bn < BN; bn++) M::mma(acc[bm][bn], fa[p][bm], fb[p][bn]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    }
   }
-}
+};
 
 // Query block handled by this workgroup.
 __device__ __forceinline__ int query_block(const TileArgs& a) {
   return a.block_list ? a.block_list[blockIdx.x] : (int)blockIdx.x;
-}
-
-// raw accumulator value for the fast threshold test (no self / FLT_MIN rule)
-template <int KIND>
-__device__ __forceinline__ float raw_value(const typename Mfma<KIND>::acc_t& c, int j) {
-  return (float)c[j];
 }
 
 // A lower bound on the score test that never rejects what the exact test
@@ -152,84 +151,65 @@ __device__ __forceinline__ float fast_bound(float tau) {
 }
 
 // ---------------------------------------------------------------------------
-// Pass 1: group maxima over a strided candidate sample (16 groups per tile:
-// wave row half x 4 blocks x lane half).
+// Pass 1: group maxima over a strided candidate sample (GROUPS per tile:
+// wave row half x BM blocks x lane half, 16 candidates each).
 // ---------------------------------------------------------------------------
-template <int KIND>
+template <int KIND, int BM, int BN>
 __global__ __launch_bounds__(256) void pass1_groupmax_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
+  using En = Engine<KIND, BM, BN>;
   using M = Mfma<KIND>;
-  typename M::acc_t acc[WB][WB];
+  typename En::acc_t acc[BM][BN];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int qb = query_block(a);
-  const int q0 = qb * MTILE;
+  const int q0 = query_block(a) * En::TQ;
   for (int t = 0; t < a.n_ctiles; t++) {
-    const int c0 = t * MTILE;
-    score_tile<KIND>(acc, lds, a, c0, q0);
+    const int c0 = t * En::TC;
+    En::score_tile(acc, lds, a, c0, q0);
 #pragma unroll
-    for (int bm = 0; bm < WB; bm++) {
+    for (int bm = 0; bm < BM; bm++) {
 #pragma unroll
-      for (int bn = 0; bn < WB; bn++) {
-        const int q = q0 + wn * 128 + bn * 32 + (lane & 31);
+      for (int bn = 0; bn < BN; bn++) {
+        const int q = q0 + wn * 32 * BN + bn * 32 + (lane & 31);
         float m = -INFINITY;
 #pragma unroll
         for (int j = 0; j < 16; j++) {
-          const int c = c0 + wm * 128 + bm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+          const int c = c0 + wm * 32 * BM + bm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
           const bool self = (int64_t)c * a.c_stride == (int64_t)q + a.q_offset;
           if (c < a.n_c && !(self && a.skip_self)) {
             const float s = self ? 1.0f : M::value(acc[bm][bn], j);
             m = fmaxf(m, s);
           }
         }
-        const int g = ((t * 2 + wm) * WB + bm) * 2 + (lane >> 5);
+        const int g = ((t * 2 + wm) * BM + bm) * 2 + (lane >> 5);
         if (q < a.n_q) a.gmax[(int64_t)q * a.ng + g] = m;
       }
     }
   }
 }
 
-// Exact test and append for one 32x32 block whose fast test fired (rare): kept
-// out of line so the unrolled epilogue stays small.  s[j] holds the score of
-// candidate cb + 8*(j/4) + 4*(lane/32) + j%4 for query q0 + ql.
-__device__ __attribute__((noinline)) void append_block(const TileArgs& a, f32x16 s, int cb, int q0, int ql,
-                                                       float tau, uint64_t tkey, int split, int* cnt) {
-  const int lane = threadIdx.x & 63;
-  const int q = q0 + ql;
-#pragma unroll
-  for (int j = 0; j < 16; j++) {
-    const int c = cb + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-    const bool self = (c == q + a.q_offset);
-    const float sc = self ? 1.0f : s[j];
-    bool take = (c < a.n_c) && (q < a.n_q) && (sc >= tau) && !(self && a.skip_self);
-    if (take && a.tau_key) take = make_key(sc, c, self) >= tkey;
-    if (take) {
-      const int pos = atomicAdd(&cnt[ql], 1);
-      if (pos < a.caps) a.bufs[((int64_t)split * a.n_q + q) * a.caps + pos] = make_key(sc, c, self);
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
-// Pass 2: threshold filter + append.  blockIdx.y = candidate split.
+// Pass 2: threshold filter + append.  blockIdx.y = candidate split.  Each
+// accumulator value is tested against a cheap bound first; the exact test and
+// the append run only in lanes whose value passes (exec-masked, inline).
 // ---------------------------------------------------------------------------
-template <int KIND>
+template <int KIND, int BM, int BN>
 __global__ __launch_bounds__(256) void pass2_append_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
+  using En = Engine<KIND, BM, BN>;
   using M = Mfma<KIND>;
-  int* cnt = (int*)(lds + 4 * OPERAND_LDS);
-  typename M::acc_t acc[WB][WB];
+  int* cnt = (int*)(lds + En::LDS);
+  typename En::acc_t acc[BM][BN];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int qb = query_block(a);
-  const int q0 = qb * MTILE;
+  const int q0 = query_block(a) * En::TQ;
   const int split = blockIdx.y;
-  cnt[threadIdx.x] = 0;  // 256 threads, 256 queries
-  float tau[WB], tfast[WB];
-  uint64_t tkey[WB];
+  for (int i = threadIdx.x; i < En::TQ; i += 256) cnt[i] = 0;
+  float tau[BN], tfast[BN];
+  uint64_t tkey[BN];
 #pragma unroll
-  for (int bn = 0; bn < WB; bn++) {
-    const int q = q0 + wn * 128 + bn * 32 + (lane & 31);
+  for (int bn = 0; bn < BN; bn++) {
+    const int q = q0 + wn * 32 * BN + bn * 32 + (lane & 31);
     tau[bn] = (q < a.n_q && a.tau) ? a.tau[q] : -INFINITY;
     tkey[bn] = (q < a.n_q && a.tau_key) ? a.tau_key[q] : 0ull;
     float tb = tau[bn];
@@ -239,30 +219,39 @@ __global__ __launch_bounds__(256) void pass2_append_kernel(TileArgs a) {
   const int t_begin = split * a.tiles_per_split;
   const int t_end = min(a.n_ctiles, t_begin + a.tiles_per_split);
   for (int t = t_begin; t < t_end; t++) {
-    const int c0 = t * MTILE;
-    score_tile<KIND>(acc, lds, a, c0, q0);
+    const int c0 = t * En::TC;
+    En::score_tile(acc, lds, a, c0, q0);
 #pragma unroll
-    for (int bm = 0; bm < WB; bm++) {
+    for (int bm = 0; bm < BM; bm++) {
 #pragma unroll
-      for (int bn = 0; bn < WB; bn++) {
-        const int cb = c0 + wm * 128 + bm * 32;            // first candidate of the block
-        const int qbk = q0 + wn * 128 + bn * 32;           // first query of the block
-        const bool diag = cb < qbk + a.q_offset + 32 && qbk + a.q_offset < cb + 32;
-        bool hit = diag;
+      for (int bn = 0; bn < BN; bn++) {
+        const int cb = c0 + wm * 32 * BM + bm * 32;        // first candidate of the block
+        const int ql = wn * 32 * BN + bn * 32 + (lane & 31);
+        const int q = q0 + ql;
+        const int qg = q + a.q_offset;
+        const bool diag = cb <= qg + 31 - (lane & 31) && qg - (lane & 31) <= cb + 31;
 #pragma unroll
-        for (int j = 0; j < 16; j++) hit |= raw_value<KIND>(acc[bm][bn], j) >= tfast[bn];
-        if (!__any(hit)) continue;
-        f32x16 sv;
-#pragma unroll
-        for (int j = 0; j < 16; j++) sv[j] = M::value(acc[bm][bn], j);
-        append_block(a, sv, cb, q0, wn * 128 + bn * 32 + (lane & 31), tau[bn], tkey[bn], split, cnt);
+        for (int j = 0; j < 16; j++) {
+          const float raw = (float)acc[bm][bn][j];
+          if (diag || raw >= tfast[bn]) {
+            const int c = cb + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+            const bool self = (c == qg);
+            const float s = self ? 1.0f : M::value(acc[bm][bn], j);
+            bool take = (c < a.n_c) && (q < a.n_q) && (s >= tau[bn]) && !(self && a.skip_self);
+            if (take && a.tau_key) take = make_key(s, c, self) >= tkey[bn];
+            if (take) {
+              const int pos = atomicAdd(&cnt[ql], 1);
+              if (pos < a.caps) a.bufs[((int64_t)split * a.n_q + q) * a.caps + pos] = make_key(s, c, self);
+            }
+          }
+        }
       }
     }
   }
   __syncthreads();
-  {
-    const int q = q0 + threadIdx.x;
-    if (q < a.n_q) a.counts[(int64_t)split * a.n_q + q] = cnt[threadIdx.x];
+  for (int i = threadIdx.x; i < En::TQ; i += 256) {
+    const int q = q0 + i;
+    if (q < a.n_q) a.counts[(int64_t)split * a.n_q + q] = cnt[i];
   }
 }
 
@@ -416,16 +405,16 @@ __global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict
   const int lane = threadIdx.x & 63;
   const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (q >= n_q) return;
-  uint64_t v[E];
+  uint32_t v[E];
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int g = e * 64 + lane;
     const float m = g < ng ? gmax[(int64_t)q * ng + g] : -INFINITY;
-    v[e] = (uint64_t)ord_f32(m);
+    v[e] = ord_f32(m);
   }
   wave_sort_desc<E>(v);
-  const uint64_t kth = wave_elem<E>(v, K - 1);
-  if (lane == 0) tau[q] = (K <= ng) ? unord_f32((uint32_t)kth) : -INFINITY;
+  const uint32_t kth = wave_elem<E>(v, K - 1);
+  if (lane == 0) tau[q] = (K <= ng) ? unord_f32(kth) : -INFINITY;
 }
 
 // ---------------------------------------------------------------------------
@@ -434,30 +423,17 @@ __global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict
 // query and stores the K-th kept key as a tighter bound for the next round:
 // the K-th best of any subset is <= the true K-th best.
 // ---------------------------------------------------------------------------
+// gather the first 64*E buffered keys of query q, sort, write the K best
 template <int E>
-__global__ __launch_bounds__(256) void select_topk_kernel(SelectArgs a) {
+__device__ __forceinline__ void select_body(const SelectArgs& a, int q, bool overflow) {
   const int lane = threadIdx.x & 63;
-  const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
-  int q;
-  if (a.query_list) {
-    if (qi >= a.n_list) return;
-    q = a.query_list[qi];
-  } else {
-    q = qi;
-    if (q >= a.n_q) return;
-  }
   constexpr int CAP = 64 * E;
   uint64_t v[E];
 #pragma unroll
   for (int e = 0; e < E; e++) v[e] = EVOC_KEY_NONE;
   int filled = 0;
-  bool overflow = false;
   for (int s = 0; s < a.n_splits; s++) {
-    int c = a.counts[(int64_t)s * a.n_q + q];
-    if (c > a.caps) {
-      overflow = true;
-      c = a.caps;
-    }
+    const int c = min(a.counts[(int64_t)s * a.n_q + q], a.caps);
     const uint64_t* src = a.bufs + ((int64_t)s * a.n_q + q) * a.caps;
 #pragma unroll
     for (int e = 0; e < E; e++) {
@@ -480,6 +456,35 @@ __global__ __launch_bounds__(256) void select_topk_kernel(SelectArgs a) {
   if (lane == 0) a.tau_key_out[q] = kth;
 }
 
+// One wave per query; the sort width follows the query's own key count
+// (wave-uniform branch), capped at 64*EMAX keys.
+template <int EMAX>
+__global__ __launch_bounds__(256) void select_topk_kernel(SelectArgs a) {
+  const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
+  int q;
+  if (a.query_list) {
+    if (qi >= a.n_list) return;
+    q = a.query_list[qi];
+  } else {
+    q = qi;
+    if (q >= a.n_q) return;
+  }
+  int total = 0;
+  bool overflow = false;
+  for (int s = 0; s < a.n_splits; s++) {
+    const int c = a.counts[(int64_t)s * a.n_q + q];
+    overflow |= c > a.caps;
+    total += min(c, a.caps);
+  }
+  const int need = max(total, a.K);
+  if (need <= 64 || EMAX == 1) select_body<1>(a, q, overflow);
+  else if (need <= 128 || EMAX == 2) select_body<2>(a, q, overflow);
+  else if (need <= 256 || EMAX == 4) select_body<4>(a, q, overflow);
+  else if (need <= 512 || EMAX == 8) select_body<8>(a, q, overflow);
+  else if (need <= 1024 || EMAX == 16) select_body<16>(a, q, overflow);
+  else select_body<EMAX>(a, q, overflow);
+}
+
 // ---------------------------------------------------------------------------
 // FP32 re-score of the BF16 survivors with a fixed reduction order: lane l
 // accumulates elements l, l+64, ... with fmaf, then a butterfly sum.  The k
@@ -490,27 +495,51 @@ __global__ __launch_bounds__(256) void rescore_kernel(const float* __restrict__ 
                                                       int n_q, int q_offset,
                                                       const uint64_t* __restrict__ cand_keys, int Kc,
                                                       int k, uint64_t* __restrict__ out_keys) {
+  // four candidates at a time, one 16-lane group each; lane t of a group sums
+  // the float4 chunks t, t + 16, ... of the row with fmaf (fixed order), then a
+  // fixed xor-butterfly over the 16 lanes
   const int lane = threadIdx.x & 63;
+  const int grp = lane >> 4, t = lane & 15;
   const int ql = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (ql >= n_q) return;
   const int q = ql + q_offset;
   const float* xq = X + (int64_t)q * ld;
+  const int nv4 = d >> 2;  // full float4 chunks; ld is a multiple of 32 and rows are zero padded
+  const int tail = d & 3;
   uint64_t v[E];
 #pragma unroll
   for (int e = 0; e < E; e++) v[e] = EVOC_KEY_NONE;
-  for (int i = 0; i < Kc; i++) {
-    const uint64_t key = cand_keys[(int64_t)ql * Kc + i];
-    if (key == EVOC_KEY_NONE) continue;
-    const int c = key_index(key, q);
-    const bool self = (c == q);
-    float part = 0.0f;
+  for (int i0 = 0; i0 < Kc; i0 += 4) {
+    const int i = i0 + grp;
+    const uint64_t key = i < Kc ? cand_keys[(int64_t)ql * Kc + i] : EVOC_KEY_NONE;
+    const int c = key != EVOC_KEY_NONE ? key_index(key, q) : q;
     const float* xc = X + (int64_t)c * ld;
-    for (int t = lane; t < d; t += 64) part = fmaf(xq[t], xc[t], part);
-    const float dot = wave_sum_f32(part);
-    const uint64_t nk = make_key(self ? 1.0f : f32_score(dot), c, self);
+    float part = 0.0f;
+    for (int j = t; j < nv4; j += 16) {
+      const float4 a4 = *(const float4*)(xq + 4 * j);
+      const float4 b4 = *(const float4*)(xc + 4 * j);
+      part = fmaf(a4.x, b4.x, part);
+      part = fmaf(a4.y, b4.y, part);
+      part = fmaf(a4.z, b4.z, part);
+      part = fmaf(a4.w, b4.w, part);
+    }
+    if (tail && t == 0)
+      for (int r = 4 * nv4; r < d; r++) part = fmaf(xq[r], xc[r], part);
 #pragma unroll
-    for (int e = 0; e < E; e++)
-      if (e * 64 + lane == i) v[e] = nk;
+    for (int m = 8; m >= 1; m >>= 1) part += __shfl_xor(part, m, 64);
+    const bool self = (c == q);
+    const uint64_t nk = key == EVOC_KEY_NONE ? EVOC_KEY_NONE : make_key(self ? 1.0f : f32_score(part), c, self);
+    // candidate i0 + g lives in lanes 16g..16g+15; element i of the sort array
+    // sits in register i / 64 of lane i % 64
+#pragma unroll
+    for (int g2 = 0; g2 < 4; g2++) {
+      const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)nk, 16 * g2, 64);
+      const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(nk >> 32), 16 * g2, 64);
+      const int ii = i0 + g2;
+#pragma unroll
+      for (int e = 0; e < E; e++)
+        if (e * 64 + lane == ii && ii < Kc) v[e] = ((uint64_t)hi << 32) | lo;
+    }
   }
   wave_sort_desc<E>(v);
 #pragma unroll
@@ -567,13 +596,43 @@ __global__ void finalize_kernel(const uint64_t* __restrict__ keys, int n_q, int 
 // ---------------------------------------------------------------------------
 namespace evoc {
 
-int tile_rows(int kind) { return kind == EVOC_ELEM_U8BITS ? TILE : MTILE; }
+// MFMA tile shapes: 0 = <2,2>, 1 = <2,4>, 2 = <4,4> (see Engine)
+static int g_forced_shape = -2;
 
-int groups_per_tile(int kind) { return kind == EVOC_ELEM_U8BITS ? 16 : 2 * WB * 2; }
+int tile_shape(int kind) {
+  if (kind == EVOC_ELEM_U8BITS) return -1;
+  if (g_forced_shape == -2) {
+    const char* e = getenv("EVOC_MI355X_TILE");
+    g_forced_shape = e ? atoi(e) : -1;
+  }
+  if (g_forced_shape >= 0 && g_forced_shape <= 2) return g_forced_shape;
+  return 0;
+}
+
+void set_tile_shape(int shape) { g_forced_shape = shape; }
+
+static void shape_dims(int kind, int& tc, int& tq, int& gpt) {
+  if (kind == EVOC_ELEM_U8BITS) {
+    tc = tq = TILE;
+    gpt = 16;
+    return;
+  }
+  switch (tile_shape(kind)) {
+    case 1: tc = 128; tq = 256; gpt = 8; return;
+    case 2: tc = 256; tq = 256; gpt = 16; return;
+    default: tc = 128; tq = 128; gpt = 8; return;
+  }
+}
+
+int tile_rows_c(int kind) { int tc, tq, g; shape_dims(kind, tc, tq, g); return tc; }
+int tile_rows_q(int kind) { int tc, tq, g; shape_dims(kind, tc, tq, g); return tq; }
+int groups_per_tile(int kind) { int tc, tq, g; shape_dims(kind, tc, tq, g); return g; }
 
 size_t tile_lds_bytes(int kind, int pass) {
   if (kind == EVOC_ELEM_U8BITS) return 2 * TILE * JAC_PITCH + (pass == 2 ? TILE * sizeof(int) : 0);
-  return 4 * OPERAND_LDS + (pass == 2 ? MTILE * sizeof(int) : 0);
+  int tc, tq, g;
+  shape_dims(kind, tc, tq, g);
+  return (size_t)2 * (tc + tq) * STAGE_BYTES + (pass == 2 ? (size_t)tq * sizeof(int) : 0);
 }
 
 template <class Kern>
@@ -585,13 +644,31 @@ static hipError_t launch_tile(Kern kern, size_t lds, dim3 grid, hipStream_t s, c
   return hipGetLastError();
 }
 
+template <int KIND>
+static hipError_t launch_pass1_kind(const TileArgs& a, dim3 g, size_t lds, hipStream_t s) {
+  switch (tile_shape(KIND)) {
+    case 1: return launch_tile(pass1_groupmax_kernel<KIND, 2, 4>, lds, g, s, a);
+    case 2: return launch_tile(pass1_groupmax_kernel<KIND, 4, 4>, lds, g, s, a);
+    default: return launch_tile(pass1_groupmax_kernel<KIND, 2, 2>, lds, g, s, a);
+  }
+}
+
+template <int KIND>
+static hipError_t launch_pass2_kind(const TileArgs& a, dim3 g, size_t lds, hipStream_t s) {
+  switch (tile_shape(KIND)) {
+    case 1: return launch_tile(pass2_append_kernel<KIND, 2, 4>, lds, g, s, a);
+    case 2: return launch_tile(pass2_append_kernel<KIND, 4, 4>, lds, g, s, a);
+    default: return launch_tile(pass2_append_kernel<KIND, 2, 2>, lds, g, s, a);
+  }
+}
+
 hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, hipStream_t s) {
   const size_t lds = tile_lds_bytes(kind, 1);
   const dim3 g((unsigned)n_blocks);
   switch (kind) {
-    case EVOC_ELEM_BF16: return launch_tile(pass1_groupmax_kernel<EVOC_ELEM_BF16>, lds, g, s, a);
-    case EVOC_ELEM_F32: return launch_tile(pass1_groupmax_kernel<EVOC_ELEM_F32>, lds, g, s, a);
-    case EVOC_ELEM_I8: return launch_tile(pass1_groupmax_kernel<EVOC_ELEM_I8>, lds, g, s, a);
+    case EVOC_ELEM_BF16: return launch_pass1_kind<EVOC_ELEM_BF16>(a, g, lds, s);
+    case EVOC_ELEM_F32: return launch_pass1_kind<EVOC_ELEM_F32>(a, g, lds, s);
+    case EVOC_ELEM_I8: return launch_pass1_kind<EVOC_ELEM_I8>(a, g, lds, s);
     case EVOC_ELEM_U8BITS: return launch_tile(jac_pass1_kernel, lds, g, s, a);
   }
   return hipErrorInvalidValue;
@@ -601,9 +678,9 @@ hipError_t launch_pass2(int kind, const TileArgs& a, int n_blocks, int n_splits,
   const size_t lds = tile_lds_bytes(kind, 2);
   const dim3 g((unsigned)n_blocks, (unsigned)n_splits);
   switch (kind) {
-    case EVOC_ELEM_BF16: return launch_tile(pass2_append_kernel<EVOC_ELEM_BF16>, lds, g, s, a);
-    case EVOC_ELEM_F32: return launch_tile(pass2_append_kernel<EVOC_ELEM_F32>, lds, g, s, a);
-    case EVOC_ELEM_I8: return launch_tile(pass2_append_kernel<EVOC_ELEM_I8>, lds, g, s, a);
+    case EVOC_ELEM_BF16: return launch_pass2_kind<EVOC_ELEM_BF16>(a, g, lds, s);
+    case EVOC_ELEM_F32: return launch_pass2_kind<EVOC_ELEM_F32>(a, g, lds, s);
+    case EVOC_ELEM_I8: return launch_pass2_kind<EVOC_ELEM_I8>(a, g, lds, s);
     case EVOC_ELEM_U8BITS: return launch_tile(jac_pass2_kernel, lds, g, s, a);
   }
   return hipErrorInvalidValue;
@@ -632,16 +709,8 @@ hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int K, float* t
 hipError_t launch_select(const SelectArgs& a, int max_entries, hipStream_t s) {
   const int nq = a.query_list ? a.n_list : a.n_q;
   const dim3 g((unsigned)((nq + 3) / 4)), b(256);
-  int E = pow2_per_lane(max_entries < a.K ? a.K : max_entries);
-  if (E > 32) E = 32;
-  switch (E) {
-    case 1: hipLaunchKernelGGL(select_topk_kernel<1>, g, b, 0, s, a); break;
-    case 2: hipLaunchKernelGGL(select_topk_kernel<2>, g, b, 0, s, a); break;
-    case 4: hipLaunchKernelGGL(select_topk_kernel<4>, g, b, 0, s, a); break;
-    case 8: hipLaunchKernelGGL(select_topk_kernel<8>, g, b, 0, s, a); break;
-    case 16: hipLaunchKernelGGL(select_topk_kernel<16>, g, b, 0, s, a); break;
-    case 32: hipLaunchKernelGGL(select_topk_kernel<32>, g, b, 0, s, a); break;
-  }
+  (void)max_entries;
+  hipLaunchKernelGGL(select_topk_kernel<32>, g, b, 0, s, a);
   return hipGetLastError();
 }
 
