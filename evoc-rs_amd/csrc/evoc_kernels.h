@@ -62,7 +62,7 @@ int tile_rows_c(int kind);        // candidate rows per tile
 int tile_rows_q(int kind);        // query rows per tile
 int groups_per_tile(int kind);    // pass-1 group maxima per candidate tile
 size_t tile_lds_bytes(int kind, int pass);
-hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, hipStream_t s);
+hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);
 hipError_t launch_pass2(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);
 hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int K, float* tau, hipStream_t s);
 hipError_t launch_select(const SelectArgs& a, int max_entries, hipStream_t s);

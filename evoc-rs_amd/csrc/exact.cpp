@@ -26,10 +26,29 @@ int64_t pow2ceil(int64_t v) {
   return p;
 }
 
+// Tiles per candidate split: the split count whose workgroups fill the last
+// dispatch round best (busy fraction of the resident slots), fewer splits on ties.
+int best_split(int64_t nqb, int64_t n_ctiles, int64_t slots) {
+  int64_t best_tps = n_ctiles;
+  double best_eff = -1.0;
+  for (int64_t ns = 1; ns <= std::min<int64_t>(16, n_ctiles); ns++) {
+    const int64_t tps = cdiv(n_ctiles, ns);
+    const int64_t real_ns = cdiv(n_ctiles, tps);
+    const double rounds = (double)cdiv(nqb * real_ns, slots);
+    const double eff = (double)(nqb * n_ctiles) / (rounds * (double)tps * (double)slots);
+    if (eff > best_eff + 1e-3) {
+      best_eff = eff;
+      best_tps = tps;
+    }
+  }
+  return (int)best_tps;
+}
+
 struct Plan {
   int n_splits = 1, tiles_per_split = 1, caps = 0;
   bool pass1 = false;
   int stride = 1, n_sample = 0, ng = 0;
+  int p1_tiles_per_split = 1, p1_splits = 1;
   int max_entries = 0;
   size_t scratch_bytes(int64_t nq, int K) const {
     size_t b = (size_t)nq * (4 + 4 + 8) + (size_t)n_splits * nq * ((size_t)caps * 8 + 4);
@@ -47,23 +66,7 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   // workgroups resident per CU: LDS-limited (64 KiB tiles fit twice in 160 KiB)
   const int64_t per_cu = tile_lds_bytes(kind, 2) <= 80 * 1024 ? 2 : 1;
   const int64_t slots = (int64_t)n_cu * per_cu;
-  // candidate splits: the count whose workgroups fill the last dispatch round
-  // best (fraction of busy slots), fewer splits on ties
-  int64_t best_ns = 1;
-  double best_eff = -1.0;
-  for (int64_t ns = 1; ns <= std::min<int64_t>(16, n_ctiles); ns++) {
-    const int64_t tps = cdiv(n_ctiles, ns);
-    const int64_t real_ns = cdiv(n_ctiles, tps);
-    const int64_t wgs = nqb * real_ns;
-    const double rounds = (double)cdiv(wgs, slots);
-    // a round costs tps tiles; efficiency = useful tile work / (rounds * tps * slots)
-    const double eff = (double)(nqb * n_ctiles) / (rounds * (double)tps * (double)slots);
-    if (eff > best_eff + 1e-3) {
-      best_eff = eff;
-      best_ns = real_ns;
-    }
-  }
-  p.tiles_per_split = (int)cdiv(n_ctiles, best_ns);
+  p.tiles_per_split = best_split(nqb, n_ctiles, slots);
   p.n_splits = (int)cdiv(n_ctiles, p.tiles_per_split);
   if (n <= MAX_SORT) {
     // every candidate fits one sorted list: no threshold pass
@@ -94,6 +97,9 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   caps = std::max<int64_t>(caps, pow2ceil(cdiv(2LL * K, p.n_splits)));
   p.caps = (int)std::min<int64_t>(caps, MAX_SORT);
   p.max_entries = (int)std::min<int64_t>((int64_t)p.n_splits * p.caps, MAX_SORT);
+  const int64_t s_tiles = cdiv(p.n_sample, TC);
+  p.p1_tiles_per_split = best_split(nqb, s_tiles, slots);
+  p.p1_splits = (int)cdiv(s_tiles, p.p1_tiles_per_split);
   return p;
 }
 
@@ -147,9 +153,10 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       a1.n_c = p.n_sample;
       a1.c_stride = p.stride;
       a1.n_ctiles = (int32_t)cdiv(p.n_sample, TC);
+      a1.tiles_per_split = p.p1_tiles_per_split;
       a1.gmax = (float*)gmax.ptr;
       a1.ng = p.ng;
-      e = launch_pass1(kind, a1, (int)nqb, s);
+      e = launch_pass1(kind, a1, (int)nqb, p.p1_splits, s);
       if (e == hipSuccess) e = launch_tau_select((const float*)gmax.ptr, (int)nq, p.ng, K, (float*)tau.ptr, s);
       if (e != hipSuccess) {
         rc = ctx->hip_fail(e, "threshold pass");

@@ -46,15 +46,15 @@ constexpr int STAGE_BYTES = 128;     // bytes of each row staged per K step
 // instruction.  The 16-byte chunks of a row are XOR-swizzled by (row & 7) so the
 // MFMA fragment reads (8 consecutive rows, same chunk) hit 8 different bank groups.
 // Rows past the end read the last valid row; the epilogues mask them.
-template <int ROWS>
+template <int ROWS, int WAVES>
 __device__ __forceinline__ void dma_stage(uint8_t* lds_operand, const uint8_t* __restrict__ base,
                                           int64_t row_bytes, int row0, int nrows, int rstride, int kbyte) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rl = lane >> 3;                 // row within the 8-row group
   const int chunk = (lane & 7) ^ rl;        // logical chunk stored at physical slot lane & 7
 #pragma unroll
-  for (int i = 0; i < ROWS / 8 / 4; i++) {
-    const int grp = wave + 4 * i;
+  for (int i = 0; i < ROWS / 8 / WAVES; i++) {
+    const int grp = wave + WAVES * i;
     int g = row0 + grp * 8 + rl;
     g = g < nrows ? g : nrows - 1;
     const uint8_t* src = base + (int64_t)g * rstride * row_bytes + kbyte + chunk * 16;
@@ -64,22 +64,25 @@ __device__ __forceinline__ void dma_stage(uint8_t* lds_operand, const uint8_t* _
   }
 }
 
-template <int KIND, int BM, int BN>
+template <int KIND, int BM, int BN, int WM = 2, int WN = 2>
 struct Engine {
   using M = Mfma<KIND>;
   typedef typename M::acc_t acc_t;
-  static constexpr int TC = 64 * BM;                    // candidate rows per tile
-  static constexpr int TQ = 64 * BN;                    // query rows per tile
+  static constexpr int WAVES = WM * WN;
+  static constexpr int THREADS = 64 * WAVES;
+  static constexpr int TC = 32 * BM * WM;               // candidate rows per tile
+  static constexpr int TQ = 32 * BN * WN;               // query rows per tile
   static constexpr int STAGE_LDS = (TC + TQ) * STAGE_BYTES;
   static constexpr int LDS = 2 * STAGE_LDS;             // double buffered
-  static constexpr int GROUPS = 2 * BM * 2;             // pass-1 groups per tile
+  static constexpr int GROUPS = WM * BM * 2;            // pass-1 groups per tile
+  __device__ static int wave_m() { return (threadIdx.x >> 6) / WN; }
+  __device__ static int wave_n() { return (threadIdx.x >> 6) % WN; }
 
   // Accumulate the TC x TQ score tile (candidates c0.., queries q0..) over all
   // K stages.
   __device__ static void score_tile(acc_t (&acc)[BM][BN], uint8_t* lds, const TileArgs& a, int c0, int q0) {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave_m(), wn = wave_n();
 #pragma unroll
     for (int i = 0; i < BM; i++)
 #pragma unroll
@@ -89,16 +92,17 @@ struct Engine {
     const int arow = (wm * 32 * BM + (lane & 31)) * STAGE_BYTES;
     const int brow = TC * STAGE_BYTES + (wn * 32 * BN + (lane & 31)) * STAGE_BYTES;
     const int sw = lane & 7;
-    dma_stage<TC>(lds, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, 0);
-    dma_stage<TQ>(lds + TC * STAGE_BYTES, a.q_rows, a.row_bytes, q0, a.n_q, 1, 0);
+    dma_stage<TC, WAVES>(lds, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, 0);
+    dma_stage<TQ, WAVES>(lds + TC * STAGE_BYTES, a.q_rows, a.row_bytes, q0, a.n_q, 1, 0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int s = 0; s < nst; s++) {
       const uint8_t* cur = lds + (s & 1) * STAGE_LDS;
       if (s + 1 < nst) {
         uint8_t* nxt = lds + ((s + 1) & 1) * STAGE_LDS;
-        dma_stage<TC>(nxt, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, (s + 1) * STAGE_BYTES);
-        dma_stage<TQ>(nxt + TC * STAGE_BYTES, a.q_rows, a.row_bytes, q0, a.n_q, 1, (s + 1) * STAGE_BYTES);
+        dma_stage<TC, WAVES>(nxt, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, (s + 1) * STAGE_BYTES);
+        dma_stage<TQ, WAVES>(nxt + TC * STAGE_BYTES, a.q_rows, a.row_bytes, q0, a.n_q, 1,
+                             (s + 1) * STAGE_BYTES);
       }
       // fragments of K step ks + 1 are read from LDS while the MFMAs of step ks run
       constexpr int KS = STAGE_BYTES / M::KSTEP_BYTES;
@@ -154,16 +158,19 @@ __device__ __forceinline__ float fast_bound(float tau) {
 // Pass 1: group maxima over a strided candidate sample (GROUPS per tile:
 // wave row half x BM blocks x lane half, 16 candidates each).
 // ---------------------------------------------------------------------------
-template <int KIND, int BM, int BN>
-__global__ __launch_bounds__(256) void pass1_groupmax_kernel(TileArgs a) {
+template <int KIND, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void pass1_groupmax_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
-  using En = Engine<KIND, BM, BN>;
+  using En = Engine<KIND, BM, BN, WM, WN>;
   using M = Mfma<KIND>;
   typename En::acc_t acc[BM][BN];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int lane = threadIdx.x & 63;
+  const int wm = En::wave_m(), wn = En::wave_n();
   const int q0 = query_block(a) * En::TQ;
-  for (int t = 0; t < a.n_ctiles; t++) {
+  // candidate tiles of this split (blockIdx.y); group maxima are per tile
+  const int t_begin = blockIdx.y * a.tiles_per_split;
+  const int t_end = min(a.n_ctiles, t_begin + a.tiles_per_split);
+  for (int t = t_begin; t < t_end; t++) {
     const int c0 = t * En::TC;
     En::score_tile(acc, lds, a, c0, q0);
 #pragma unroll
@@ -181,7 +188,7 @@ __global__ __launch_bounds__(256) void pass1_groupmax_kernel(TileArgs a) {
             m = fmaxf(m, s);
           }
         }
-        const int g = ((t * 2 + wm) * BM + bm) * 2 + (lane >> 5);
+        const int g = ((t * WM + wm) * BM + bm) * 2 + (lane >> 5);
         if (q < a.n_q) a.gmax[(int64_t)q * a.ng + g] = m;
       }
     }
@@ -193,18 +200,18 @@ __global__ __launch_bounds__(256) void pass1_groupmax_kernel(TileArgs a) {
 // accumulator value is tested against a cheap bound first; the exact test and
 // the append run only in lanes whose value passes (exec-masked, inline).
 // ---------------------------------------------------------------------------
-template <int KIND, int BM, int BN>
-__global__ __launch_bounds__(256) void pass2_append_kernel(TileArgs a) {
+template <int KIND, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
-  using En = Engine<KIND, BM, BN>;
+  using En = Engine<KIND, BM, BN, WM, WN>;
   using M = Mfma<KIND>;
   int* cnt = (int*)(lds + En::LDS);
   typename En::acc_t acc[BM][BN];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int lane = threadIdx.x & 63;
+  const int wm = En::wave_m(), wn = En::wave_n();
   const int q0 = query_block(a) * En::TQ;
   const int split = blockIdx.y;
-  for (int i = threadIdx.x; i < En::TQ; i += 256) cnt[i] = 0;
+  for (int i = threadIdx.x; i < En::TQ; i += En::THREADS) cnt[i] = 0;
   float tau[BN], tfast[BN];
   uint64_t tkey[BN];
 #pragma unroll
@@ -249,7 +256,7 @@ __global__ __launch_bounds__(256) void pass2_append_kernel(TileArgs a) {
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < En::TQ; i += 256) {
+  for (int i = threadIdx.x; i < En::TQ; i += En::THREADS) {
     const int q = q0 + i;
     if (q < a.n_q) a.counts[(int64_t)split * a.n_q + q] = cnt[i];
   }
@@ -318,7 +325,9 @@ __global__ __launch_bounds__(256) void jac_pass1_kernel(TileArgs a) {
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   const int qb = query_block(a);
   const int q0 = qb * TILE;
-  for (int t = 0; t < a.n_ctiles; t++) {
+  const int t_begin = blockIdx.y * a.tiles_per_split;
+  const int t_end = min(a.n_ctiles, t_begin + a.tiles_per_split);
+  for (int t = t_begin; t < t_end; t++) {
     const int c0 = t * TILE;
     jac_tile(inter, lds, a, c0, q0);
 #pragma unroll
@@ -596,7 +605,8 @@ __global__ void finalize_kernel(const uint64_t* __restrict__ keys, int n_q, int 
 // ---------------------------------------------------------------------------
 namespace evoc {
 
-// MFMA tile shapes: 0 = <2,2>, 1 = <2,4>, 2 = <4,4> (see Engine)
+// MFMA tile shapes (see Engine): 0 = 4 waves of 2x2 blocks (128 x 128 tile),
+// 1 = 4 waves of 2x4 (128 x 256), 2 = 8 waves of 2x4 (256 x 256)
 static int g_forced_shape = -2;
 
 int tile_shape(int kind) {
@@ -636,35 +646,38 @@ size_t tile_lds_bytes(int kind, int pass) {
 }
 
 template <class Kern>
-static hipError_t launch_tile(Kern kern, size_t lds, dim3 grid, hipStream_t s, const TileArgs& a) {
+static hipError_t launch_tile(Kern kern, size_t lds, dim3 grid, hipStream_t s, const TileArgs& a,
+                              int threads = 256) {
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
+  hipLaunchKernelGGL(kern, grid, dim3(threads), lds, s, a);
   return hipGetLastError();
 }
+
+int tile_threads(int kind) { return tile_shape(kind) == 2 ? 512 : 256; }
 
 template <int KIND>
 static hipError_t launch_pass1_kind(const TileArgs& a, dim3 g, size_t lds, hipStream_t s) {
   switch (tile_shape(KIND)) {
-    case 1: return launch_tile(pass1_groupmax_kernel<KIND, 2, 4>, lds, g, s, a);
-    case 2: return launch_tile(pass1_groupmax_kernel<KIND, 4, 4>, lds, g, s, a);
-    default: return launch_tile(pass1_groupmax_kernel<KIND, 2, 2>, lds, g, s, a);
+    case 1: return launch_tile(pass1_groupmax_kernel<KIND, 2, 4, 2, 2>, lds, g, s, a);
+    case 2: return launch_tile(pass1_groupmax_kernel<KIND, 2, 4, 4, 2>, lds, g, s, a, 512);
+    default: return launch_tile(pass1_groupmax_kernel<KIND, 2, 2, 2, 2>, lds, g, s, a);
   }
 }
 
 template <int KIND>
 static hipError_t launch_pass2_kind(const TileArgs& a, dim3 g, size_t lds, hipStream_t s) {
   switch (tile_shape(KIND)) {
-    case 1: return launch_tile(pass2_append_kernel<KIND, 2, 4>, lds, g, s, a);
-    case 2: return launch_tile(pass2_append_kernel<KIND, 4, 4>, lds, g, s, a);
-    default: return launch_tile(pass2_append_kernel<KIND, 2, 2>, lds, g, s, a);
+    case 1: return launch_tile(pass2_append_kernel<KIND, 2, 4, 2, 2>, lds, g, s, a);
+    case 2: return launch_tile(pass2_append_kernel<KIND, 2, 4, 4, 2>, lds, g, s, a, 512);
+    default: return launch_tile(pass2_append_kernel<KIND, 2, 2, 2, 2>, lds, g, s, a);
   }
 }
 
-hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, hipStream_t s) {
+hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s) {
   const size_t lds = tile_lds_bytes(kind, 1);
-  const dim3 g((unsigned)n_blocks);
+  const dim3 g((unsigned)n_blocks, (unsigned)n_splits);
   switch (kind) {
     case EVOC_ELEM_BF16: return launch_pass1_kind<EVOC_ELEM_BF16>(a, g, lds, s);
     case EVOC_ELEM_F32: return launch_pass1_kind<EVOC_ELEM_F32>(a, g, lds, s);
