@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip timing the plain FP32 form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (known traffic, for scaling "
+                         "FETCH_SIZE / WRITE_SIZE counter readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus "
@@ -176,6 +179,14 @@ def main():
                           "score_kernel_tflops": flops / s32 / 1e12,
                           "score_kernel_frac": flops / s32 / 1e12 / PEAK["fp32"]}
         result["bf16_over_fp32"] = (n / sec) / (n / sec32)
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copied 1 GiB device to device")
+        del src, dst
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         result["cpu_baseline"] = cpu_baseline(x_np, k, args.cpu_seconds)

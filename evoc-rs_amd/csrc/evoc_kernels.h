@@ -23,8 +23,13 @@ struct TileArgs {
   int32_t n_c;             // candidates in candidate index space
   int32_t c_stride;        // row stride of the candidate sample (pass 1)
   int32_t n_ctiles;        // candidate tiles
-  int32_t tiles_per_split; // pass 2: tiles per blockIdx.y split
+  int32_t tiles_per_split; // candidate tiles per split
   int32_t q_offset;        // global index of query 0 (candidates are global)
+  // grid: n_qblocks x n_splits workgroups, laid out by tile_coords()
+  int32_t n_qblocks;       // query blocks (entries of block_list when given)
+  int32_t n_splits;        // candidate splits
+  int32_t group_q;         // query blocks per XCD group (they share candidate tiles)
+  int32_t pad1;
   const int32_t* block_list;  // optional list of query blocks (re-scoring)
   const uint32_t* row_pop;    // binary rows: popcount per row
   // pass 1

@@ -28,10 +28,13 @@ int64_t pow2ceil(int64_t v) {
 
 // Tiles per candidate split: the split count whose workgroups fill the last
 // dispatch round best (busy fraction of the resident slots), fewer splits on ties.
+// At least 4 splits when there are enough tiles: the workgroups resident on an
+// XCD then share query tiles as well as candidate tiles (see tile_coords).
 int best_split(int64_t nqb, int64_t n_ctiles, int64_t slots) {
   int64_t best_tps = n_ctiles;
   double best_eff = -1.0;
-  for (int64_t ns = 1; ns <= std::min<int64_t>(16, n_ctiles); ns++) {
+  const int64_t ns_min = n_ctiles >= 64 ? 4 : 1;
+  for (int64_t ns = ns_min; ns <= std::min<int64_t>(16, n_ctiles); ns++) {
     const int64_t tps = cdiv(n_ctiles, ns);
     const int64_t real_ns = cdiv(n_ctiles, tps);
     const double rounds = (double)cdiv(nqb * real_ns, slots);
@@ -49,6 +52,7 @@ struct Plan {
   bool pass1 = false;
   int stride = 1, n_sample = 0, ng = 0;
   int p1_tiles_per_split = 1, p1_splits = 1;
+  int group_q = 1, p1_group_q = 1;  // query blocks per XCD group (tile_coords)
   int max_entries = 0;
   size_t scratch_bytes(int64_t nq, int K) const {
     size_t b = (size_t)nq * (4 + 4 + 8) + (size_t)n_splits * nq * ((size_t)caps * 8 + 4);
@@ -68,6 +72,8 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   const int64_t slots = (int64_t)n_cu * per_cu;
   p.tiles_per_split = best_split(nqb, n_ctiles, slots);
   p.n_splits = (int)cdiv(n_ctiles, p.tiles_per_split);
+  const int64_t per_xcd = std::max<int64_t>(slots / 8, 1);  // workgroups resident per XCD
+  p.group_q = (int)std::max<int64_t>(1, per_xcd / p.n_splits);
   if (n <= MAX_SORT) {
     // every candidate fits one sorted list: no threshold pass
     p.pass1 = false;
@@ -100,6 +106,7 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   const int64_t s_tiles = cdiv(p.n_sample, TC);
   p.p1_tiles_per_split = best_split(nqb, s_tiles, slots);
   p.p1_splits = (int)cdiv(s_tiles, p.p1_tiles_per_split);
+  p.p1_group_q = (int)std::max<int64_t>(1, per_xcd / p.p1_splits);
   return p;
 }
 
@@ -156,6 +163,7 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       a1.tiles_per_split = p.p1_tiles_per_split;
       a1.gmax = (float*)gmax.ptr;
       a1.ng = p.ng;
+      a1.group_q = p.p1_group_q;
       e = launch_pass1(kind, a1, (int)nqb, p.p1_splits, s);
       if (e == hipSuccess) e = launch_tau_select((const float*)gmax.ptr, (int)nq, p.ng, K, (float*)tau.ptr, s);
       if (e != hipSuccess) {
@@ -172,6 +180,7 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
     a2.bufs = (uint64_t*)bufs.ptr;
     a2.counts = (int32_t*)counts.ptr;
     a2.caps = p.caps;
+    a2.group_q = p.group_q;
     SelectArgs sa = {};
     sa.bufs = (const uint64_t*)bufs.ptr;
     sa.counts = (const int32_t*)counts.ptr;

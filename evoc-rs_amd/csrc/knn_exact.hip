@@ -43,18 +43,25 @@ constexpr int STAGE_BYTES = 128;     // bytes of each row staged per K step
 
 // Stage loads go straight from global memory into LDS (global_load_lds_dwordx4:
 // lane i of a wave writes 16 bytes at M0 + 16 * i), 8 rows of 128 bytes per
-// instruction.  The 16-byte chunks of a row are XOR-swizzled by (row & 7) so the
-// MFMA fragment reads (8 consecutive rows, same chunk) hit 8 different bank groups.
+// instruction.  Logical 16-byte chunk c of local row r sits in physical slot
+// c ^ key(r), key(r) = (r & 7) ^ ((r >> 3) & 1).  A b128 fragment read takes the
+// same chunk of 16 consecutive rows per LDS cycle over 64 banks of 4 bytes
+// (256 bytes: rows of equal parity share a 128-byte bank half); key() puts
+// rows r and r + 8 in different slots, so the 16 reads cover all 64 banks.
+// For 8-row groups over 32 banks it is a permutation as well.
 // Rows past the end read the last valid row; the epilogues mask them.
+__device__ __forceinline__ int lds_chunk_key(int r) { return (r & 7) ^ ((r >> 3) & 1); }
+
 template <int ROWS, int WAVES>
 __device__ __forceinline__ void dma_stage(uint8_t* lds_operand, const uint8_t* __restrict__ base,
                                           int64_t row_bytes, int row0, int nrows, int rstride, int kbyte) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rl = lane >> 3;                 // row within the 8-row group
-  const int chunk = (lane & 7) ^ rl;        // logical chunk stored at physical slot lane & 7
 #pragma unroll
   for (int i = 0; i < ROWS / 8 / WAVES; i++) {
     const int grp = wave + WAVES * i;
+    // logical chunk stored at physical slot lane & 7 of local row grp * 8 + rl
+    const int chunk = (lane & 7) ^ lds_chunk_key(grp * 8 + rl);
     int g = row0 + grp * 8 + rl;
     g = g < nrows ? g : nrows - 1;
     const uint8_t* src = base + (int64_t)g * rstride * row_bytes + kbyte + chunk * 16;
@@ -88,10 +95,11 @@ struct Engine {
 #pragma unroll
       for (int j = 0; j < BN; j++) acc[i][j] = M::zero();
     const int nst = (int)(a.row_bytes / STAGE_BYTES);
-    // fragment rows of this lane; (row & 7) == (lane & 7) for every block
+    // fragment rows of this lane; every block starts at a multiple of 32 rows,
+    // so the swizzle key of the lane's row is the same in every block
     const int arow = (wm * 32 * BM + (lane & 31)) * STAGE_BYTES;
     const int brow = TC * STAGE_BYTES + (wn * 32 * BN + (lane & 31)) * STAGE_BYTES;
-    const int sw = lane & 7;
+    const int sw = lds_chunk_key(lane & 31);
     dma_stage<TC, WAVES>(lds, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, 0);
     dma_stage<TQ, WAVES>(lds + TC * STAGE_BYTES, a.q_rows, a.row_bytes, q0, a.n_q, 1, 0);
     __builtin_amdgcn_s_waitcnt(0);
@@ -140,9 +148,27 @@ struct Engine {
   }
 };
 
-// Query block handled by this workgroup.
-__device__ __forceinline__ int query_block(const TileArgs& a) {
-  return a.block_list ? a.block_list[blockIdx.x] : (int)blockIdx.x;
+// Workgroup -> (query block, candidate split), XCD-aware.  The hardware deals
+// workgroups of a launch round-robin to the 8 XCDs (linear id mod 8), each with
+// its own L2.  The grid is 1-D, padded to a multiple of 8, and XCD x takes the
+// x-th contiguous range of logical ids.  Logical ids run over groups of
+// group_q query blocks; inside a group, split-major.  The workgroups resident
+// on one XCD at a time then cover about group_q query blocks x n_splits
+// candidate splits: each query tile is shared by n_splits workgroups and each
+// candidate tile by group_q, so the L2 of that XCD fetches group_q + n_splits
+// tiles per tile step instead of one per workgroup.
+__device__ __forceinline__ bool tile_coords(const TileArgs& a, int& qb, int& split) {
+  const int per_xcd = (int)gridDim.x >> 3;
+  const int logical = ((int)blockIdx.x & 7) * per_xcd + ((int)blockIdx.x >> 3);
+  if (logical >= a.n_qblocks * a.n_splits) return false;
+  const int span = a.group_q * a.n_splits;
+  const int g = logical / span;
+  const int r = logical - g * span;
+  const int gq = min(a.group_q, a.n_qblocks - g * a.group_q);
+  split = r / gq;
+  const int slot = g * a.group_q + (r - split * gq);
+  qb = a.block_list ? a.block_list[slot] : slot;
+  return true;
 }
 
 // A lower bound on the score test that never rejects what the exact test
@@ -166,9 +192,11 @@ __global__ __launch_bounds__(64 * WM * WN) void pass1_groupmax_kernel(TileArgs a
   typename En::acc_t acc[BM][BN];
   const int lane = threadIdx.x & 63;
   const int wm = En::wave_m(), wn = En::wave_n();
-  const int q0 = query_block(a) * En::TQ;
-  // candidate tiles of this split (blockIdx.y); group maxima are per tile
-  const int t_begin = blockIdx.y * a.tiles_per_split;
+  int qb, split;
+  if (!tile_coords(a, qb, split)) return;
+  const int q0 = qb * En::TQ;
+  // candidate tiles of this split; group maxima are per tile
+  const int t_begin = split * a.tiles_per_split;
   const int t_end = min(a.n_ctiles, t_begin + a.tiles_per_split);
   for (int t = t_begin; t < t_end; t++) {
     const int c0 = t * En::TC;
@@ -196,7 +224,7 @@ __global__ __launch_bounds__(64 * WM * WN) void pass1_groupmax_kernel(TileArgs a
 }
 
 // ---------------------------------------------------------------------------
-// Pass 2: threshold filter + append.  blockIdx.y = candidate split.  Each
+// Pass 2: threshold filter + append, one candidate split per workgroup.  Each
 // accumulator value is tested against a cheap bound first; the exact test and
 // the append run only in lanes whose value passes (exec-masked, inline).
 // ---------------------------------------------------------------------------
@@ -209,8 +237,9 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
   typename En::acc_t acc[BM][BN];
   const int lane = threadIdx.x & 63;
   const int wm = En::wave_m(), wn = En::wave_n();
-  const int q0 = query_block(a) * En::TQ;
-  const int split = blockIdx.y;
+  int qb, split;
+  if (!tile_coords(a, qb, split)) return;
+  const int q0 = qb * En::TQ;
   for (int i = threadIdx.x; i < En::TQ; i += En::THREADS) cnt[i] = 0;
   float tau[BN], tfast[BN];
   uint64_t tkey[BN];
@@ -323,9 +352,10 @@ __global__ __launch_bounds__(256) void jac_pass1_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   uint32_t inter[8][8];
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int qb = query_block(a);
+  int qb, split;
+  if (!tile_coords(a, qb, split)) return;
   const int q0 = qb * TILE;
-  const int t_begin = blockIdx.y * a.tiles_per_split;
+  const int t_begin = split * a.tiles_per_split;
   const int t_end = min(a.n_ctiles, t_begin + a.tiles_per_split);
   for (int t = t_begin; t < t_end; t++) {
     const int c0 = t * TILE;
@@ -357,9 +387,9 @@ __global__ __launch_bounds__(256) void jac_pass2_kernel(TileArgs a) {
   int* cnt = (int*)(lds + 2 * TILE * JAC_PITCH);
   uint32_t inter[8][8];
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int qb = query_block(a);
+  int qb, split;
+  if (!tile_coords(a, qb, split)) return;
   const int q0 = qb * TILE;
-  const int split = blockIdx.y;
   if (threadIdx.x < TILE) cnt[threadIdx.x] = 0;
   float tau[8];
   uint64_t tkey[8];
@@ -616,7 +646,7 @@ int tile_shape(int kind) {
     g_forced_shape = e ? atoi(e) : -1;
   }
   if (g_forced_shape >= 0 && g_forced_shape <= 2) return g_forced_shape;
-  return 0;
+  return 2;  // fastest on 100k x 768 bf16: 19.6 ms vs 22.4 (shape 0) and 24.5 (shape 1)
 }
 
 void set_tile_shape(int shape) { g_forced_shape = shape; }
@@ -675,9 +705,20 @@ static hipError_t launch_pass2_kind(const TileArgs& a, dim3 g, size_t lds, hipSt
   }
 }
 
-hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s) {
+// 1-D grid of n_blocks x n_splits workgroups rounded up to whole XCD rounds
+// (tile_coords turns the padding into empty workgroups).
+static dim3 tile_grid(TileArgs& a, int n_blocks, int n_splits) {
+  a.n_qblocks = n_blocks;
+  a.n_splits = n_splits;
+  if (a.group_q < 1) a.group_q = 1;
+  const int64_t w = (int64_t)n_blocks * n_splits;
+  return dim3((unsigned)((w + 7) / 8 * 8));
+}
+
+hipError_t launch_pass1(int kind, const TileArgs& a0, int n_blocks, int n_splits, hipStream_t s) {
   const size_t lds = tile_lds_bytes(kind, 1);
-  const dim3 g((unsigned)n_blocks, (unsigned)n_splits);
+  TileArgs a = a0;
+  const dim3 g = tile_grid(a, n_blocks, n_splits);
   switch (kind) {
     case EVOC_ELEM_BF16: return launch_pass1_kind<EVOC_ELEM_BF16>(a, g, lds, s);
     case EVOC_ELEM_F32: return launch_pass1_kind<EVOC_ELEM_F32>(a, g, lds, s);
@@ -687,9 +728,10 @@ hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, int n_splits,
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_pass2(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s) {
+hipError_t launch_pass2(int kind, const TileArgs& a0, int n_blocks, int n_splits, hipStream_t s) {
   const size_t lds = tile_lds_bytes(kind, 2);
-  const dim3 g((unsigned)n_blocks, (unsigned)n_splits);
+  TileArgs a = a0;
+  const dim3 g = tile_grid(a, n_blocks, n_splits);
   switch (kind) {
     case EVOC_ELEM_BF16: return launch_pass2_kind<EVOC_ELEM_BF16>(a, g, lds, s);
     case EVOC_ELEM_F32: return launch_pass2_kind<EVOC_ELEM_F32>(a, g, lds, s);

@@ -19,7 +19,8 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libevoc_mi355x.so")
+# EVOC_MI355X_LIB names another build of the library (same ABI) to load instead
+LIB_PATH = os.environ.get("EVOC_MI355X_LIB") or os.path.join(_HERE, "libevoc_mi355x.so")
 
 DTYPE_F32, DTYPE_I8, DTYPE_U8 = 0, 1, 2
 MEM_HOST, MEM_DEVICE = 0, 1
