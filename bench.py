@@ -13,6 +13,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for p in (ROOT, os.path.join(ROOT, "evoc-rs_amd")):
     if p not in sys.path:
@@ -50,9 +52,10 @@ def recall_gpu(torch, idx, truth):
     return hits / (idx.shape[0] * k)
 
 
-def cpu_baseline(x_np, k, target_s):
+def cpu_baseline(x_np, k, target_s, n_iters=8, delta=0.0):
     """The reference's CPU path (oracle restatement): non-deterministic NN-descent,
-    T host threads, 8 iterations with delta = delta_improv = 0, on the first m rows."""
+    T host threads, n_iters iterations (delta = delta_improv as given; 0 runs them
+    all), on the first m rows."""
     import oracle as O
 
     try:
@@ -60,7 +63,7 @@ def cpu_baseline(x_np, k, target_s):
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(16, cores))
-    opts = O.KnnGraphOptions(n_neighbors=k, n_iters=8, delta=0.0, delta_improv=0.0)
+    opts = O.KnnGraphOptions(n_neighbors=k, n_iters=n_iters, delta=delta, delta_improv=delta)
 
     def run(m):
         t0 = time.perf_counter()
@@ -73,8 +76,255 @@ def cpu_baseline(x_np, k, target_s):
     if m2 > m:
         m, t = m2, run(m2)
     return {"value": m / t, "unit": "points/s", "cores": cores, "kind": "port",
-            "sample": f"first {m} rows of the workload, NN-descent k={k}, 8 iterations (delta=0), "
+            "sample": f"first {m} rows of the workload, NN-descent k={k}, {n_iters} iterations (delta={delta}), "
                       f"{cores} worker threads, {t:.1f} s"}
+
+
+def serving_bench(torch, E, n_req, rate, slots, budget_gb, seed=42):
+    """Poisson stream of knn_graph requests through the serving loop.
+
+    Request i: n log-uniform in [1e3, 2e5], d in {384, 768, 1024}, k in {10, 15, 30},
+    rows from Gaussian blobs (10-100 centres, std 0.25, L2-normalised), BF16 + FP32
+    re-score, exact search.  Rows are slices of device-resident pools (one per d and
+    blob count), outputs are device tensors: the graph inputs and results stay in
+    HBM.  Latency = completion - arrival (queueing included).
+    """
+    import math
+
+    import numpy as np
+
+    rs = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    pool_n, blob_choices = 200000, (10, 32, 100)
+    pools = {}
+    for d in (384, 768, 1024):
+        for nb in blob_choices:
+            c = torch.randn((nb, d), generator=g, device="cuda")
+            lab = torch.randint(0, nb, (pool_n,), generator=g, device="cuda")
+            x = c[lab] + 0.25 * torch.randn((pool_n, d), generator=g, device="cuda")
+            pools[(d, nb)] = torch.nn.functional.normalize(x, dim=1).contiguous()
+    torch.cuda.synchronize()
+    reqs = []
+    for _ in range(n_req):
+        n = int(round(math.exp(rs.uniform(math.log(1e3), math.log(2e5)))))
+        d = int(rs.choice([384, 768, 1024]))
+        k = int(rs.choice([10, 15, 30]))
+        nb = blob_choices[int(np.argmin([abs(b - rs.integers(10, 101)) for b in blob_choices]))]
+        off = int(rs.integers(0, pool_n - n + 1))
+        reqs.append((n, d, k, nb, off))
+    gaps = rs.exponential(1.0 / rate, size=n_req)
+    srv = E.Server(0, slots=slots, hbm_budget_bytes=int(budget_gb * (1 << 30)))
+    # warm the slots' pools and code paths
+    for (n, d, k, nb, off) in reqs[:slots]:
+        x = pools[(d, nb)][off:off + n]
+        srv.submit(x, E.KnnGraphOptions(n_neighbors=k), tau_state=[1, 2, 3], tree_states=None,
+                   out=(torch.empty((n, k), dtype=torch.int32, device="cuda"),
+                        torch.empty((n, k), dtype=torch.float32, device="cuda"))).wait(timeout=120.0)
+    tickets = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for i, (n, d, k, nb, off) in enumerate(reqs):
+        t_next += gaps[i]
+        now = time.perf_counter()
+        if t_next > now:
+            time.sleep(t_next - now)
+        x = pools[(d, nb)][off:off + n]
+        out = (torch.empty((n, k), dtype=torch.int32, device="cuda"),
+               torch.empty((n, k), dtype=torch.float32, device="cuda"))
+        t = srv.submit(x, E.KnnGraphOptions(n_neighbors=k), tau_state=[i + 1, 7, 11], out=out)
+        t.arrival = time.perf_counter() - t0
+        tickets.append(t)
+        if i % 250 == 249:
+            log(f"[bench] serving: {i + 1}/{n_req} submitted, {srv.stats()}")
+    for t in tickets:
+        t.wait(timeout=300.0)
+    makespan = time.perf_counter() - t0
+    srv.close()
+    lat = np.array([t.stats.queue_seconds + t.stats.seconds for t in tickets]) * 1e3
+    queue = np.array([t.stats.queue_seconds for t in tickets]) * 1e3
+    flops = sum(2.0 * n * n * d for (n, d, k, nb, off) in reqs)
+    pts = sum(n for (n, d, k, nb, off) in reqs)
+    return {"requests": n_req, "rate_per_s": rate, "slots": slots, "hbm_budget_gb": budget_gb,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "p99_queue_ms": float(np.percentile(queue, 99)),
+            "makespan_s": makespan, "requests_per_s": n_req / makespan, "points_per_s": pts / makespan,
+            "scoring_tflops": flops / makespan / 1e12,
+            "mfma_frac": flops / makespan / 1e12 / PEAK["bf16"],
+            "workload": "Poisson arrivals; n log-uniform [1e3, 2e5]; d in {384, 768, 1024}; "
+                        "k in {10, 15, 30}; Gaussian blobs (10/32/100 centres, std 0.25), L2-normalised, "
+                        "device-resident; exact BF16 + FP32 re-score"}
+
+
+def _unpack_bits(torch, xb):
+    """[m, B] uint8 -> [m, 8B] float32 bits, MSB first (numpy.packbits order)."""
+    shifts = torch.arange(7, -1, -1, device=xb.device, dtype=torch.int32)
+    return ((xb.to(torch.int32).unsqueeze(2) >> shifts) & 1).reshape(xb.shape[0], -1).float()
+
+
+def sampled_exact(torch, x, kind, sample, k, chunk=65536):
+    """Exact top-k of the sampled rows under the library's ranking: score desc,
+    self first (self scores 1.0, the reference's NN-descent seed), lower index
+    first.  int8: exact integer dots in FP64; binary: exact counts, IEEE f32
+    inter / union (FP64 quotient rounded once to f32: exact for counts <= 2^24).
+    Returns (indices [m, k] int64, scores [m, k] float64)."""
+    n = x.shape[0]
+    m = sample.shape[0]
+    lo_bits = (1 << 21) - 1
+    if kind == "i8":
+        xs = x[sample].double()
+    else:
+        xs = _unpack_bits(torch, x[sample])
+        ps = xs.sum(dim=1)
+    best = None
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        idx = torch.arange(s, e, device=x.device)
+        if kind == "i8":
+            sc = xs @ x[s:e].double().T
+            score_key = sc.long() + (1 << 24)
+        else:
+            xc = _unpack_bits(torch, x[s:e])
+            inter = xs @ xc.T
+            union = ps[:, None] + xc.sum(dim=1)[None, :] - inter
+            q = torch.where(union > 0, inter.double() / union.double().clamp(min=1), torch.zeros_like(inter, dtype=torch.float64))
+            score_key = q.float().view(torch.int32).long()
+        selfm = sample[:, None] == idx[None, :]
+        one = (1 << 24) + 1 if kind == "i8" else int(np.float32(1.0).view(np.int32))
+        score_key = torch.where(selfm, torch.full_like(score_key, one), score_key)
+        slot = torch.where(selfm, torch.zeros_like(idx)[None, :].expand(m, -1), (idx + 1)[None, :].expand(m, -1))
+        key = (score_key << 21) | (lo_bits - slot)
+        top = torch.topk(key, k, dim=1).values
+        best = top if best is None else torch.topk(torch.cat([best, top], dim=1), k, dim=1).values
+    slot = lo_bits - (best & lo_bits)
+    ids = torch.where(slot == 0, sample[:, None].expand(-1, k), slot - 1)
+    sk = best >> 21
+    if kind == "i8":
+        scores = (sk - (1 << 24)).double()
+    else:
+        scores = sk.int().view(torch.float32).double()
+    return ids, scores
+
+
+def _timed(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn(None)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        import evoc_mi355x as E
+        st = E.CallStats()
+        fn(st)
+        stats.append(st)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, stats
+
+
+def _base_line(args, metric_value, sec, workload, dtype, extra):
+    out = {"metric": METRIC, "value": metric_value, "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": workload}
+    out.update(extra)
+    return out
+
+
+def run_million(args, torch, E, kind):
+    """C3 (1,000,000 x 512 int8, k = 15) or C4 (1,000,000 x 1024 bits, k = 15): exact
+    search, checked bit-exact against a GPU brute force on 256 sampled rows."""
+    from evoc_mi355x import synth
+
+    n = args.n if args.n != 100000 else 1000000
+    k = 15
+    if kind == "i8":
+        d = 512
+        log(f"[bench] generating {n}x{d} int8 blobs on the GPU")
+        x = synth.blobs_i8_device(n, d, 200, 0.2, seed=42)
+        desc = (f"{n}x{d} int8 (200 Gaussian blobs, std 0.2, L2-normalised, x127, rounded), "
+                f"inner-product kNN k={k}, exact i8 MFMA into i32")
+        dtype, peak, unit, work = "int8 (i32 accumulate)", PEAK["int8"], "TOP/s", 2.0 * n * n * d
+    else:
+        nbytes = 128
+        log(f"[bench] generating {n}x{nbytes * 8}-bit rows on the GPU")
+        x = synth.bits_u8_device(n, nbytes * 8, 500, 0.3, 0.08, seed=42)
+        desc = (f"{n}x{nbytes * 8}-bit rows (500 Bernoulli(0.3) prototypes, bit flips p=0.08), "
+                f"bitwise Jaccard k={k}, VALU popcount")
+        # AND + popcount-accumulate per 32-bit word; peak = 256 CU x 64 lanes x 2.4 GHz int32 ops
+        dtype, peak, unit, work = "u8 bits (i32 popcount)", 256 * 64 * 2.4e9 / 1e12, "TOP/s", 2.0 * n * n * (nbytes / 4)
+    ctx = E.Context(0)
+    idx = torch.empty((n, k), dtype=torch.int32, device="cuda")
+    dist = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    opts = E.KnnGraphOptions(n_neighbors=k)
+    if kind == "i8":
+        log(f"[bench] autotune: {E.autotune(x, opts, ctx=ctx)}")
+
+    def step(st):
+        E.knn_graph(x, opts, E.NumpyRandomState(42), ctx=ctx, stats=st, out=(idx, dist))
+
+    step(None)
+    torch.cuda.synchronize()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    sample = torch.randperm(n, generator=g, device="cuda")[:256]
+    log("[bench] sampled exact check")
+    ti, _ = sampled_exact(torch, x, kind, sample, k)
+    exact_rows = int((idx[sample].long() == ti).all(dim=1).sum().item())
+    log(f"[bench] {exact_rows}/256 sampled rows bit-exact")
+    sec, stats = _timed(torch, step, args.steps, args.warmup)
+    score_s = float(np.mean([s.score_seconds for s in stats]))
+    achieved = work / score_s / 1e12
+    res = _base_line(args, n / sec, sec, {"workload": desc, "n": n, "k": k, "method": "exact"}, dtype, {
+        "sampled_rows_bit_exact": f"{exact_rows}/256",
+        "roofline": {"bound": "mfma" if kind == "i8" else "valu", "kernel":
+                     "pass2_append_kernel<i8>" if kind == "i8" else "jac_pass2_kernel",
+                     "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak, "traffic": None,
+                     "algorithmic_ops_per_launch": work, "avg_launch_ms": score_s * 1e3},
+    })
+    if not args.no_cpu_baseline:
+        # the reference's defaults at 1M rows: 20 iterations, delta = 0.001
+        xs = x[: min(n, 200000)].cpu().numpy()
+        res["cpu_baseline"] = cpu_baseline(xs, k, args.cpu_seconds, n_iters=20, delta=0.001)
+    ctx.close()
+    return res
+
+
+def run_nndescent(args, torch, E):
+    """C2 with the reference's algorithm: device NN-descent, 8 iterations,
+    delta = delta_improv = 0, FP32 and BF16 + FP32 re-score, recall vs FP64."""
+    from evoc_mi355x import synth
+
+    n, d, k = args.n, args.d, args.k
+    x = synth.blobs_f32_device(n, d, 50, 0.25, seed=42, zipf=1.1)
+    truth = fp64_truth(torch, x, k)
+    ctx = E.Context(0)
+    idx = torch.empty((n, k), dtype=torch.int32, device="cuda")
+    dist = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    out = {}
+    for name, prec in (("bf16", E.PREC_BF16_RESCORE), ("fp32", E.PREC_FP32)):
+        opts = E.KnnGraphOptions(n_neighbors=k, n_iters=8, delta=0.0, delta_improv=0.0, mode=E.MODE_NNDESCENT,
+                                 precision=prec, margin=args.margin)
+
+        def step(st, opts=opts):
+            E.knn_graph(x, opts, E.NumpyRandomState(42), ctx=ctx, stats=st, out=(idx, dist))
+
+        step(None)
+        torch.cuda.synchronize()
+        rec = recall_gpu(torch, idx, truth)
+        sec, stats = _timed(torch, step, args.steps, args.warmup)
+        out[name] = {"points_per_s": n / sec, "ms_per_step": sec * 1e3, "recall_at_k": rec,
+                     "iterations": stats[-1].iterations, "offers_dropped": stats[-1].offers_dropped}
+        log(f"[bench] nndescent {name}: {out[name]}")
+    ctx.close()
+    best = out["bf16"]
+    res = _base_line(args, best["points_per_s"], best["ms_per_step"] / 1e3,
+                     {"workload": f"{n}x{d} f32 blobs (50 centres, std 0.25, Zipf 1.1), cosine k={k}, "
+                                  f"device NN-descent 8 iterations (delta=0), BF16 + FP32 re-score of k+{args.margin}",
+                      "n": n, "d": d, "k": k, "method": "nndescent"},
+                     "bf16 (fp32 accumulate) + fp32 rescore", {"recall_at_k": best["recall_at_k"], "fp32": out["fp32"],
+                                                               "bf16": out["bf16"]})
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(x.cpu().numpy(), k, args.cpu_seconds)
+    return res
 
 
 def main():
@@ -89,6 +339,13 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip timing the plain FP32 form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c2-nndescent", "c3", "c4"],
+                    help="c2: 100k x 768 f32 exact + serving stream (default); c2-nndescent: the same rows "
+                         "through device NN-descent; c3: 1M x 512 int8; c4: 1M x 1024-bit binary")
+    ap.add_argument("--serving-requests", type=int, default=2000,
+                    help="requests in the serving stream (0: skip the serving leg)")
+    ap.add_argument("--serving-rate", type=float, default=60.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy (known traffic, for scaling "
                          "FETCH_SIZE / WRITE_SIZE counter readings)")
@@ -98,11 +355,17 @@ def main():
               f"{args.gpus}). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     import evoc_mi355x as E
     from evoc_mi355x import synth
+
+    if args.workload in ("c3", "c4"):
+        print(json.dumps(run_million(args, torch, E, "i8" if args.workload == "c3" else "u8")), flush=True)
+        return
+    if args.workload == "c2-nndescent":
+        print(json.dumps(run_nndescent(args, torch, E)), flush=True)
+        return
 
     n, d, k = args.n, args.d, args.k
     log(f"[bench] generating {n}x{d} blobs")
@@ -179,6 +442,12 @@ def main():
                           "score_kernel_tflops": flops / s32 / 1e12,
                           "score_kernel_frac": flops / s32 / 1e12 / PEAK["fp32"]}
         result["bf16_over_fp32"] = (n / sec) / (n / sec32)
+    if args.serving_requests > 0:
+        log(f"[bench] serving: {args.serving_requests} requests at {args.serving_rate}/s")
+        result["serving"] = serving_bench(torch, E, args.serving_requests, args.serving_rate,
+                                          args.serving_slots, 200.0)
+        result["latency_ms"] = {"p50": result["serving"]["p50_ms"], "p99": result["serving"]["p99_ms"],
+                                "source": "serving stream"}
     if args.copy_calibration:
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
         dst = torch.empty_like(src)

@@ -14,6 +14,9 @@ struct evoc_mi355x_ctx {
   uint64_t in_use = 0;   // bytes currently held
   hipStream_t stream = nullptr;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  // stream-ordered device memory of this context: frees do not synchronise the
+  // device, so contexts on other streams keep running (serving slots)
+  hipMemPool_t pool = nullptr;
   std::mutex mu;
   std::string last_error;
 
@@ -23,7 +26,8 @@ struct evoc_mi355x_ctx {
 };
 typedef evoc_mi355x_ctx Ctx;
 
-// Device allocation accounted against the context's HBM budget.
+// Device allocation accounted against the context's HBM budget, from the
+// context's memory pool in the order of its stream.
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
@@ -75,6 +79,20 @@ struct NndStats {
 int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, const uint32_t* pop,
                      const float* frows, int64_t frow_ld, int dim, int64_t n, int Kg,
                      const NndParams& P, uint64_t* out_keys, NndStats* st);
+
+// Device bytes of the NN-descent working set (nnd_driver.cpp).
+uint64_t nndescent_bytes(int kind, int64_t n, int Kg, int max_candidates, int64_t leaf_size, int64_t row_bytes,
+                         int dim);
+
+// Tile shape for a search over n rows of row_bytes (tuner.cpp): the
+// autotuner's choice for the size bucket, else a size heuristic.
+int choose_tile_shape(int kind, int64_t n, int64_t row_bytes);
+
+// Exact search planning (exact.cpp): the query slab that fits `avail` bytes of
+// scratch (called with the search's tile shape in force), and the scratch
+// bytes of the whole search in one slab or in the smallest slab.
+int64_t exact_slab(int kind, int64_t n, int K, int n_cu, uint64_t avail);
+uint64_t exact_scratch_bytes(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, bool minimum);
 
 // Exact K best candidates per query over all rows (see knn_exact.hip).
 // rows: n x row_bytes scored rows; pop: per-row popcounts (binary only).

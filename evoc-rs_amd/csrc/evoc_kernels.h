@@ -61,8 +61,8 @@ struct SelectArgs {
 };
 
 // ---- launchers (knn_exact.hip) ----
-int tile_shape(int kind);         // MFMA tile shape in use (-1 for binary rows)
-void set_tile_shape(int shape);   // force a shape (-1: default)
+int tile_shape(int kind);         // MFMA tile shape in force on this thread (-1 for binary rows)
+void set_tile_shape(int shape);   // this thread's shape for the following launches (-1: default)
 int tile_rows_c(int kind);        // candidate rows per tile
 int tile_rows_q(int kind);        // query rows per tile
 int groups_per_tile(int kind);    // pass-1 group maxima per candidate tile

@@ -144,7 +144,11 @@ evoc_mi355x_ctx::~evoc_mi355x_ctx() {
     (void)hipSetDevice(device);
     if (ev_begin) (void)hipEventDestroy(ev_begin);
     if (ev_end) (void)hipEventDestroy(ev_end);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+    if (pool) (void)hipMemPoolDestroy(pool);
   }
 }
 
@@ -162,7 +166,7 @@ DevBuf::~DevBuf() { release(); }
 
 void DevBuf::release() {
   if (ptr) {
-    (void)hipFree(ptr);
+    (void)hipFreeAsync(ptr, owner->stream);
     if (owner) owner->in_use -= bytes;
     ptr = nullptr;
     bytes = 0;
@@ -179,7 +183,7 @@ int DevBuf::alloc(Ctx* c, size_t nbytes, const char* what) {
              what, nbytes, (size_t)c->in_use, (size_t)c->budget);
     return c->fail(EVOC_ERR_OUT_OF_BUDGET, buf);
   }
-  hipError_t e = hipMalloc(&ptr, nbytes);
+  hipError_t e = hipMallocFromPoolAsync(&ptr, nbytes, c->pool, c->stream);
   if (e != hipSuccess) {
     ptr = nullptr;
     (void)hipGetLastError();
@@ -228,6 +232,18 @@ int evoc_mi355x_create(int32_t device, uint64_t hbm_budget_bytes, evoc_mi355x_ct
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)((double)free_b * 0.9);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+    delete c;
+    return EVOC_ERR_HIP;
+  }
+  // private pool; freed blocks stay cached for the next call until destroy
+  hipMemPoolProps props = {};
+  props.allocType = hipMemAllocationTypePinned;
+  props.location.type = hipMemLocationTypeDevice;
+  props.location.id = device;
+  uint64_t keep = UINT64_MAX;
+  if (hipMemPoolCreate(&c->pool, &props) != hipSuccess ||
+      hipMemPoolSetAttribute(c->pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
+    (void)hipGetLastError();
     delete c;
     return EVOC_ERR_HIP;
   }
@@ -304,6 +320,90 @@ int evoc_mi355x_knn_graph(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const 
 static size_t elem_size(int dtype) { return dtype == EVOC_DTYPE_F32 ? 4 : 1; }
 
 static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+static uint64_t r256(uint64_t b) { return (b + 255) & ~(uint64_t)255; }
+
+// Everything about a call that its validated arguments fix: search method,
+// scored element kind, candidate count K, row layouts.  Shared by the entry
+// point and by the device-memory estimate of the serving loop.
+struct CallShape {
+  int dtype = 0, kind = 0;
+  bool bf16 = false, seed_self = true, nnd = false, dev_out = false, host_in = false;
+  int64_t n = 0, d = 0, k = 0, K = 0;
+  int64_t row_bytes = 0, f32_ld = 0, in_span = 0;
+};
+
+static CallShape call_shape(const evoc_knn_input* in, const evoc_knn_options* o, int32_t out_location,
+                            int64_t out_capacity_k) {
+  CallShape c;
+  c.dtype = in->dtype;
+  c.n = in->n;
+  c.d = in->d;
+  c.k = evoc_knn_effective_k(o->n_neighbors, c.n);
+  c.bf16 = c.dtype == EVOC_DTYPE_F32 && o->precision == EVOC_PREC_BF16_RESCORE;
+  // brute_force_knn (src/knn.rs:189-281) skips self; the NN-descent layout seeds it.
+  c.seed_self = !brute_force_branch(c.dtype, c.n, o);
+  // NN-descent mode runs the device NN-descent; the brute-force branch
+  // (src/knn.rs:418-422) and the exact mode run the tiled exact search.
+  c.nnd = o->mode == EVOC_MODE_NNDESCENT && c.seed_self;
+  c.K = c.bf16 ? std::min<int64_t>(c.k + std::max(0, o->margin), c.n) : c.k;
+  if (c.nnd) c.K = std::min<int64_t>(c.K, 64);
+  int64_t elem_bytes = 1;
+  if (c.dtype == EVOC_DTYPE_F32) {
+    c.kind = c.bf16 ? EVOC_ELEM_BF16 : EVOC_ELEM_F32;
+    elem_bytes = c.bf16 ? 2 : 4;
+    c.f32_ld = round_up(c.d, 32);
+  } else {
+    c.kind = c.dtype == EVOC_DTYPE_I8 ? EVOC_ELEM_I8 : EVOC_ELEM_U8BITS;
+  }
+  c.row_bytes = round_up(c.d * elem_bytes, 128);
+  c.dev_out = out_location == EVOC_MEM_DEVICE && out_capacity_k == c.k;
+  c.host_in = in->location != EVOC_MEM_DEVICE;
+  c.in_span = (c.n - 1) * in->ld + c.d;
+  return c;
+}
+
+// Device bytes of everything the call holds besides the search's own scratch.
+static uint64_t fixed_bytes(const CallShape& c) {
+  uint64_t b = 0;
+  if (c.host_in) b += r256((uint64_t)c.in_span * elem_size(c.dtype));
+  b += r256((uint64_t)(c.n * c.row_bytes));
+  if (c.dtype == EVOC_DTYPE_F32) {
+    if (c.bf16) b += r256((uint64_t)(c.n * c.f32_ld * 4));
+    b += r256((uint64_t)c.n * 4) + 256;
+  }
+  if (c.dtype == EVOC_DTYPE_U8) b += r256((uint64_t)c.n * 4);
+  b += r256((uint64_t)(c.n * c.K * 8));
+  if (c.bf16 || c.K != c.k) b += r256((uint64_t)(c.n * c.k * 8));
+  if (!c.dev_out) b += 2 * r256((uint64_t)(c.n * c.k * 4));
+  return b;
+}
+
+static int64_t nnd_max_candidates(const evoc_knn_options* o, int64_t k) {
+  return o->max_candidates > 0 ? o->max_candidates : std::min<int64_t>(60, (int64_t)((float)k * 1.5f));
+}
+static int64_t nnd_leaf_size(const evoc_knn_options* o, int64_t k) {
+  return std::max<int64_t>(o->leaf_size > 0 ? o->leaf_size : std::max<int64_t>(10, k), std::max<int64_t>(10, k));
+}
+
+extern "C" int evoc_mi355x_request_bytes(const evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
+                                         const evoc_knn_options* o, int32_t out_location, uint64_t* full,
+                                         uint64_t* minimum) {
+  if (!ctx || !in || !o || in->n <= 0 || in->d <= 0 || o->n_neighbors <= 0) return EVOC_ERR_INVALID_ARGUMENT;
+  const CallShape c = call_shape(in, o, out_location, evoc_knn_effective_k(o->n_neighbors, in->n));
+  const uint64_t f = fixed_bytes(c);
+  uint64_t hi, lo;
+  if (c.nnd) {
+    hi = lo = f + nndescent_bytes(c.kind, c.n, (int)c.K, (int)nnd_max_candidates(o, c.k), nnd_leaf_size(o, c.k),
+                                  c.row_bytes, (int)c.d);
+  } else {
+    const int K = (int)std::min<int64_t>(c.K, 1024);
+    hi = f + exact_scratch_bytes(c.kind, c.n, c.row_bytes, K, ctx->n_cu, false);
+    lo = f + exact_scratch_bytes(c.kind, c.n, c.row_bytes, K, ctx->n_cu, true);
+  }
+  if (full) *full = hi;
+  if (minimum) *minimum = lo;
+  return EVOC_OK;
+}
 
 extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
                                             const evoc_knn_options* o, const int64_t tau_state[3],
@@ -330,9 +430,6 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
   const int64_t k = evoc_knn_effective_k(o->n_neighbors, n);
   if (out->capacity_k < k) return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "output capacity_k < effective k");
   if (!out->indices || !out->distances) return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "null output arrays");
-  (void)tau_state;
-  (void)tree_states;
-  (void)n_tree_rows;
   out->k_out = k;
   out->offers_dropped = 0;
   out->score_seconds = 0.0;
@@ -340,23 +437,28 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
   out->rescore_rounds = 0;
   if (o->mode != EVOC_MODE_EXACT && o->mode != EVOC_MODE_NNDESCENT)
     return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "unknown mode");
+  const CallShape cs = call_shape(in, o, out->location, out->capacity_k);
+  const bool bf16 = cs.bf16, nnd = cs.nnd;
+  const int64_t K = cs.K;
+  if (bf16 && K > 256)
+    return ctx->fail(EVOC_ERR_UNSUPPORTED, "BF16 re-score keeps at most 256 candidates: need k + margin <= 256");
+  if (K > 1024) return ctx->fail(EVOC_ERR_UNSUPPORTED, "exact search supports k <= 1024");
+  if (nnd && k > 64) return ctx->fail(EVOC_ERR_UNSUPPORTED, "NN-descent mode supports k <= 64");
   if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(EVOC_ERR_HIP, "hipSetDevice failed");
 
   hipStream_t st = ctx->stream;
   const int dtype = in->dtype;
-  const bool bf16 = dtype == EVOC_DTYPE_F32 && o->precision == EVOC_PREC_BF16_RESCORE;
-  // brute_force_knn (src/knn.rs:189-281) skips self; the NN-descent layout seeds it.
-  const bool seed_self = !brute_force_branch(dtype, n, o);
+  const bool seed_self = cs.seed_self;
 
   // ---- stage the input on the device ----
   const int64_t in_row_bytes = d * (int64_t)elem_size(dtype);
   DevBuf src;
   const uint8_t* d_src;
-  if (in->location == EVOC_MEM_DEVICE) {
+  if (!cs.host_in) {
     d_src = (const uint8_t*)in->data;
   } else {
     // the caller's matrix spans (n - 1) * ld + d elements
-    const size_t span = (size_t)((n - 1) * in->ld + d) * elem_size(dtype);
+    const size_t span = (size_t)cs.in_span * elem_size(dtype);
     int rc = src.alloc(ctx, span, "input copy");
     if (rc) return rc;
     hipError_t e = hipMemcpyAsync(src.ptr, in->data, span, hipMemcpyHostToDevice, st);
@@ -366,27 +468,14 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
   (void)hipEventRecord(ctx->ev_begin, st);
 
   // scored rows: padded to a multiple of 128 bytes of the scored element type
-  int kind;
-  int64_t elem_bytes;
-  if (dtype == EVOC_DTYPE_F32) {
-    kind = bf16 ? EVOC_ELEM_BF16 : EVOC_ELEM_F32;
-    elem_bytes = bf16 ? 2 : 4;
-  } else if (dtype == EVOC_DTYPE_I8) {
-    kind = EVOC_ELEM_I8;
-    elem_bytes = 1;
-  } else {
-    kind = EVOC_ELEM_U8BITS;
-    elem_bytes = 1;
-  }
-  const int64_t row_bytes = round_up(d * elem_bytes, 128);
+  const int kind = cs.kind;
+  const int64_t row_bytes = cs.row_bytes, f32_ld = cs.f32_ld;
   DevBuf rows, xf32, norms, flag, pop;
   int rc = rows.alloc(ctx, (size_t)(n * row_bytes), "scored rows");
   if (rc) return rc;
   hipError_t e = hipSuccess;
-  int64_t f32_ld = 0;
   if (dtype == EVOC_DTYPE_F32) {
     // prepared f32 rows (kept for the FP32 re-score) + the scored copy
-    f32_ld = round_up(d, 32);
     if (bf16) {
       if ((rc = xf32.alloc(ctx, (size_t)(n * f32_ld * 4), "prepared f32 rows"))) return rc;
     }
@@ -406,27 +495,34 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
   }
   if (e != hipSuccess) return ctx->hip_fail(e, "input preparation");
 
-  // ---- search ----
-  // NN-descent mode runs the device NN-descent; the brute-force branch
-  // (src/knn.rs:418-422) and the exact mode run the tiled exact search.
-  const bool nnd = o->mode == EVOC_MODE_NNDESCENT && seed_self;
-  int64_t K = bf16 ? std::min<int64_t>(k + std::max(0, o->margin), n) : k;
-  if (nnd) K = std::min<int64_t>(K, 64);
-  if (bf16 && K > 256)
-    return ctx->fail(EVOC_ERR_UNSUPPORTED, "BF16 re-score keeps at most 256 candidates: need k + margin <= 256");
-  if (K > 1024) return ctx->fail(EVOC_ERR_UNSUPPORTED, "exact search supports k <= 1024");
-  if (nnd && k > 64) return ctx->fail(EVOC_ERR_UNSUPPORTED, "NN-descent mode supports k <= 64");
-  DevBuf keys;
+  // ---- result buffers first, so the search can plan with what is left ----
+  DevBuf keys, fkeys, oidx, odist;
   if ((rc = keys.alloc(ctx, (size_t)(n * K * 8), "candidate keys"))) return rc;
+  if (bf16 || K != k) {
+    if ((rc = fkeys.alloc(ctx, (size_t)(n * k * 8), "rescored keys"))) return rc;
+  }
+  int32_t* d_idx;
+  float* d_dist;
+  const bool dev_out = cs.dev_out;
+  if (dev_out) {
+    d_idx = out->indices;
+    d_dist = out->distances;
+  } else {
+    if ((rc = oidx.alloc(ctx, (size_t)(n * k * 4), "output indices"))) return rc;
+    if ((rc = odist.alloc(ctx, (size_t)(n * k * 4), "output distances"))) return rc;
+    d_idx = (int32_t*)oidx.ptr;
+    d_dist = (float*)odist.ptr;
+  }
+
+  // ---- search ----
   if (nnd) {
     NndParams P;
     P.k = (int)k;
     P.n_iters = o->n_iters > 0 ? (int)o->n_iters
                                : (int)std::max<int64_t>(5, (int64_t)std::llround(std::log2((double)n)));
-    P.max_candidates = o->max_candidates > 0 ? (int)o->max_candidates
-                                             : (int)std::min<int64_t>(60, (int64_t)((float)k * 1.5f));
+    P.max_candidates = (int)nnd_max_candidates(o, k);
     P.n_trees = (int)default_n_trees(o);
-    P.leaf_size = std::max<int64_t>(o->leaf_size > 0 ? o->leaf_size : std::max<int64_t>(10, k), std::max<int64_t>(10, k));
+    P.leaf_size = nnd_leaf_size(o, k);
     P.max_depth = o->max_rptree_depth;
     P.delta = o->delta;
     P.delta_improv = o->delta_improv;
@@ -456,10 +552,8 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
     out->score_seconds = xs.pass2_ms * 1e-3;
   }
 
-  DevBuf fkeys;
   const uint64_t* final_keys = (const uint64_t*)keys.ptr;
   if (bf16 || K != k) {
-    if ((rc = fkeys.alloc(ctx, (size_t)(n * k * 8), "rescored keys"))) return rc;
     if (dtype == EVOC_DTYPE_F32) {
       e = launch_rescore((const float*)(bf16 ? xf32.ptr : rows.ptr), f32_ld, (int)d, (int)n, 0,
                          (const uint64_t*)keys.ptr, (int)K, (int)k, (uint64_t*)fkeys.ptr, st);
@@ -472,19 +566,6 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
   }
 
   // ---- transform + write out ----
-  DevBuf oidx, odist;
-  int32_t* d_idx;
-  float* d_dist;
-  const bool dev_out = out->location == EVOC_MEM_DEVICE && out->capacity_k == k;
-  if (dev_out) {
-    d_idx = out->indices;
-    d_dist = out->distances;
-  } else {
-    if ((rc = oidx.alloc(ctx, (size_t)(n * k * 4), "output indices"))) return rc;
-    if ((rc = odist.alloc(ctx, (size_t)(n * k * 4), "output distances"))) return rc;
-    d_idx = (int32_t*)oidx.ptr;
-    d_dist = (float*)odist.ptr;
-  }
   e = launch_finalize(final_keys, (int)n, 0, (int)k, dtype, d_idx, d_dist, st);
   if (e != hipSuccess) return ctx->hip_fail(e, "finalize");
   (void)hipEventRecord(ctx->ev_end, st);

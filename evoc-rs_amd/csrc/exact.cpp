@@ -54,10 +54,13 @@ struct Plan {
   int p1_tiles_per_split = 1, p1_splits = 1;
   int group_q = 1, p1_group_q = 1;  // query blocks per XCD group (tile_coords)
   int max_entries = 0;
-  size_t scratch_bytes(int64_t nq, int K) const {
-    size_t b = (size_t)nq * (4 + 4 + 8) + (size_t)n_splits * nq * ((size_t)caps * 8 + 4);
-    if (pass1) b += (size_t)nq * ng * 4;
-    (void)K;
+  // device bytes exact_search allocates for a slab of nq queries (each buffer
+  // rounded to 256 bytes by DevBuf)
+  size_t scratch_bytes(int64_t nq, int64_t nqb) const {
+    auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t b = r((size_t)n_splits * nq * caps * 8) + r((size_t)n_splits * nq * 4) + r((size_t)nq * 4) +
+               r((size_t)nq * 8) + r((size_t)(nqb + nq) * 4);
+    if (pass1) b += r((size_t)nq * ng * 4) + r((size_t)nq * 4);
     return b;
   }
 };
@@ -112,20 +115,42 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
 
 }  // namespace
 
+namespace {
+// the tile shape of one search, on this thread, for the duration of the search
+struct ShapeScope {
+  explicit ShapeScope(int shape) { set_tile_shape(shape); }
+  ~ShapeScope() { set_tile_shape(-1); }
+};
+}  // namespace
+
+int64_t exact_slab(int kind, int64_t n, int K, int n_cu, uint64_t avail) {
+  const int TQ = tile_rows_q(kind);
+  int64_t slab = n;
+  for (;;) {
+    const Plan p = make_plan(kind, n, slab, K, n_cu);
+    if (p.scratch_bytes(slab, cdiv(slab, TQ)) <= avail || slab <= 4 * TQ) return slab;
+    slab = std::max<int64_t>(4 * TQ, cdiv(slab / 2, TQ) * TQ);
+  }
+}
+
+uint64_t exact_scratch_bytes(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, bool minimum) {
+  ShapeScope shape_scope(choose_tile_shape(kind, n, row_bytes));
+  const int TQ = tile_rows_q(kind);
+  const int64_t nq = std::min<int64_t>(minimum ? 4 * (int64_t)TQ : n, n);
+  return make_plan(kind, n, nq, K, n_cu).scratch_bytes(nq, cdiv(nq, TQ));
+}
+
 int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, const uint32_t* pop,
                  int64_t n, int K, bool seed_self, uint64_t* keys, ExactStats* stats) {
+  ShapeScope shape_scope(choose_tile_shape(kind, n, row_bytes));
   hipStream_t s = ctx->stream;
   const int TC = tile_rows_c(kind), TQ = tile_rows_q(kind);
   if (K < 1 || K > MAX_SORT / 2 || K > n)
     return ctx->fail(EVOC_ERR_UNSUPPORTED, "exact search needs 1 <= K <= min(n, 1024)");
-  // slab of queries whose scratch fits a quarter of the remaining budget
+  // largest slab of queries whose scratch fits the rest of the budget (the
+  // caller has allocated its outputs already)
   const uint64_t avail = ctx->budget > ctx->in_use ? ctx->budget - ctx->in_use : 0;
-  int64_t slab = n;
-  for (;;) {
-    Plan p = make_plan(kind, n, slab, K, ctx->n_cu);
-    if (p.scratch_bytes(slab, K) <= avail / 4 || slab <= 4 * TQ) break;
-    slab = std::max<int64_t>(4 * TQ, cdiv(slab / 2, TQ) * TQ);
-  }
+  const int64_t slab = exact_slab(kind, n, K, ctx->n_cu, avail);
   const int64_t n_ctiles = cdiv(n, TC);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   (void)hipEventCreate(&e0);

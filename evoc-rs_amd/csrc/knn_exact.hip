@@ -39,36 +39,64 @@ namespace evoc {
 // (B operand), so lane l of a 32x32 block holds query column l % 32 and the 16
 // candidate rows 8*(j/4) + 4*(l/32) + j%4, j = 0..15.
 // ---------------------------------------------------------------------------
-constexpr int STAGE_BYTES = 128;     // bytes of each row staged per K step
+constexpr int STAGE_BYTES = 128;     // row bytes per stage of the Jaccard tiles
+constexpr int MFMA_SB = 64;          // row bytes per stage of the MFMA tiles
+constexpr int MFMA_NS = 4;           // stages in flight (LDS ring buffers)
 
-// Stage loads go straight from global memory into LDS (global_load_lds_dwordx4:
-// lane i of a wave writes 16 bytes at M0 + 16 * i), 8 rows of 128 bytes per
-// instruction.  Logical 16-byte chunk c of local row r sits in physical slot
-// c ^ key(r), key(r) = (r & 7) ^ ((r >> 3) & 1).  A b128 fragment read takes the
-// same chunk of 16 consecutive rows per LDS cycle over 64 banks of 4 bytes
-// (256 bytes: rows of equal parity share a 128-byte bank half); key() puts
-// rows r and r + 8 in different slots, so the 16 reads cover all 64 banks.
-// For 8-row groups over 32 banks it is a permutation as well.
-// Rows past the end read the last valid row; the epilogues mask them.
-__device__ __forceinline__ int lds_chunk_key(int r) { return (r & 7) ^ ((r >> 3) & 1); }
+// MFMA stage loads go straight from global memory into LDS
+// (global_load_lds_dwordx4: lane i of a wave writes 16 bytes at M0 + 16 * i),
+// 16 rows of 64 bytes per instruction.  Logical 16-byte chunk c of local row r
+// sits in physical slot c ^ key(r), key(r) = ((r >> 2) ^ (r >> 1)) & 3.  A b128
+// fragment read takes the same chunk of 16 consecutive rows per LDS cycle over
+// 64 banks of 4 bytes; four 64-byte rows share a 256-byte bank line, and key()
+// gives rows r, r + 4, r + 8, r + 12 four different slots, so the 16 reads cover
+// all 64 banks.  Over 32 banks (8 rows per cycle, two rows per line) it is
+// conflict-free as well.  Rows past the end read the last valid row; the
+// epilogues mask them.
+__device__ __forceinline__ int lds_chunk_key(int r) { return ((r >> 2) ^ (r >> 1)) & 3; }
 
 template <int ROWS, int WAVES>
 __device__ __forceinline__ void dma_stage(uint8_t* lds_operand, const uint8_t* __restrict__ base,
                                           int64_t row_bytes, int row0, int nrows, int rstride, int kbyte) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rl = lane >> 3;                 // row within the 8-row group
+  const int rl = lane >> 2;                 // row within the 16-row group
+  static_assert(ROWS % (16 * WAVES) == 0, "stage rows must split evenly over the waves");
+  // uniform 64-bit base of the tile's first row; per-lane 32-bit offsets
+  // (a tile spans ROWS * rstride rows, well under 4 GiB)
+  const uint32_t pitch = (uint32_t)(rstride * row_bytes);
+  const uint8_t* tile = base + (int64_t)row0 * rstride * row_bytes;
+  const int last = nrows - 1 - row0;
 #pragma unroll
-  for (int i = 0; i < ROWS / 8 / WAVES; i++) {
+  for (int i = 0; i < ROWS / 16 / WAVES; i++) {
     const int grp = wave + WAVES * i;
-    // logical chunk stored at physical slot lane & 7 of local row grp * 8 + rl
-    const int chunk = (lane & 7) ^ lds_chunk_key(grp * 8 + rl);
-    int g = row0 + grp * 8 + rl;
-    g = g < nrows ? g : nrows - 1;
-    const uint8_t* src = base + (int64_t)g * rstride * row_bytes + kbyte + chunk * 16;
-    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+    // logical chunk stored at physical slot lane & 3 of local row grp * 16 + rl
+    const int chunk = (lane & 3) ^ lds_chunk_key(grp * 16 + rl);
+    const int r = min(grp * 16 + rl, last);
+    const uint32_t off = (uint32_t)r * pitch + (uint32_t)(kbyte + chunk * 16);
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(tile + off),
                                      (__attribute__((address_space(3))) void*)(lds_operand + grp * 1024), 16, 0,
                                      0);
   }
+}
+
+// Workgroup barrier without the release/acquire fences of __syncthreads(),
+// which would drain every outstanding load (vmcnt(0)) and so the stage ring.
+// Sufficient for the ring: each wave waits for its own DMA writes (vmcnt)
+// before arriving, and every LDS read it made of the retiring stage has been
+// consumed by its MFMAs.  The empty asm statements keep the compiler from
+// moving memory accesses across it.
+__device__ __forceinline__ void ring_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// s_waitcnt with only the vector-memory count bounded (gfx9 encoding: vmcnt in
+// bits 3:0 and 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their maxima)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
 template <int KIND, int BM, int BN, int WM = 2, int WN = 2>
@@ -79,14 +107,27 @@ struct Engine {
   static constexpr int THREADS = 64 * WAVES;
   static constexpr int TC = 32 * BM * WM;               // candidate rows per tile
   static constexpr int TQ = 32 * BN * WN;               // query rows per tile
-  static constexpr int STAGE_LDS = (TC + TQ) * STAGE_BYTES;
-  static constexpr int LDS = 2 * STAGE_LDS;             // double buffered
+  static constexpr int STAGE_LDS = (TC + TQ) * MFMA_SB;
+  static constexpr int LDS = MFMA_NS * STAGE_LDS;       // ring of MFMA_NS stages
   static constexpr int GROUPS = WM * BM * 2;            // pass-1 groups per tile
+  // DMA instructions each wave issues per stage
+  static constexpr int DMA_PER_STAGE = (TC + TQ) / 16 / WAVES;
   __device__ static int wave_m() { return (threadIdx.x >> 6) / WN; }
   __device__ static int wave_n() { return (threadIdx.x >> 6) % WN; }
 
+  __device__ static void issue_stage(uint8_t* lds, const uint8_t* c_rows, const uint8_t* q_rows,
+                                     const TileArgs& a, int c0, int q0, int st) {
+    uint8_t* buf = lds + (st % MFMA_NS) * STAGE_LDS;
+    dma_stage<TC, WAVES>(buf, c_rows, a.row_bytes, c0, a.n_c, a.c_stride, st * MFMA_SB);
+    dma_stage<TQ, WAVES>(buf + TC * MFMA_SB, q_rows, a.row_bytes, q0, a.n_q, 1, st * MFMA_SB);
+  }
+
   // Accumulate the TC x TQ score tile (candidates c0.., queries q0..) over all
-  // K stages.
+  // stages.  Stage st lives in ring buffer st % MFMA_NS; loads run
+  // MFMA_NS - 1 stages ahead of the MFMAs.  Iteration st waits for its own
+  // stage's loads only (newer ones stay in flight), then one barrier both
+  // publishes stage st and retires stage st - 1, whose buffer the loads of
+  // stage st + MFMA_NS - 1 then refill.
   __device__ static void score_tile(acc_t (&acc)[BM][BN], uint8_t* lds, const TileArgs& a, int c0, int q0) {
     const int lane = threadIdx.x & 63;
     const int wm = wave_m(), wn = wave_n();
@@ -94,34 +135,45 @@ struct Engine {
     for (int i = 0; i < BM; i++)
 #pragma unroll
       for (int j = 0; j < BN; j++) acc[i][j] = M::zero();
-    const int nst = (int)(a.row_bytes / STAGE_BYTES);
+    const int nst = (int)(a.row_bytes / MFMA_SB);
     // fragment rows of this lane; every block starts at a multiple of 32 rows,
     // so the swizzle key of the lane's row is the same in every block
-    const int arow = (wm * 32 * BM + (lane & 31)) * STAGE_BYTES;
-    const int brow = TC * STAGE_BYTES + (wn * 32 * BN + (lane & 31)) * STAGE_BYTES;
+    const int arow = (wm * 32 * BM + (lane & 31)) * MFMA_SB;
+    const int brow = TC * MFMA_SB + (wn * 32 * BN + (lane & 31)) * MFMA_SB;
     const int sw = lds_chunk_key(lane & 31);
-    dma_stage<TC, WAVES>(lds, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, 0);
-    dma_stage<TQ, WAVES>(lds + TC * STAGE_BYTES, a.q_rows, a.row_bytes, q0, a.n_q, 1, 0);
-    __builtin_amdgcn_s_waitcnt(0);
+    // uniform row bases, opaque to the compiler per tile: per-lane 64-bit
+    // addresses derived from them are then rebuilt from SGPRs + 32-bit offsets
+    // (saddr loads) instead of living in VGPR pairs across the tile loop
+    uint64_t cb = (uint64_t)a.c_rows, qb = (uint64_t)a.q_rows;
+    asm volatile("" : "+s"(cb), "+s"(qb));
+    const uint8_t* c_rows = (const uint8_t*)cb;
+    const uint8_t* q_rows = (const uint8_t*)qb;
+    // the ring may still be read by waves finishing the previous tile
     __syncthreads();
+    // (kept rolled: unrolled, the compiler precomputes each stage's 64-bit
+    // addresses, runs out of registers and spills them)
+#pragma unroll 1
+    for (int st = 0; st < MFMA_NS - 1; st++)
+      if (st < nst) issue_stage(lds, c_rows, q_rows, a, c0, q0, st);
     for (int s = 0; s < nst; s++) {
-      const uint8_t* cur = lds + (s & 1) * STAGE_LDS;
-      if (s + 1 < nst) {
-        uint8_t* nxt = lds + ((s + 1) & 1) * STAGE_LDS;
-        dma_stage<TC, WAVES>(nxt, a.c_rows, a.row_bytes, c0, a.n_c, a.c_stride, (s + 1) * STAGE_BYTES);
-        dma_stage<TQ, WAVES>(nxt + TC * STAGE_BYTES, a.q_rows, a.row_bytes, q0, a.n_q, 1,
-                             (s + 1) * STAGE_BYTES);
-      }
+      // stages issued after s: min(MFMA_NS - 2, nst - 1 - s)
+      const int newer = min(MFMA_NS - 2, nst - 1 - s);
+      if (newer >= 2) wait_vmcnt<2 * DMA_PER_STAGE>();
+      else if (newer == 1) wait_vmcnt<DMA_PER_STAGE>();
+      else wait_vmcnt<0>();
+      ring_barrier();
+      if (s + MFMA_NS - 1 < nst) issue_stage(lds, c_rows, q_rows, a, c0, q0, s + MFMA_NS - 1);
+      const uint8_t* cur = lds + (s % MFMA_NS) * STAGE_LDS;
       // fragments of K step ks + 1 are read from LDS while the MFMAs of step ks run
-      constexpr int KS = STAGE_BYTES / M::KSTEP_BYTES;
+      constexpr int KS = MFMA_SB / M::KSTEP_BYTES;
       typename M::frag_t fa[2][BM], fb[2][BN];
       {
         const int kb = (lane >> 5) * M::LANE_BYTES;
         const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
 #pragma unroll
-        for (int b = 0; b < BM; b++) fa[0][b] = M::load(cur + arow + off + b * 32 * STAGE_BYTES);
+        for (int b = 0; b < BM; b++) fa[0][b] = M::load(cur + arow + off + b * 32 * MFMA_SB);
 #pragma unroll
-        for (int b = 0; b < BN; b++) fb[0][b] = M::load(cur + brow + off + b * 32 * STAGE_BYTES);
+        for (int b = 0; b < BN; b++) fb[0][b] = M::load(cur + brow + off + b * 32 * MFMA_SB);
       }
 #pragma unroll
       for (int ks = 0; ks < KS; ks++) {
@@ -130,9 +182,9 @@ struct Engine {
           const int kb = (ks + 1) * M::KSTEP_BYTES + (lane >> 5) * M::LANE_BYTES;
           const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
 #pragma unroll
-          for (int b = 0; b < BM; b++) fa[p ^ 1][b] = M::load(cur + arow + off + b * 32 * STAGE_BYTES);
+          for (int b = 0; b < BM; b++) fa[p ^ 1][b] = M::load(cur + arow + off + b * 32 * MFMA_SB);
 #pragma unroll
-          for (int b = 0; b < BN; b++) fb[p ^ 1][b] = M::load(cur + brow + off + b * 32 * STAGE_BYTES);
+          for (int b = 0; b < BN; b++) fb[p ^ 1][b] = M::load(cur + brow + off + b * 32 * MFMA_SB);
         }
         // the reads above stay ahead of these MFMAs: their wait lands at the next step
         __builtin_amdgcn_sched_barrier(0);
@@ -142,9 +194,10 @@ struct Engine {
           for (int bn = 0; bn < BN; bn++) M::mma(acc[bm][bn], fa[p][bm], fb[p][bn]);
         __builtin_amdgcn_sched_barrier(0);
       }
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
     }
+    // every stage load has landed (the last iteration waited for all); saying
+    // so lets the epilogue's LDS accesses go without further vmcnt waits
+    wait_vmcnt<0>();
   }
 };
 
@@ -233,48 +286,54 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
   extern __shared__ __align__(16) uint8_t lds[];
   using En = Engine<KIND, BM, BN, WM, WN>;
   using M = Mfma<KIND>;
+  // LDS after the stage ring: per-query append counters and bounds (the bounds
+  // live here rather than in registers, which the MFMA tile needs)
   int* cnt = (int*)(lds + En::LDS);
+  float* tau_l = (float*)(cnt + En::TQ);
+  float* tfast_l = tau_l + En::TQ;
+  uint64_t* tkey_l = (uint64_t*)(tfast_l + En::TQ);
   typename En::acc_t acc[BM][BN];
   const int lane = threadIdx.x & 63;
   const int wm = En::wave_m(), wn = En::wave_n();
   int qb, split;
   if (!tile_coords(a, qb, split)) return;
   const int q0 = qb * En::TQ;
-  for (int i = threadIdx.x; i < En::TQ; i += En::THREADS) cnt[i] = 0;
-  float tau[BN], tfast[BN];
-  uint64_t tkey[BN];
-#pragma unroll
-  for (int bn = 0; bn < BN; bn++) {
-    const int q = q0 + wn * 32 * BN + bn * 32 + (lane & 31);
-    tau[bn] = (q < a.n_q && a.tau) ? a.tau[q] : -INFINITY;
-    tkey[bn] = (q < a.n_q && a.tau_key) ? a.tau_key[q] : 0ull;
-    float tb = tau[bn];
-    if (a.tau_key && tkey[bn] != EVOC_KEY_NONE) tb = fmaxf(tb, key_score(tkey[bn]));
-    tfast[bn] = fast_bound<KIND>(tb);
+  for (int i = threadIdx.x; i < En::TQ; i += En::THREADS) {
+    const int q = q0 + i;
+    const float tau = (q < a.n_q && a.tau) ? a.tau[q] : -INFINITY;
+    const uint64_t tkey = (q < a.n_q && a.tau_key) ? a.tau_key[q] : 0ull;
+    float tb = tau;
+    if (a.tau_key && tkey != EVOC_KEY_NONE) tb = fmaxf(tb, key_score(tkey));
+    cnt[i] = 0;
+    tau_l[i] = tau;
+    tfast_l[i] = fast_bound<KIND>(tb);
+    tkey_l[i] = tkey;
   }
+  // (score_tile begins with a barrier, which publishes these)
   const int t_begin = split * a.tiles_per_split;
   const int t_end = min(a.n_ctiles, t_begin + a.tiles_per_split);
   for (int t = t_begin; t < t_end; t++) {
     const int c0 = t * En::TC;
     En::score_tile(acc, lds, a, c0, q0);
 #pragma unroll
-    for (int bm = 0; bm < BM; bm++) {
+    for (int bn = 0; bn < BN; bn++) {
+      const int ql = wn * 32 * BN + bn * 32 + (lane & 31);
+      const int q = q0 + ql;
+      const int qg = q + a.q_offset;
+      const float tfast = tfast_l[ql];
 #pragma unroll
-      for (int bn = 0; bn < BN; bn++) {
+      for (int bm = 0; bm < BM; bm++) {
         const int cb = c0 + wm * 32 * BM + bm * 32;        // first candidate of the block
-        const int ql = wn * 32 * BN + bn * 32 + (lane & 31);
-        const int q = q0 + ql;
-        const int qg = q + a.q_offset;
         const bool diag = cb <= qg + 31 - (lane & 31) && qg - (lane & 31) <= cb + 31;
 #pragma unroll
         for (int j = 0; j < 16; j++) {
           const float raw = (float)acc[bm][bn][j];
-          if (diag || raw >= tfast[bn]) {
+          if (diag || raw >= tfast) {
             const int c = cb + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
             const bool self = (c == qg);
             const float s = self ? 1.0f : M::value(acc[bm][bn], j);
-            bool take = (c < a.n_c) && (q < a.n_q) && (s >= tau[bn]) && !(self && a.skip_self);
-            if (take && a.tau_key) take = make_key(s, c, self) >= tkey[bn];
+            bool take = (c < a.n_c) && (q < a.n_q) && (s >= tau_l[ql]) && !(self && a.skip_self);
+            if (take && a.tau_key) take = make_key(s, c, self) >= tkey_l[ql];
             if (take) {
               const int pos = atomicAdd(&cnt[ql], 1);
               if (pos < a.caps) a.bufs[((int64_t)split * a.n_q + q) * a.caps + pos] = make_key(s, c, self);
@@ -636,20 +695,26 @@ __global__ void finalize_kernel(const uint64_t* __restrict__ keys, int n_q, int 
 namespace evoc {
 
 // MFMA tile shapes (see Engine): 0 = 4 waves of 2x2 blocks (128 x 128 tile),
-// 1 = 4 waves of 2x4 (128 x 256), 2 = 8 waves of 2x4 (256 x 256)
-static int g_forced_shape = -2;
+// 1 = 4 waves of 2x4 (128 x 256), 2 = 8 waves of 2x4 (256 x 256).  The shape
+// in force is the calling thread's (set per search by the host, which takes it
+// from the autotuner), unless EVOC_MI355X_TILE pins one for the process.  The
+// shape never changes results: every shape accumulates each pair's products in
+// the same MFMA K order.
+static int g_env_shape = -2;
+static thread_local int t_shape = -1;
 
 int tile_shape(int kind) {
   if (kind == EVOC_ELEM_U8BITS) return -1;
-  if (g_forced_shape == -2) {
+  if (g_env_shape == -2) {
     const char* e = getenv("EVOC_MI355X_TILE");
-    g_forced_shape = e ? atoi(e) : -1;
+    g_env_shape = e ? atoi(e) : -1;
   }
-  if (g_forced_shape >= 0 && g_forced_shape <= 2) return g_forced_shape;
-  return 2;  // fastest on 100k x 768 bf16: 19.6 ms vs 22.4 (shape 0) and 24.5 (shape 1)
+  if (g_env_shape >= 0 && g_env_shape <= 2) return g_env_shape;
+  if (t_shape >= 0 && t_shape <= 2) return t_shape;
+  return 2;
 }
 
-void set_tile_shape(int shape) { g_forced_shape = shape; }
+void set_tile_shape(int shape) { t_shape = shape; }
 
 static void shape_dims(int kind, int& tc, int& tq, int& gpt) {
   if (kind == EVOC_ELEM_U8BITS) {
@@ -672,7 +737,8 @@ size_t tile_lds_bytes(int kind, int pass) {
   if (kind == EVOC_ELEM_U8BITS) return 2 * TILE * JAC_PITCH + (pass == 2 ? TILE * sizeof(int) : 0);
   int tc, tq, g;
   shape_dims(kind, tc, tq, g);
-  return (size_t)2 * (tc + tq) * STAGE_BYTES + (pass == 2 ? (size_t)tq * sizeof(int) : 0);
+  // pass 2 adds per-query counters, score bounds, fast bounds and key bounds
+  return (size_t)MFMA_NS * (tc + tq) * MFMA_SB + (pass == 2 ? (size_t)tq * (4 + 4 + 4 + 8) : 0);
 }
 
 template <class Kern>

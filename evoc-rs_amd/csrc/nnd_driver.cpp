@@ -42,6 +42,20 @@ struct Check {
 
 }  // namespace
 
+uint64_t nndescent_bytes(int kind, int64_t n, int Kg, int max_candidates, int64_t leaf_size, int64_t row_bytes,
+                         int dim) {
+  auto r = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
+  const int64_t mc = std::min(max_candidates, 64);
+  const int64_t ls = std::min<int64_t>(std::max<int64_t>(leaf_size, 2), 64);
+  const int64_t nb = (n + 1023) / 1024;
+  uint64_t b = r(n * Kg * 8) + 2 * r(n * 4) + r(n * UCAP * 8) + 2 * r(n * mc * 4) + 2 * r(n * 4) +
+               r((n + 1) * 4) + r(n * Kg * 4) + r((nb + 2) * 4) + 2 * 256;
+  const int64_t max_seg = n / (ls + 1) + 2;
+  const int64_t hyper_ld = kind == EVOC_ELEM_U8BITS ? (row_bytes * 2 + 3) / 4 : dim;
+  b += 3 * r(n * 4) + r(n) + 2 * r(max_seg * 16) + r((2 * n + 2) * 8) + 256 + r(max_seg * hyper_ld * 4);
+  return b;
+}
+
 int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, const uint32_t* pop,
                      const float* frows, int64_t frow_ld, int dim, int64_t n64, int Kg,
                      const NndParams& P, uint64_t* out_keys, NndStats* st) {

@@ -1,0 +1,237 @@
+// Serving loop: continuous batching of knn_graph requests on one MI355X.
+//
+// Requests arrive through evoc_mi355x_submit at any time.  A fixed set of
+// slots (one device context = one HIP stream + stream-ordered memory pool
+// each, all at the default priority) takes requests off the queue as soon as a
+// slot is free and the request's device memory fits the budget the running
+// requests leave.  Requests on different slots run concurrently on the device,
+// so a small request that arrives while a large one is scoring starts at once
+// instead of waiting behind it.
+//
+// Admission is FIFO with bounded bypass: if the oldest request does not fit,
+// younger requests that fit may start, but only MAX_BYPASS times before the
+// queue holds everything back for the oldest one.  A request gets its full
+// working set when that fits, otherwise at least its minimum (the exact search
+// then works in smaller query slabs); the context's budget is set to the grant.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "evoc_host.h"
+#include "evoc_mi355x.h"
+
+namespace {
+
+constexpr int MAX_BYPASS = 16;
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  uint64_t ticket = 0;
+  evoc_knn_input in{};
+  evoc_knn_options opts{};
+  int64_t tau[3] = {0, 0, 0};
+  std::vector<int64_t> trees;
+  evoc_knn_output* out = nullptr;
+  uint64_t need_full = 0, need_min = 0;
+  int bypassed = 0;
+  bool done = false;
+  int status = EVOC_ERR_PENDING;
+  std::string error;
+  Clock::time_point t_submit;
+};
+
+}  // namespace
+
+struct evoc_mi355x_server {
+  int device = 0;
+  uint64_t budget = 0;
+  uint64_t granted = 0;
+  int64_t running = 0;
+  std::vector<evoc_mi355x_ctx*> slots;
+  std::vector<std::thread> workers;
+  mutable std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> queue;
+  std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+  uint64_t next_ticket = 1;
+  bool stopping = false;
+
+  // next request to start and its grant, or nullptr (mu held)
+  Request* pick(uint64_t& grant) {
+    const uint64_t avail = budget > granted ? budget - granted : 0;
+    for (size_t i = 0; i < queue.size(); i++) {
+      Request* r = queue[i];
+      if (r->need_min <= avail) {
+        grant = std::min(r->need_full, avail);
+        // passing the oldest request counts against it
+        if (i > 0) queue.front()->bypassed++;
+        queue.erase(queue.begin() + (std::ptrdiff_t)i);
+        return r;
+      }
+      if (i == 0 && r->bypassed >= MAX_BYPASS) return nullptr;  // hold the line for it
+    }
+    return nullptr;
+  }
+
+  void work(evoc_mi355x_ctx* ctx) {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      Request* r = nullptr;
+      uint64_t grant = 0;
+      cv_work.wait(lk, [&] { return stopping || (r = pick(grant)) != nullptr; });
+      if (!r) break;  // stopping
+      granted += grant;
+      running++;
+      lk.unlock();
+      const double queued = std::chrono::duration<double>(Clock::now() - r->t_submit).count();
+      ctx->budget = grant;
+      const int rc = evoc_mi355x_knn_graph_seeded(ctx, &r->in, &r->opts, r->tau,
+                                                  r->trees.empty() ? nullptr : r->trees.data(),
+                                                  (int64_t)(r->trees.size() / 3), r->out);
+      r->out->queue_seconds = queued;
+      // a slot's pool keeps at most a share of the budget cached between
+      // requests, so cached blocks and granted bytes together stay bounded
+      (void)hipMemPoolTrimTo(ctx->pool, budget / (4 * slots.size()));
+      lk.lock();
+      granted -= grant;
+      running--;
+      r->status = rc;
+      if (rc != EVOC_OK) r->error = ctx->last_error;
+      r->done = true;
+      cv_done.notify_all();
+      cv_work.notify_all();  // memory came back: queued requests may fit now
+    }
+  }
+};
+
+extern "C" {
+
+int evoc_mi355x_server_create(int32_t device, int32_t n_slots, uint64_t hbm_budget_bytes,
+                              evoc_mi355x_server** out) {
+  if (!out) return EVOC_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (n_slots <= 0) n_slots = 4;
+  if (n_slots > 32) return EVOC_ERR_INVALID_ARGUMENT;
+  auto* s = new (std::nothrow) evoc_mi355x_server;
+  if (!s) return EVOC_ERR_OUT_OF_BUDGET;
+  s->device = device;
+  for (int i = 0; i < n_slots; i++) {
+    evoc_mi355x_ctx* c = nullptr;
+    const int rc = evoc_mi355x_create(device, 0, &c);
+    if (rc != EVOC_OK) {
+      for (auto* x : s->slots) evoc_mi355x_destroy(x);
+      delete s;
+      return rc;
+    }
+    s->slots.push_back(c);
+  }
+  // the budget is shared by all slots: by default 90% of what is free now
+  s->budget = hbm_budget_bytes ? hbm_budget_bytes : s->slots[0]->budget;
+  for (auto* c : s->slots) s->workers.emplace_back([s, c] { s->work(c); });
+  *out = s;
+  return EVOC_OK;
+}
+
+void evoc_mi355x_server_destroy(evoc_mi355x_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+    for (Request* r : s->queue) {
+      r->status = EVOC_ERR_CANCELLED;
+      r->error = "server destroyed before the request started";
+      r->done = true;
+    }
+    s->queue.clear();
+  }
+  s->cv_work.notify_all();
+  s->cv_done.notify_all();
+  for (auto& t : s->workers) t.join();
+  for (auto* c : s->slots) evoc_mi355x_destroy(c);
+  delete s;
+}
+
+int evoc_mi355x_submit(evoc_mi355x_server* s, const evoc_knn_input* in, const evoc_knn_options* opts,
+                       const int64_t tau_state[3], const int64_t* tree_states, int64_t n_tree_rows,
+                       evoc_knn_output* out, uint64_t* ticket) {
+  if (!s || !in || !opts || !tau_state || !out || !ticket) return EVOC_ERR_INVALID_ARGUMENT;
+  if (opts->n_neighbors <= 0) return EVOC_ERR_INVALID_NEIGHBORS;
+  if (in->n <= 0) return EVOC_ERR_EMPTY_DATA;
+  if (in->d <= 0 || in->ld < in->d || !in->data || in->dtype < EVOC_DTYPE_F32 || in->dtype > EVOC_DTYPE_U8)
+    return EVOC_ERR_INVALID_ARGUMENT;
+  auto r = std::make_unique<Request>();
+  r->in = *in;
+  r->opts = *opts;
+  std::memcpy(r->tau, tau_state, sizeof(r->tau));
+  if (tree_states && n_tree_rows > 0) r->trees.assign(tree_states, tree_states + 3 * n_tree_rows);
+  r->out = out;
+  const int rc = evoc_mi355x_request_bytes(s->slots[0], in, opts, out->location, &r->need_full, &r->need_min);
+  if (rc != EVOC_OK) return rc;
+  // DevBuf rounding and small buffers of the search: keep 1% + 4 MiB headroom
+  r->need_full += r->need_full / 100 + (4u << 20);
+  r->need_min += r->need_min / 100 + (4u << 20);
+  if (r->need_min > s->budget) return EVOC_ERR_OUT_OF_BUDGET;
+  r->t_submit = Clock::now();
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stopping) return EVOC_ERR_CANCELLED;
+  r->ticket = s->next_ticket++;
+  *ticket = r->ticket;
+  s->queue.push_back(r.get());
+  s->reqs.emplace(r->ticket, std::move(r));
+  s->cv_work.notify_one();
+  return EVOC_OK;
+}
+
+int evoc_mi355x_poll(evoc_mi355x_server* s, uint64_t ticket) {
+  if (!s) return EVOC_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return EVOC_ERR_INVALID_ARGUMENT;
+  return it->second->done ? it->second->status : EVOC_ERR_PENDING;
+}
+
+int evoc_mi355x_wait(evoc_mi355x_server* s, uint64_t ticket, double timeout_seconds, char* err,
+                     int64_t err_len) {
+  if (!s) return EVOC_ERR_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return EVOC_ERR_INVALID_ARGUMENT;
+  Request* r = it->second.get();
+  auto ready = [r] { return r->done; };
+  if (timeout_seconds < 0) {
+    s->cv_done.wait(lk, ready);
+  } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_seconds), ready)) {
+    return EVOC_ERR_PENDING;
+  }
+  const int status = r->status;
+  if (err && err_len > 0) {
+    const size_t m = std::min<size_t>((size_t)err_len - 1, r->error.size());
+    std::memcpy(err, r->error.data(), m);
+    err[m] = 0;
+  }
+  s->reqs.erase(it);
+  return status;
+}
+
+void evoc_mi355x_server_stats(const evoc_mi355x_server* s, int64_t* queued, int64_t* running,
+                              uint64_t* granted_bytes, uint64_t* budget_bytes) {
+  if (!s) return;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (queued) *queued = (int64_t)s->queue.size();
+  if (running) *running = s->running;
+  if (granted_bytes) *granted_bytes = s->granted;
+  if (budget_bytes) *budget_bytes = s->budget;
+}
+
+}  // extern "C"

@@ -37,6 +37,7 @@ ERR_HIP = 6
 ERR_CANCELLED = 7
 ERR_NOT_BITWISE = 8
 ERR_NO_DEVICE = 9
+ERR_PENDING = 10
 
 _DTYPES = {np.dtype(np.float32): DTYPE_F32, np.dtype(np.int8): DTYPE_I8, np.dtype(np.uint8): DTYPE_U8}
 
@@ -84,7 +85,8 @@ class _Output(C.Structure):
     _fields_ = [("location", C.c_int32), ("reserved0", C.c_int32), ("indices", C.c_void_p),
                 ("distances", C.c_void_p), ("capacity_k", C.c_int64), ("k_out", C.c_int64),
                 ("seconds", C.c_double), ("device_seconds", C.c_double), ("score_seconds", C.c_double),
-                ("iterations", C.c_int64), ("rescore_rounds", C.c_int64), ("offers_dropped", C.c_int64)]
+                ("iterations", C.c_int64), ("rescore_rounds", C.c_int64), ("offers_dropped", C.c_int64),
+                ("queue_seconds", C.c_double)]
 
 
 _lib = None
@@ -104,6 +106,18 @@ SIGNATURES = {
                               C.c_int),
     "evoc_mi355x_knn_graph_seeded": ([_P, C.POINTER(_Input), C.POINTER(_Options), _P, _P, C.c_int64,
                                       C.POINTER(_Output)], C.c_int),
+    "evoc_mi355x_request_bytes": ([_P, C.POINTER(_Input), C.POINTER(_Options), C.c_int32,
+                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+    "evoc_mi355x_autotune": ([_P, C.POINTER(_Input), C.POINTER(_Options), C.c_int64, C.POINTER(C.c_int32),
+                              C.POINTER(C.c_double)], C.c_int),
+    "evoc_mi355x_server_create": ([C.c_int32, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
+    "evoc_mi355x_server_destroy": ([_P], None),
+    "evoc_mi355x_submit": ([_P, C.POINTER(_Input), C.POINTER(_Options), _P, _P, C.c_int64,
+                            C.POINTER(_Output), C.POINTER(C.c_uint64)], C.c_int),
+    "evoc_mi355x_poll": ([_P, C.c_uint64], C.c_int),
+    "evoc_mi355x_wait": ([_P, C.c_uint64, C.c_double, C.c_char_p, C.c_int64], C.c_int),
+    "evoc_mi355x_server_stats": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
+                                  C.POINTER(C.c_uint64)], None),
     "evoc_numpy_rng_new": ([C.c_uint32], _P),
     "evoc_numpy_rng_free": ([_P], None),
     "evoc_numpy_rng_randint": ([_P, C.c_int64, C.c_int64], C.c_int64),
@@ -301,7 +315,31 @@ class CallStats:
     iterations: int = 0
     rescore_rounds: int = 0
     offers_dropped: int = 0
+    queue_seconds: float = 0.0
     extra: dict = field(default_factory=dict)
+
+
+def _sync_torch(*tensors) -> None:
+    """The library runs on its own streams: finish the caller's pending torch
+    work on any device tensor it is handed."""
+    for t in tensors:
+        if t is not None and type(t).__module__.startswith("torch") and t.is_cuda:
+            import torch
+
+            torch.cuda.current_stream(t.device).synchronize()
+            return
+
+
+def _fill_stats(stats, res) -> None:
+    if stats is None:
+        return
+    stats.seconds = res.seconds
+    stats.device_seconds = res.device_seconds
+    stats.score_seconds = res.score_seconds
+    stats.iterations = res.iterations
+    stats.rescore_rounds = res.rescore_rounds
+    stats.offers_dropped = res.offers_dropped
+    stats.queue_seconds = res.queue_seconds
 
 
 def _input_from(data, dtype_code=None):
@@ -365,17 +403,12 @@ def knn_graph(data, options: KnnGraphOptions, rng: NumpyRandomState,
         res.indices = C.c_void_p(idx.ctypes.data)
         res.distances = C.c_void_p(dist.ctypes.data)
     res.capacity_k = k
+    _sync_torch(keep, *(out or ()))
     rc = lib().evoc_mi355x_knn_graph(ctx.handle, C.byref(inp), C.byref(o), rng._h, C.byref(res))
     del keep
     if rc != OK:
         _raise(rc, ctx.handle)
-    if stats is not None:
-        stats.seconds = res.seconds
-        stats.device_seconds = res.device_seconds
-        stats.score_seconds = res.score_seconds
-        stats.iterations = res.iterations
-        stats.rescore_rounds = res.rescore_rounds
-        stats.offers_dropped = res.offers_dropped
+    _fill_stats(stats, res)
     if out is not None:
         return out
     return idx, dist
@@ -405,6 +438,7 @@ def knn_graph_seeded(data, options: KnnGraphOptions, tau_state, tree_states=None
         res.indices = C.c_void_p(idx.ctypes.data)
         res.distances = C.c_void_p(dist.ctypes.data)
     res.capacity_k = k
+    _sync_torch(keep, *(out or ()))
     rc = lib().evoc_mi355x_knn_graph_seeded(
         ctx.handle, C.byref(inp), C.byref(o), C.c_void_p(tau.ctypes.data),
         C.c_void_p(trees.ctypes.data) if trees is not None else None,
@@ -412,13 +446,148 @@ def knn_graph_seeded(data, options: KnnGraphOptions, tau_state, tree_states=None
     del keep
     if rc != OK:
         _raise(rc, ctx.handle)
-    if stats is not None:
-        stats.seconds = res.seconds
-        stats.device_seconds = res.device_seconds
-        stats.score_seconds = res.score_seconds
-        stats.iterations = res.iterations
-        stats.rescore_rounds = res.rescore_rounds
-        stats.offers_dropped = res.offers_dropped
+    _fill_stats(stats, res)
     if out is not None:
         return out
     return idx, dist
+
+
+def request_bytes(data, options: KnnGraphOptions, out_on_device: bool = False,
+                  ctx: Optional[Context] = None):
+    """(peak, minimum) device bytes of a knn_graph call (evoc_mi355x_request_bytes)."""
+    ctx = ctx or default_context()
+    inp, keep = _input_from(data)
+    o = options._c()
+    full, low = C.c_uint64(), C.c_uint64()
+    rc = lib().evoc_mi355x_request_bytes(ctx.handle, C.byref(inp), C.byref(o),
+                                         MEM_DEVICE if out_on_device else MEM_HOST,
+                                         C.byref(full), C.byref(low))
+    del keep
+    if rc != OK:
+        raise BackendError(rc, "invalid request")
+    return int(full.value), int(low.value)
+
+
+def autotune(data, options: KnnGraphOptions, max_rows: int = 65536, ctx: Optional[Context] = None):
+    """Pick the fastest MFMA tile shape for searches like knn_graph(data, options)
+    (evoc_mi355x_autotune).  Returns {"shape": s, "ms": [t0, t1, t2]}."""
+    ctx = ctx or default_context()
+    inp, keep = _input_from(data)
+    _sync_torch(keep)
+    o = options._c()
+    best = C.c_int32(-1)
+    ms = (C.c_double * 3)()
+    rc = lib().evoc_mi355x_autotune(ctx.handle, C.byref(inp), C.byref(o), max_rows, C.byref(best), ms)
+    del keep
+    if rc != OK:
+        _raise(rc, ctx.handle)
+    return {"shape": best.value, "ms": [float(v) for v in ms]}
+
+
+class Ticket:
+    """A submitted request: keeps its buffers alive until it is waited for."""
+
+    def __init__(self, server, ticket, res, keep, result):
+        self.server, self.id, self._res, self._keep, self._result = server, ticket, res, keep, result
+        self.status = None
+        self.stats = CallStats()
+        self.submitted = None
+
+    def poll(self) -> bool:
+        """True once the request has finished (successfully or not)."""
+        if self.status is not None:
+            return True
+        return lib().evoc_mi355x_poll(self.server._h, self.id) != ERR_PENDING
+
+    def wait(self, timeout: float = -1.0):
+        """Block until done; returns the result (numpy pair or the caller's out tensors)."""
+        if self.status is None:
+            buf = C.create_string_buffer(1024)
+            rc = lib().evoc_mi355x_wait(self.server._h, self.id, timeout, buf, 1024)
+            if rc == ERR_PENDING:
+                raise TimeoutError(f"request {self.id} still running after {timeout} s")
+            self.status = rc
+            self._msg = buf.value.decode()
+            _fill_stats(self.stats, self._res)
+            self._keep = None
+        if self.status == ERR_EMPTY_DATA:
+            raise EmptyData(self.status, self._msg or "empty embedding matrix")
+        if self.status == ERR_INVALID_NEIGHBORS:
+            raise InvalidNeighbors(self.status, self._msg or "n_neighbors must be positive")
+        if self.status != OK:
+            raise BackendError(self.status, self._msg or "mi355x serving error")
+        return self._result
+
+
+class Server:
+    """Continuous-batching request loop on one GPU (evoc_mi355x_server_*).
+
+    `slots` requests run concurrently on their own HIP streams; further
+    requests queue and are admitted as slots and HBM budget free up.
+    """
+
+    def __init__(self, device: int = 0, slots: int = 4, hbm_budget_bytes: int = 0):
+        h = C.c_void_p()
+        rc = lib().evoc_mi355x_server_create(device, slots, hbm_budget_bytes, C.byref(h))
+        if rc != OK:
+            raise BackendError(rc, f"evoc_mi355x_server_create(device={device}) failed with status {rc}")
+        self._h = h
+
+    def submit(self, data, options: KnnGraphOptions, rng: Optional[NumpyRandomState] = None,
+               tau_state=None, tree_states=None, out=None) -> Ticket:
+        """Queue knn_graph(data, options, rng).  With `rng`, its draws are made
+        here, in submission order, exactly as knn_graph would make them."""
+        inp, keep = _input_from(data)
+        n = inp.n
+        if options.n_neighbors <= 0:
+            raise InvalidNeighbors(ERR_INVALID_NEIGHBORS, "n_neighbors must be positive")
+        if n <= 0:
+            raise EmptyData(ERR_EMPTY_DATA, "empty embedding matrix")
+        if tau_state is None:
+            if rng is None:
+                raise ValueError("submit needs rng or tau_state")
+            draws = rng_draws(inp.dtype, n, options)
+            vals = [rng.randint(-2147483647, 2147483646) for _ in range(draws)]
+            tau_state, tree_states = vals[:3], (np.array(vals[3:], np.int64).reshape(-1, 3)
+                                                if draws > 3 else None)
+        tau = np.ascontiguousarray(tau_state, np.int64)
+        trees = None if tree_states is None else np.ascontiguousarray(tree_states, np.int64)
+        k = effective_k(options.n_neighbors, n)
+        res = _Output()
+        if out is not None:
+            idx_t, dist_t = out
+            res.location = MEM_DEVICE if idx_t.is_cuda else MEM_HOST
+            res.indices = C.c_void_p(idx_t.data_ptr())
+            res.distances = C.c_void_p(dist_t.data_ptr())
+            result = out
+        else:
+            idx = np.empty((n, k), np.int32)
+            dist = np.empty((n, k), np.float32)
+            res.location = MEM_HOST
+            res.indices = C.c_void_p(idx.ctypes.data)
+            res.distances = C.c_void_p(dist.ctypes.data)
+            result = (idx, dist)
+        res.capacity_k = k
+        o = options._c()
+        _sync_torch(keep, *(out or ()))
+        t = C.c_uint64()
+        rc = lib().evoc_mi355x_submit(self._h, C.byref(inp), C.byref(o), C.c_void_p(tau.ctypes.data),
+                                      C.c_void_p(trees.ctypes.data) if trees is not None else None,
+                                      0 if trees is None else trees.shape[0], C.byref(res), C.byref(t))
+        if rc != OK:
+            raise BackendError(rc, f"submit failed with status {rc}")
+        return Ticket(self, t.value, res, (keep, res, tau, trees, o), result)
+
+    def stats(self):
+        q, r, g, b = C.c_int64(), C.c_int64(), C.c_uint64(), C.c_uint64()
+        lib().evoc_mi355x_server_stats(self._h, C.byref(q), C.byref(r), C.byref(g), C.byref(b))
+        return {"queued": q.value, "running": r.value, "granted_bytes": g.value, "budget_bytes": b.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().evoc_mi355x_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define EVOC_MI355X_ABI_VERSION 1
+#define EVOC_MI355X_ABI_VERSION 2
 
 /* EmbeddingData::{Float32, Int8, UInt8}, src/knn.rs:16-20.  UInt8 rows are
  * packed bits compared with bitwise Jaccard; d counts bytes. */
@@ -52,7 +52,8 @@ enum evoc_status {
   EVOC_ERR_HIP = 6,               /* HIP runtime error (message in last_error) */
   EVOC_ERR_CANCELLED = 7,
   EVOC_ERR_NOT_BITWISE = 8,       /* strict_precision requested: results are not bitwise the CPU path's */
-  EVOC_ERR_NO_DEVICE = 9
+  EVOC_ERR_NO_DEVICE = 9,
+  EVOC_ERR_PENDING = 10           /* serving: the request has not finished yet */
 };
 
 /* KnnGraphOptions, src/knn.rs:43-76, plus the mi355x extensions.
@@ -101,6 +102,7 @@ typedef struct evoc_knn_output {
   int64_t iterations;      /* NN-descent iterations run (0 for exact) */
   int64_t rescore_rounds;  /* exact: extra scoring rounds for overflowing buffers */
   int64_t offers_dropped;  /* NN-descent: candidate offers lost to full per-vertex buffers */
+  double queue_seconds;    /* serving: time from submit until a slot started the request */
 } evoc_knn_output;
 
 typedef struct evoc_mi355x_ctx evoc_mi355x_ctx;
@@ -136,6 +138,55 @@ int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
                                  const evoc_knn_options* opts, const int64_t tau_state[3],
                                  const int64_t* tree_states, int64_t n_tree_rows,
                                  evoc_knn_output* out);
+
+/* Device bytes a call with these arguments holds at its peak (`full`) and the
+ * least it can run in (`minimum`: exact search in small query slabs).  No
+ * counterpart in the reference; used for admission by HBM budget. */
+int evoc_mi355x_request_bytes(const evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
+                              const evoc_knn_options* opts, int32_t out_location,
+                              uint64_t* full, uint64_t* minimum);
+
+/* Tile-shape autotuner.  No counterpart in the reference.  Times the exact
+ * search on the first min(n, max_rows) rows of `in` (max_rows <= 0: all) once
+ * per MFMA tile shape (0: 128x128, 1: 128x256, 2: 256x256) and makes the
+ * fastest the shape of every later exact search of the same element kind, row
+ * count (power-of-two bucket) and row width in this process.  ms[3] receives
+ * the device milliseconds per shape (may be NULL).  Binary rows have a single
+ * engine: *best_shape = -1.  Shapes change speed only, never results. */
+int evoc_mi355x_autotune(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const evoc_knn_options* opts,
+                         int64_t max_rows, int32_t* best_shape, double ms[3]);
+
+/* ---- serving: continuous batching of knn_graph requests ----
+ * No counterpart in the reference (its knn_graph is one blocking call,
+ * src/knn.rs:345-359).  A server owns `slots` device executors, each with its
+ * own HIP stream, on one GPU.  Requests queue in arrival order and are admitted
+ * whenever a slot is free and their device memory fits what the running
+ * requests leave of the HBM budget; a request that does not fit may be passed
+ * by smaller ones a bounded number of times.  Running requests overlap on the
+ * device.  Each request computes exactly what evoc_mi355x_knn_graph_seeded
+ * computes for it alone.  Input and output buffers stay owned by the caller and
+ * must stay valid until the request is waited for. */
+typedef struct evoc_mi355x_server evoc_mi355x_server;
+/* slots <= 0: 4; hbm_budget_bytes = 0: 90% of free device memory */
+int evoc_mi355x_server_create(int32_t device, int32_t slots, uint64_t hbm_budget_bytes,
+                              evoc_mi355x_server** out);
+/* waits for running requests; queued ones finish with EVOC_ERR_CANCELLED */
+void evoc_mi355x_server_destroy(evoc_mi355x_server* srv);
+/* copies *in, *opts, tau_state and tree_states; `out` must stay valid */
+int evoc_mi355x_submit(evoc_mi355x_server* srv, const evoc_knn_input* in, const evoc_knn_options* opts,
+                       const int64_t tau_state[3], const int64_t* tree_states, int64_t n_tree_rows,
+                       evoc_knn_output* out, uint64_t* ticket);
+/* EVOC_ERR_PENDING while the request runs or waits, else its final status
+ * (the ticket stays valid until evoc_mi355x_wait returns it) */
+int evoc_mi355x_poll(evoc_mi355x_server* srv, uint64_t ticket);
+/* blocks up to timeout_seconds (< 0: forever); returns EVOC_ERR_PENDING on
+ * timeout, else the request's status, copies its error message into err
+ * (err_len bytes, may be NULL) and releases the ticket */
+int evoc_mi355x_wait(evoc_mi355x_server* srv, uint64_t ticket, double timeout_seconds, char* err,
+                     int64_t err_len);
+/* requests queued / running and bytes granted to running requests */
+void evoc_mi355x_server_stats(const evoc_mi355x_server* srv, int64_t* queued, int64_t* running,
+                              uint64_t* granted_bytes, uint64_t* budget_bytes);
 
 /* ---- NumPy RandomState (src/numpy_rng.rs:12-226), for C / Python callers ---- */
 evoc_numpy_rng* evoc_numpy_rng_new(uint32_t seed);        /* check_random_state(Some(seed)) */
