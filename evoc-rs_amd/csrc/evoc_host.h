@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -17,6 +18,17 @@ struct evoc_mi355x_ctx {
   // stream-ordered device memory of this context: frees do not synchronise the
   // device, so contexts on other streams keep running (serving slots)
   hipMemPool_t pool = nullptr;
+  // serving slot: searches work in bounded query slabs (host syncs between
+  // them are where other slots' kernels get the device), smaller while
+  // `contention` (requests queued or running beside this one) is non-zero
+  bool serving = false;
+  const std::atomic<int>* contention = nullptr;
+  // serving slot: pinned staging ring for host inputs and outputs, so their
+  // copies are asynchronous DMA on this stream (pageable copies go through
+  // the runtime's own staging and can wait behind other streams' work)
+  void* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  size_t stage_bytes = 0;
   std::mutex mu;
   std::string last_error;
 
@@ -39,6 +51,14 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf();
 };
+
+// Host <-> device copies through the context's pinned staging ring (serving
+// slots); both return when the copy is complete.  staged_download writes
+// `rows` rows of `row_bytes` from a packed device array into host rows
+// `dst_pitch` bytes apart.
+hipError_t staged_upload(Ctx* c, void* dst, const void* src, size_t bytes);
+hipError_t staged_download(Ctx* c, void* dst, size_t dst_pitch, const void* src, size_t row_bytes, size_t rows);
+int enable_staging(Ctx* c, size_t bytes);
 
 struct ExactStats {
   int64_t extra_rounds = 0;  // re-scoring rounds caused by buffer overflow
@@ -92,7 +112,9 @@ int choose_tile_shape(int kind, int64_t n, int64_t row_bytes);
 // scratch (called with the search's tile shape in force), and the scratch
 // bytes of the whole search in one slab or in the smallest slab.
 int64_t exact_slab(int kind, int64_t n, int K, int n_cu, uint64_t avail);
-uint64_t exact_scratch_bytes(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, bool minimum);
+int64_t time_slab(int kind, int64_t n, int64_t row_bytes, double ms);
+uint64_t exact_scratch_bytes(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, bool minimum,
+                             bool serving);
 
 // Exact K best candidates per query over all rows (see knn_exact.hip).
 // rows: n x row_bytes scored rows; pop: per-row popcounts (binary only).

@@ -149,6 +149,10 @@ evoc_mi355x_ctx::~evoc_mi355x_ctx() {
       (void)hipStreamDestroy(stream);
     }
     if (pool) (void)hipMemPoolDestroy(pool);
+    for (int i = 0; i < 2; i++) {
+      if (stage[i]) (void)hipHostFree(stage[i]);
+      if (stage_ev[i]) (void)hipEventDestroy(stage_ev[i]);
+    }
   }
 }
 
@@ -193,6 +197,67 @@ int DevBuf::alloc(Ctx* c, size_t nbytes, const char* what) {
   bytes = nbytes;
   c->in_use += nbytes;
   return EVOC_OK;
+}
+
+int enable_staging(Ctx* c, size_t bytes) {
+  if (c->stage_bytes) return EVOC_OK;
+  if (hipSetDevice(c->device) != hipSuccess) return EVOC_ERR_HIP;
+  for (int i = 0; i < 2; i++) {
+    if (hipHostMalloc(&c->stage[i], bytes, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      return EVOC_ERR_HIP;
+    }
+  }
+  c->stage_bytes = bytes;
+  return EVOC_OK;
+}
+
+hipError_t staged_upload(Ctx* c, void* dst, const void* src, size_t bytes) {
+  const size_t CH = c->stage_bytes;
+  hipError_t e = hipSuccess;
+  size_t i = 0;
+  for (size_t off = 0; off < bytes && e == hipSuccess; off += CH, i++) {
+    const int b = (int)(i & 1);
+    const size_t len = std::min(CH, bytes - off);
+    if (i >= 2) e = hipEventSynchronize(c->stage_ev[b]);  // the DMA that last read this buffer
+    if (e != hipSuccess) break;
+    std::memcpy(c->stage[b], (const uint8_t*)src + off, len);
+    e = hipMemcpyAsync((uint8_t*)dst + off, c->stage[b], len, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->stage_ev[b], c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e;
+}
+
+hipError_t staged_download(Ctx* c, void* dst, size_t dst_pitch, const void* src, size_t row_bytes, size_t rows) {
+  const size_t per = std::max<size_t>(1, c->stage_bytes / row_bytes);
+  hipError_t e = hipSuccess;
+  size_t r0 = 0, pending_r0 = 0, pending_n = 0;
+  int b = 0, pending_b = 0;
+  auto drain = [&]() {
+    e = hipEventSynchronize(c->stage_ev[pending_b]);
+    if (e != hipSuccess) return;
+    for (size_t r = 0; r < pending_n; r++)
+      std::memcpy((uint8_t*)dst + (pending_r0 + r) * dst_pitch, (const uint8_t*)c->stage[pending_b] + r * row_bytes,
+                  row_bytes);
+    pending_n = 0;
+  };
+  while (r0 < rows && e == hipSuccess) {
+    const size_t nr = std::min(per, rows - r0);
+    e = hipMemcpyAsync(c->stage[b], (const uint8_t*)src + r0 * row_bytes, nr * row_bytes, hipMemcpyDeviceToHost,
+                       c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->stage_ev[b], c->stream);
+    if (e != hipSuccess) break;
+    if (pending_n) drain();  // the previous chunk, while this one is in flight
+    pending_r0 = r0;
+    pending_n = nr;
+    pending_b = b;
+    b ^= 1;
+    r0 += nr;
+  }
+  if (e == hipSuccess && pending_n) drain();
+  return e;
 }
 
 extern "C" {
@@ -397,8 +462,8 @@ extern "C" int evoc_mi355x_request_bytes(const evoc_mi355x_ctx* ctx, const evoc_
                                   c.row_bytes, (int)c.d);
   } else {
     const int K = (int)std::min<int64_t>(c.K, 1024);
-    hi = f + exact_scratch_bytes(c.kind, c.n, c.row_bytes, K, ctx->n_cu, false);
-    lo = f + exact_scratch_bytes(c.kind, c.n, c.row_bytes, K, ctx->n_cu, true);
+    hi = f + exact_scratch_bytes(c.kind, c.n, c.row_bytes, K, ctx->n_cu, false, ctx->serving);
+    lo = f + exact_scratch_bytes(c.kind, c.n, c.row_bytes, K, ctx->n_cu, true, ctx->serving);
   }
   if (full) *full = hi;
   if (minimum) *minimum = lo;
@@ -461,7 +526,8 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
     const size_t span = (size_t)cs.in_span * elem_size(dtype);
     int rc = src.alloc(ctx, span, "input copy");
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(src.ptr, in->data, span, hipMemcpyHostToDevice, st);
+    hipError_t e = ctx->stage_bytes ? staged_upload(ctx, src.ptr, in->data, span)
+                                    : hipMemcpyAsync(src.ptr, in->data, span, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return ctx->hip_fail(e, "input upload");
     d_src = (const uint8_t*)src.ptr;
   }
@@ -569,7 +635,12 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
   e = launch_finalize(final_keys, (int)n, 0, (int)k, dtype, d_idx, d_dist, st);
   if (e != hipSuccess) return ctx->hip_fail(e, "finalize");
   (void)hipEventRecord(ctx->ev_end, st);
-  if (!dev_out) {
+  if (!dev_out && out->location != EVOC_MEM_DEVICE && ctx->stage_bytes) {
+    e = staged_download(ctx, out->indices, (size_t)out->capacity_k * 4, d_idx, (size_t)k * 4, (size_t)n);
+    if (e == hipSuccess)
+      e = staged_download(ctx, out->distances, (size_t)out->capacity_k * 4, d_dist, (size_t)k * 4, (size_t)n);
+    if (e != hipSuccess) return ctx->hip_fail(e, "output copy");
+  } else if (!dev_out) {
     const hipMemcpyKind kd = out->location == EVOC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     e = hipMemcpy2DAsync(out->indices, (size_t)out->capacity_k * 4, d_idx, (size_t)k * 4, (size_t)k * 4,
                          (size_t)n, kd, st);

@@ -123,6 +123,23 @@ struct ShapeScope {
 };
 }  // namespace
 
+// Query rows whose all-pairs scoring takes about `ms` milliseconds at the
+// sustained rate of the element kind's scorer (measured on the MI355X, rounded
+// down), in whole query tiles, at least four of them.
+int64_t time_slab(int kind, int64_t n, int64_t row_bytes, double ms) {
+  double rate, elem;  // pair-element operations per second, bytes per element
+  switch (kind) {
+    case EVOC_ELEM_BF16: rate = 7e14; elem = 2; break;
+    case EVOC_ELEM_F32: rate = 1.1e14; elem = 4; break;
+    case EVOC_ELEM_I8: rate = 1.2e15; elem = 1; break;
+    default: rate = 2e13; elem = 1.0 / 8; break;  // binary: bits, AND + popcount per 32-bit word
+  }
+  const int64_t TQ = tile_rows_q(kind);
+  const double per_row = 2.0 * (double)n * ((double)row_bytes / elem);
+  const int64_t rows = (int64_t)(ms * 1e-3 * rate / per_row);
+  return std::max<int64_t>(4 * TQ, rows / TQ * TQ);
+}
+
 int64_t exact_slab(int kind, int64_t n, int K, int n_cu, uint64_t avail) {
   const int TQ = tile_rows_q(kind);
   int64_t slab = n;
@@ -133,11 +150,23 @@ int64_t exact_slab(int kind, int64_t n, int K, int n_cu, uint64_t avail) {
   }
 }
 
-uint64_t exact_scratch_bytes(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, bool minimum) {
+static uint64_t slab_scratch(int kind, int64_t n, int64_t nq, int K, int n_cu) {
+  const int TQ = tile_rows_q(kind);
+  nq = std::min(nq, n);
+  return make_plan(kind, n, nq, K, n_cu).scratch_bytes(nq, cdiv(nq, TQ));
+}
+
+uint64_t exact_scratch_bytes(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, bool minimum,
+                             bool serving) {
   ShapeScope shape_scope(choose_tile_shape(kind, n, row_bytes));
   const int TQ = tile_rows_q(kind);
-  const int64_t nq = std::min<int64_t>(minimum ? 4 * (int64_t)TQ : n, n);
-  return make_plan(kind, n, nq, K, n_cu).scratch_bytes(nq, cdiv(nq, TQ));
+  if (minimum) return slab_scratch(kind, n, 4 * (int64_t)TQ, K, n_cu);
+  // the plan's bytes per query are not monotone in the slab size (more splits
+  // for smaller slabs): take the largest over the slabs the search may use
+  uint64_t b = slab_scratch(kind, n, n, K, n_cu);
+  if (serving)
+    for (double ms : {8.0, 2.0}) b = std::max(b, slab_scratch(kind, n, time_slab(kind, n, row_bytes, ms), K, n_cu));
+  return b;
 }
 
 int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, const uint32_t* pop,
@@ -150,14 +179,25 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
   // largest slab of queries whose scratch fits the rest of the budget (the
   // caller has allocated its outputs already)
   const uint64_t avail = ctx->budget > ctx->in_use ? ctx->budget - ctx->in_use : 0;
-  const int64_t slab = exact_slab(kind, n, K, ctx->n_cu, avail);
+  const int64_t mem_slab = exact_slab(kind, n, K, ctx->n_cu, avail);
   const int64_t n_ctiles = cdiv(n, TC);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   int rc = EVOC_OK;
-  for (int64_t q0 = 0; q0 < n && rc == EVOC_OK; q0 += slab) {
-    const int64_t nq = std::min(slab, n - q0);
+  int64_t nq = 0;
+  for (int64_t q0 = 0; q0 < n && rc == EVOC_OK; q0 += nq) {
+    int64_t slab = mem_slab;
+    if (ctx->serving) {
+      // about 8 ms of scoring per slab, 2 ms while other requests wait or run
+      const bool busy = ctx->contention && ctx->contention->load(std::memory_order_relaxed) > 0;
+      slab = std::min(slab, time_slab(kind, n, row_bytes, busy ? 2.0 : 8.0));
+    }
+    // a slab whose plan needs more than what is left is cut further
+    const uint64_t left = ctx->budget > ctx->in_use ? ctx->budget - ctx->in_use : 0;
+    while (slab > 4 * TQ && slab_scratch(kind, n, std::min(slab, n - q0), K, ctx->n_cu) > left)
+      slab = std::max<int64_t>(4 * TQ, cdiv(slab / 2, TQ) * TQ);
+    nq = std::min(slab, n - q0);
     const int64_t nqb = cdiv(nq, TQ);
     const Plan p = make_plan(kind, n, nq, K, ctx->n_cu);
     DevBuf gmax, tau, bufs, counts, ovf, tkey, list;

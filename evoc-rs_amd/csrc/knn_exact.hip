@@ -315,28 +315,53 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
   for (int t = t_begin; t < t_end; t++) {
     const int c0 = t * En::TC;
     En::score_tile(acc, lds, a, c0, q0);
+    // lane / wave indices made opaque here: what the epilogue derives from
+    // them is recomputed per tile instead of living in registers through the
+    // MFMA loop (which needs them all)
+    int lane_e = lane, wm_e = wm, wn_e = wn;
+    asm volatile("" : "+v"(lane_e), "+v"(wm_e), "+v"(wn_e));
 #pragma unroll
     for (int bn = 0; bn < BN; bn++) {
-      const int ql = wn * 32 * BN + bn * 32 + (lane & 31);
+      const int ql = wn_e * 32 * BN + bn * 32 + (lane_e & 31);
       const int q = q0 + ql;
       const int qg = q + a.q_offset;
       const float tfast = tfast_l[ql];
 #pragma unroll
       for (int bm = 0; bm < BM; bm++) {
-        const int cb = c0 + wm * 32 * BM + bm * 32;        // first candidate of the block
-        const bool diag = cb <= qg + 31 - (lane & 31) && qg - (lane & 31) <= cb + 31;
+        const int cb = c0 + wm_e * 32 * BM + bm * 32;      // first candidate of the block
+        const bool diag = cb <= qg + 31 - (lane_e & 31) && qg - (lane_e & 31) <= cb + 31;
+        // values of this lane's 16 candidates that pass the cheap bound
+        uint32_t m = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) m |= ((float)acc[bm][bn][j] >= tfast ? 1u : 0u) << j;
+        if (diag) m = 0xffffu;  // the self pair takes the exact path
+        if (__ballot(m != 0) == 0) continue;
+        // the exact test on those values
+        const float tau = tau_l[ql];
+        const uint64_t tkey = tkey_l[ql];
+        uint32_t take = 0;
 #pragma unroll
         for (int j = 0; j < 16; j++) {
-          const float raw = (float)acc[bm][bn][j];
-          if (diag || raw >= tfast) {
-            const int c = cb + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+          if (m & (1u << j)) {
+            const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
             const bool self = (c == qg);
-            const float s = self ? 1.0f : M::value(acc[bm][bn], j);
-            bool take = (c < a.n_c) && (q < a.n_q) && (s >= tau_l[ql]) && !(self && a.skip_self);
-            if (take && a.tau_key) take = make_key(s, c, self) >= tkey_l[ql];
-            if (take) {
-              const int pos = atomicAdd(&cnt[ql], 1);
-              if (pos < a.caps) a.bufs[((int64_t)split * a.n_q + q) * a.caps + pos] = make_key(s, c, self);
+            const float sc = self ? 1.0f : M::value(acc[bm][bn], j);
+            bool t = (c < a.n_c) && (q < a.n_q) && (sc >= tau) && !(self && a.skip_self);
+            if (t && a.tau_key) t = make_key(sc, c, self) >= tkey;
+            take |= (t ? 1u : 0u) << j;
+          }
+        }
+        // one slot reservation per lane and block, then the appends
+        if (take) {
+          int pos = atomicAdd(&cnt[ql], __popc(take));
+          uint64_t* dst = a.bufs + ((int64_t)split * a.n_q + q) * a.caps;
+#pragma unroll
+          for (int j = 0; j < 16; j++) {
+            if (take & (1u << j)) {
+              const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+              const bool self = (c == qg);
+              if (pos < a.caps) dst[pos] = make_key(self ? 1.0f : M::value(acc[bm][bn], j), c, self);
+              pos++;
             }
           }
         }

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -66,6 +67,11 @@ struct evoc_mi355x_server {
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
   uint64_t next_ticket = 1;
   bool stopping = false;
+  // requests beyond one that are queued or running: running searches read it
+  // at every query slab and shrink their slabs while it is non-zero
+  std::atomic<int> contention{0};
+
+  void publish() { contention.store((int)std::max<int64_t>(0, (int64_t)queue.size() + running - 1)); }
 
   // next request to start and its grant, or nullptr (mu held)
   Request* pick(uint64_t& grant) {
@@ -93,6 +99,7 @@ struct evoc_mi355x_server {
       if (!r) break;  // stopping
       granted += grant;
       running++;
+      publish();
       lk.unlock();
       const double queued = std::chrono::duration<double>(Clock::now() - r->t_submit).count();
       ctx->budget = grant;
@@ -106,6 +113,7 @@ struct evoc_mi355x_server {
       lk.lock();
       granted -= grant;
       running--;
+      publish();
       r->status = rc;
       if (rc != EVOC_OK) r->error = ctx->last_error;
       r->done = true;
@@ -138,6 +146,16 @@ int evoc_mi355x_server_create(int32_t device, int32_t n_slots, uint64_t hbm_budg
   }
   // the budget is shared by all slots: by default 90% of what is free now
   s->budget = hbm_budget_bytes ? hbm_budget_bytes : s->slots[0]->budget;
+  for (auto* c : s->slots) {
+    c->serving = true;
+    c->contention = &s->contention;
+    const int rc = enable_staging(c, 32u << 20);
+    if (rc != EVOC_OK) {
+      for (auto* x : s->slots) evoc_mi355x_destroy(x);
+      delete s;
+      return rc;
+    }
+  }
   for (auto* c : s->slots) s->workers.emplace_back([s, c] { s->work(c); });
   *out = s;
   return EVOC_OK;
@@ -189,6 +207,7 @@ int evoc_mi355x_submit(evoc_mi355x_server* s, const evoc_knn_input* in, const ev
   *ticket = r->ticket;
   s->queue.push_back(r.get());
   s->reqs.emplace(r->ticket, std::move(r));
+  s->publish();
   s->cv_work.notify_one();
   return EVOC_OK;
 }

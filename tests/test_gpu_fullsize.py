@@ -1,0 +1,66 @@
+"""Million-row shapes on the MI355X: C3 (1,000,000 x 512 int8, k = 15) and
+C4 (1,000,000 x 1024 bits, k = 15), exact search through the C ABI.
+
+The CPU oracle cannot brute-force a million rows in test time, so the checks
+are: 256 sampled rows bit-exact against a GPU brute force with exact integer
+scores (bench.sampled_exact: FP64 dots of int8 rows, exact popcounts and one
+IEEE f32 division for Jaccard) under the library's ranking (score desc, self
+first, lower index first); and, over all rows, the size-independent
+invariants of the output (indices in range, no repeats, rows ordered by score).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+E = pytest.importorskip("evoc_mi355x")
+torch = pytest.importorskip("torch")
+from evoc_mi355x import synth  # noqa: E402
+
+from bench import sampled_exact  # noqa: E402
+
+
+def _run(x, k):
+    n = x.shape[0]
+    idx = torch.empty((n, k), dtype=torch.int32, device="cuda")
+    dist = torch.empty((n, k), dtype=torch.float32, device="cuda")
+    st = E.CallStats()
+    E.knn_graph(x, E.KnnGraphOptions(n_neighbors=k), E.NumpyRandomState(42), stats=st, out=(idx, dist))
+    return idx, dist, st
+
+
+def _invariants(idx, n):
+    assert int(idx.min()) >= 0 and int(idx.max()) < n
+    s = torch.sort(idx.long(), dim=1).values
+    assert not bool((s[:, 1:] == s[:, :-1]).any()), "a row lists a neighbour twice"
+
+
+@pytest.mark.parametrize("kind", ["i8", "u8"])
+def test_million_rows_sampled_bit_exact(kind):
+    n, k = 1_000_000, 15
+    print(f"[gpu] fullsize {kind}: generating {n} rows", flush=True)
+    if kind == "i8":
+        x = synth.blobs_i8_device(n, 512, 200, 0.2, seed=42)
+    else:
+        x = synth.bits_u8_device(n, 1024, 500, 0.3, 0.08, seed=42)
+    idx, dist, st = _run(x, k)
+    torch.cuda.synchronize()
+    print(f"[gpu] fullsize {kind}: {st.seconds * 1e3:.1f} ms, scoring {st.score_seconds * 1e3:.1f} ms, "
+          f"extra rounds {st.rescore_rounds}", flush=True)
+    _invariants(idx, n)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    sample = torch.randperm(n, generator=g, device="cuda")[:256]
+    ti, ts = sampled_exact(torch, x, kind, sample, k)
+    got = idx[sample].long()
+    bad = (got != ti).any(dim=1)
+    print(f"[gpu] fullsize {kind}: {int(bad.sum())} of 256 sampled rows differ", flush=True)
+    assert not bool(bad.any())
+    # distances: the reference's transforms (src/knn.rs:156-187, restated in the
+    # oracle) of the exact scores, raw = -score as the heap stores it
+    raw = (-ts.cpu().numpy()).astype(np.float32)
+    exp = O.transform(O.I8 if kind == "i8" else O.U8, raw)
+    got_d = dist[sample].cpu().numpy()
+    assert np.array_equal(got_d.view(np.uint32), exp.astype(np.float32).view(np.uint32))

@@ -110,7 +110,8 @@ def test_small_request_not_held_behind_large():
     tb.wait(timeout=120.0)
     srv.close()
     print(f"[gpu] large request done before the small ones: {big_done_when_small_done}; "
-          f"small latencies {[round(t.stats.seconds * 1e3, 2) for t in ts]} ms, "
+          f"small latencies {[round(t.stats.seconds * 1e3, 2) for t in ts]} ms "
+          f"(queued {[round(t.stats.queue_seconds * 1e3, 2) for t in ts]} ms), "
           f"large {tb.stats.seconds * 1e3:.1f} ms", flush=True)
     assert not big_done_when_small_done
     ctx = E.Context(0)
