@@ -16,7 +16,21 @@
 
 #define EVOC_WAVE 64
 
+#include <utility>
+
 namespace evoc {
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i = 0 .. N-1, with
+// no loop for the unroller to decline (accumulator blocks must be indexed by
+// constants or they leave registers).
+template <class F, int... Is>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 // float -> uint32 with the same order; -0.0 is folded onto +0.0 so the two
 // compare equal, as they do as IEEE floats in the reference.

@@ -28,13 +28,14 @@
 namespace evoc {
 
 // ---------------------------------------------------------------------------
-// MFMA tile engine.  256 threads = 4 waves in a 2 (candidates) x 2 (queries)
-// arrangement; each wave owns BM x BN MFMA 32x32 blocks, so a workgroup tile is
-// (64 BM) candidates x (64 BN) queries.  Shapes (see tile_shape()):
-//   <2,2>  128 x 128, 64 accumulators per lane, LDS 64 KiB: two workgroups per CU
-//          so one workgroup's epilogue overlaps the other's MFMAs
-//   <2,4>  128 x 256, 128 accumulators, LDS 96 KiB
-//   <4,4>  256 x 256, 256 accumulators, LDS 128 KiB: least LDS traffic per MFMA
+// MFMA tile engine.  WM x WN waves, each owning BM x BN MFMA 32x32 blocks, so a
+// workgroup tile is (32 BM WM) candidates x (32 BN WN) queries.  Shapes (see
+// tile_shape()), LDS = stage ring + pass-2 per-query arrays:
+//   <2,2> x (2,2)  128 x 128, 64 accumulators per lane, 64 KiB: two workgroups per CU
+//   <2,4> x (2,2)  128 x 256, 128 accumulators, 96 KiB
+//   <2,4> x (4,2)  256 x 256, 8 waves of 64 x 128, 128 KiB: two waves per SIMD
+//   <4,4> x (2,2)  256 x 256, 4 waves of 128 x 128, 256 accumulators in AGPRs,
+//                  128 KiB: a third less LDS operand traffic per MFMA than 64 x 128
 // MFMA orientation: M (rows) = candidates (A operand), N (cols) = queries
 // (B operand), so lane l of a 32x32 block holds query column l % 32 and the 16
 // candidate rows 8*(j/4) + 4*(l/32) + j%4, j = 0..15.
@@ -99,6 +100,16 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
+// A wave-uniform pointer the compiler cannot see through (volatile asm, not
+// hoisted out of loops), returned in SGPRs (readfirstlane).
+__device__ __forceinline__ const uint8_t* opaque_uniform_ptr(const uint8_t* p) {
+  uint32_t lo = (uint32_t)(uint64_t)p, hi = (uint32_t)((uint64_t)p >> 32);
+  asm volatile("" : "+v"(lo), "+v"(hi));
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  hi = __builtin_amdgcn_readfirstlane(hi);
+  return (const uint8_t*)(((uint64_t)hi << 32) | lo);
+}
+
 template <int KIND, int BM, int BN, int WM = 2, int WN = 2>
 struct Engine {
   using M = Mfma<KIND>;
@@ -115,7 +126,7 @@ struct Engine {
   __device__ static int wave_m() { return (threadIdx.x >> 6) / WN; }
   __device__ static int wave_n() { return (threadIdx.x >> 6) % WN; }
 
-  __device__ static void issue_stage(uint8_t* lds, const uint8_t* c_rows, const uint8_t* q_rows,
+  __device__ __forceinline__ static void issue_stage(uint8_t* lds, const uint8_t* c_rows, const uint8_t* q_rows,
                                      const TileArgs& a, int c0, int q0, int st) {
     uint8_t* buf = lds + (st % MFMA_NS) * STAGE_LDS;
     dma_stage<TC, WAVES>(buf, c_rows, a.row_bytes, c0, a.n_c, a.c_stride, st * MFMA_SB);
@@ -128,7 +139,8 @@ struct Engine {
   // stage's loads only (newer ones stay in flight), then one barrier both
   // publishes stage st and retires stage st - 1, whose buffer the loads of
   // stage st + MFMA_NS - 1 then refill.
-  __device__ static void score_tile(acc_t (&acc)[BM][BN], uint8_t* lds, const TileArgs& a, int c0, int q0) {
+  __device__ __forceinline__ static void score_tile(acc_t (&acc)[BM][BN], uint8_t* lds, const TileArgs& a, int c0,
+                                                    int q0) {
     const int lane = threadIdx.x & 63;
     const int wm = wave_m(), wn = wave_n();
 #pragma unroll
@@ -144,10 +156,8 @@ struct Engine {
     // uniform row bases, opaque to the compiler per tile: per-lane 64-bit
     // addresses derived from them are then rebuilt from SGPRs + 32-bit offsets
     // (saddr loads) instead of living in VGPR pairs across the tile loop
-    uint64_t cb = (uint64_t)a.c_rows, qb = (uint64_t)a.q_rows;
-    asm volatile("" : "+s"(cb), "+s"(qb));
-    const uint8_t* c_rows = (const uint8_t*)cb;
-    const uint8_t* q_rows = (const uint8_t*)qb;
+    const uint8_t* c_rows = opaque_uniform_ptr(a.c_rows);
+    const uint8_t* q_rows = opaque_uniform_ptr(a.q_rows);
     // the ring may still be read by waves finishing the previous tile
     __syncthreads();
     // (kept rolled: unrolled, the compiler precomputes each stage's 64-bit
@@ -254,25 +264,29 @@ __global__ __launch_bounds__(64 * WM * WN) void pass1_groupmax_kernel(TileArgs a
   for (int t = t_begin; t < t_end; t++) {
     const int c0 = t * En::TC;
     En::score_tile(acc, lds, a, c0, q0);
-#pragma unroll
-    for (int bm = 0; bm < BM; bm++) {
-#pragma unroll
-      for (int bn = 0; bn < BN; bn++) {
-        const int q = q0 + wn * 32 * BN + bn * 32 + (lane & 31);
+    // (see pass 2: indices opaque here so derived values are not held through the MFMA loop)
+    int lane_e = lane, wm_e = wm, wn_e = wn;
+    asm volatile("" : "+v"(lane_e), "+v"(wm_e), "+v"(wn_e));
+    static_for<BM>([&](auto bmc) __attribute__((always_inline)) {
+      constexpr int bm = decltype(bmc)::value;
+      static_for<BN>([&](auto bnc) __attribute__((always_inline)) {
+        constexpr int bn = decltype(bnc)::value;
+        __builtin_amdgcn_sched_barrier(0);  // keep this block's accumulator reads here
+        const int q = q0 + wn_e * 32 * BN + bn * 32 + (lane_e & 31);
         float m = -INFINITY;
 #pragma unroll
         for (int j = 0; j < 16; j++) {
-          const int c = c0 + wm * 32 * BM + bm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+          const int c = c0 + wm_e * 32 * BM + bm * 32 + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
           const bool self = (int64_t)c * a.c_stride == (int64_t)q + a.q_offset;
           if (c < a.n_c && !(self && a.skip_self)) {
             const float s = self ? 1.0f : M::value(acc[bm][bn], j);
             m = fmaxf(m, s);
           }
         }
-        const int g = ((t * WM + wm) * BM + bm) * 2 + (lane >> 5);
+        const int g = ((t * WM + wm_e) * BM + bm) * 2 + (lane_e >> 5);
         if (q < a.n_q) a.gmax[(int64_t)q * a.ng + g] = m;
-      }
-    }
+      });
+    });
   }
 }
 
@@ -320,14 +334,15 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
     // MFMA loop (which needs them all)
     int lane_e = lane, wm_e = wm, wn_e = wn;
     asm volatile("" : "+v"(lane_e), "+v"(wm_e), "+v"(wn_e));
-#pragma unroll
-    for (int bn = 0; bn < BN; bn++) {
+    static_for<BN>([&](auto bnc) __attribute__((always_inline)) {
+      constexpr int bn = decltype(bnc)::value;
       const int ql = wn_e * 32 * BN + bn * 32 + (lane_e & 31);
       const int q = q0 + ql;
       const int qg = q + a.q_offset;
       const float tfast = tfast_l[ql];
-#pragma unroll
-      for (int bm = 0; bm < BM; bm++) {
+      static_for<BM>([&](auto bmc) __attribute__((always_inline)) {
+        constexpr int bm = decltype(bmc)::value;
+        __builtin_amdgcn_sched_barrier(0);  // keep this block's accumulator reads here
         const int cb = c0 + wm_e * 32 * BM + bm * 32;      // first candidate of the block
         const bool diag = cb <= qg + 31 - (lane_e & 31) && qg - (lane_e & 31) <= cb + 31;
         // values of this lane's 16 candidates that pass the cheap bound
@@ -335,7 +350,7 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
 #pragma unroll
         for (int j = 0; j < 16; j++) m |= ((float)acc[bm][bn][j] >= tfast ? 1u : 0u) << j;
         if (diag) m = 0xffffu;  // the self pair takes the exact path
-        if (__ballot(m != 0) == 0) continue;
+        if (__ballot(m != 0) == 0) return;
         // the exact test on those values
         const float tau = tau_l[ql];
         const uint64_t tkey = tkey_l[ql];
@@ -365,8 +380,8 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
             }
           }
         }
-      }
-    }
+      });
+    });
   }
   __syncthreads();
   for (int i = threadIdx.x; i < En::TQ; i += En::THREADS) {
@@ -720,7 +735,8 @@ __global__ void finalize_kernel(const uint64_t* __restrict__ keys, int n_q, int 
 namespace evoc {
 
 // MFMA tile shapes (see Engine): 0 = 4 waves of 2x2 blocks (128 x 128 tile),
-// 1 = 4 waves of 2x4 (128 x 256), 2 = 8 waves of 2x4 (256 x 256).  The shape
+// 1 = 4 waves of 2x4 (128 x 256), 2 = 8 waves of 2x4 (256 x 256), 3 = 4 waves
+// of 4x4 (256 x 256, accumulators in AGPRs, least LDS traffic per MFMA).  The shape
 // in force is the calling thread's (set per search by the host, which takes it
 // from the autotuner), unless EVOC_MI355X_TILE pins one for the process.  The
 // shape never changes results: every shape accumulates each pair's products in
@@ -734,8 +750,8 @@ int tile_shape(int kind) {
     const char* e = getenv("EVOC_MI355X_TILE");
     g_env_shape = e ? atoi(e) : -1;
   }
-  if (g_env_shape >= 0 && g_env_shape <= 2) return g_env_shape;
-  if (t_shape >= 0 && t_shape <= 2) return t_shape;
+  if (g_env_shape >= 0 && g_env_shape <= 3) return g_env_shape;
+  if (t_shape >= 0 && t_shape <= 3) return t_shape;
   return 2;
 }
 
@@ -750,6 +766,7 @@ static void shape_dims(int kind, int& tc, int& tq, int& gpt) {
   switch (tile_shape(kind)) {
     case 1: tc = 128; tq = 256; gpt = 8; return;
     case 2: tc = 256; tq = 256; gpt = 16; return;
+    case 3: tc = 256; tq = 256; gpt = 16; return;
     default: tc = 128; tq = 128; gpt = 8; return;
   }
 }
@@ -776,13 +793,14 @@ static hipError_t launch_tile(Kern kern, size_t lds, dim3 grid, hipStream_t s, c
   return hipGetLastError();
 }
 
-int tile_threads(int kind) { return tile_shape(kind) == 2 ? 512 : 256; }
+int tile_threads(int kind) { return tile_shape(kind) == 2 ? 512 : 256; }  // shape 3: 4 waves
 
 template <int KIND>
 static hipError_t launch_pass1_kind(const TileArgs& a, dim3 g, size_t lds, hipStream_t s) {
   switch (tile_shape(KIND)) {
     case 1: return launch_tile(pass1_groupmax_kernel<KIND, 2, 4, 2, 2>, lds, g, s, a);
     case 2: return launch_tile(pass1_groupmax_kernel<KIND, 2, 4, 4, 2>, lds, g, s, a, 512);
+    case 3: return launch_tile(pass1_groupmax_kernel<KIND, 4, 4, 2, 2>, lds, g, s, a);
     default: return launch_tile(pass1_groupmax_kernel<KIND, 2, 2, 2, 2>, lds, g, s, a);
   }
 }
@@ -792,6 +810,7 @@ static hipError_t launch_pass2_kind(const TileArgs& a, dim3 g, size_t lds, hipSt
   switch (tile_shape(KIND)) {
     case 1: return launch_tile(pass2_append_kernel<KIND, 2, 4, 2, 2>, lds, g, s, a);
     case 2: return launch_tile(pass2_append_kernel<KIND, 2, 4, 4, 2>, lds, g, s, a, 512);
+    case 3: return launch_tile(pass2_append_kernel<KIND, 4, 4, 2, 2>, lds, g, s, a);
     default: return launch_tile(pass2_append_kernel<KIND, 2, 2, 2, 2>, lds, g, s, a);
   }
 }

@@ -1,6 +1,6 @@
 // Tile-shape autotuner.
 //
-// The MFMA scorer has three tile shapes (knn_exact.hip).  Which is fastest
+// The MFMA scorer has four tile shapes (knn_exact.hip).  Which is fastest
 // depends on the element kind and on the problem size: small matrices need
 // small tiles to fill 256 CUs, large ones want the 256 x 256 tile with the
 // least operand traffic per MFMA.  evoc_mi355x_autotune times a knn_graph call
@@ -49,7 +49,7 @@ int choose_tile_shape(int kind, int64_t n, int64_t row_bytes) {
 }
 
 extern "C" int evoc_mi355x_autotune(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const evoc_knn_options* o,
-                                    int64_t max_rows, int32_t* best_shape, double ms[3]) {
+                                    int64_t max_rows, int32_t* best_shape, double ms[4]) {
   if (!ctx || !in || !o || in->n <= 1) return EVOC_ERR_INVALID_ARGUMENT;
   if (in->dtype == EVOC_DTYPE_U8) {  // one (VALU) engine: nothing to choose
     if (best_shape) *best_shape = -1;
@@ -76,7 +76,7 @@ extern "C" int evoc_mi355x_autotune(evoc_mi355x_ctx* ctx, const evoc_knn_input* 
   // every shape must reproduce the first shape's graph bit for bit
   std::vector<int32_t> ref_idx, got_idx;
   std::vector<float> ref_dist, got_dist;
-  for (int shape = 0; shape < 3 && rc == EVOC_OK; shape++) {
+  for (int shape = 0; shape < 4 && rc == EVOC_OK; shape++) {
     t_force = shape;
     double t = 1e30;
     for (int rep = 0; rep < 3 && rc == EVOC_OK; rep++) {  // first run warms up

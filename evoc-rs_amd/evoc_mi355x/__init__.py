@@ -470,13 +470,13 @@ def request_bytes(data, options: KnnGraphOptions, out_on_device: bool = False,
 
 def autotune(data, options: KnnGraphOptions, max_rows: int = 65536, ctx: Optional[Context] = None):
     """Pick the fastest MFMA tile shape for searches like knn_graph(data, options)
-    (evoc_mi355x_autotune).  Returns {"shape": s, "ms": [t0, t1, t2]}."""
+    (evoc_mi355x_autotune).  Returns {"shape": s, "ms": [t0, t1, t2, t3]}."""
     ctx = ctx or default_context()
     inp, keep = _input_from(data)
     _sync_torch(keep)
     o = options._c()
     best = C.c_int32(-1)
-    ms = (C.c_double * 3)()
+    ms = (C.c_double * 4)()
     rc = lib().evoc_mi355x_autotune(ctx.handle, C.byref(inp), C.byref(o), max_rows, C.byref(best), ms)
     del keep
     if rc != OK:

@@ -148,13 +148,14 @@ int evoc_mi355x_request_bytes(const evoc_mi355x_ctx* ctx, const evoc_knn_input* 
 
 /* Tile-shape autotuner.  No counterpart in the reference.  Times the exact
  * search on the first min(n, max_rows) rows of `in` (max_rows <= 0: all) once
- * per MFMA tile shape (0: 128x128, 1: 128x256, 2: 256x256) and makes the
- * fastest the shape of every later exact search of the same element kind, row
- * count (power-of-two bucket) and row width in this process.  ms[3] receives
+ * per MFMA tile shape (0: 128x128, 1: 128x256, 2: 256x256 in 8 waves,
+ * 3: 256x256 in 4 waves) and makes the fastest the shape of every later exact
+ * search of the same element kind, row count (power-of-two bucket) and row
+ * width in this process.  ms[4] receives
  * the device milliseconds per shape (may be NULL).  Binary rows have a single
  * engine: *best_shape = -1.  Shapes change speed only, never results. */
 int evoc_mi355x_autotune(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const evoc_knn_options* opts,
-                         int64_t max_rows, int32_t* best_shape, double ms[3]);
+                         int64_t max_rows, int32_t* best_shape, double ms[4]);
 
 /* ---- serving: continuous batching of knn_graph requests ----
  * No counterpart in the reference (its knn_graph is one blocking call,
