@@ -322,7 +322,8 @@ def run_north_star(args, torch, E):
     idx = torch.empty((n, k), dtype=torch.int32, device="cuda")
     dist = torch.empty((n, k), dtype=torch.float32, device="cuda")
     out = {}
-    for name, prec, steps in (("bf16", E.PREC_BF16_RESCORE, args.steps), ("fp32", E.PREC_FP32, 1)):
+    runs = [("bf16", E.PREC_BF16_RESCORE, args.steps)] + ([] if args.no_fp32 else [("fp32", E.PREC_FP32, 1)])
+    for name, prec, steps in runs:
         opts = E.KnnGraphOptions(n_neighbors=k, precision=prec, margin=args.margin)
 
         def step(st, opts=opts):
@@ -342,6 +343,7 @@ def run_north_star(args, torch, E):
         log(f"[bench] 1M {name}: {out[name]}")
     ctx.close()
     b = out["bf16"]
+    f32 = out.get("fp32")
     return _base_line(args, b["points_per_s"], b["ms_per_step"] / 1e3,
                       {"workload": f"{n}x{d} f32 L2-normalised Gaussian blobs (50 centres, std 0.25, Zipf 1.1), "
                                    f"cosine kNN k={k}, exact BF16-MFMA candidates + FP32 re-score of k+{args.margin}",
@@ -352,7 +354,7 @@ def run_north_star(args, torch, E):
                                     "achieved": b["score_tflops"], "peak": PEAK["bf16"], "unit": "TFLOP/s",
                                     "frac": b["score_frac"], "traffic": None,
                                     "algorithmic_flop_per_launch": 2.0 * n * n * d, "avg_launch_ms": b["score_ms"]},
-                       "fp32": out["fp32"], "bf16_over_fp32": b["points_per_s"] / out["fp32"]["points_per_s"]})
+                       "fp32": f32, "bf16_over_fp32": b["points_per_s"] / f32["points_per_s"] if f32 else None})
 
 
 def run_nndescent(args, torch, E):

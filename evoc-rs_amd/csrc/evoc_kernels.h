@@ -70,7 +70,7 @@ size_t tile_lds_bytes(int kind, int pass);
 hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);
 hipError_t launch_pass2(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);
 hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int K, float* tau, hipStream_t s);
-hipError_t launch_select(const SelectArgs& a, int max_entries, hipStream_t s);
+hipError_t launch_select(const SelectArgs& a, hipStream_t s);
 hipError_t launch_rescore(const float* X, int64_t ld, int d, int n_q, int q_offset,
                           const uint64_t* cand, int Kc, int k, uint64_t* out, hipStream_t s);
 hipError_t launch_finalize(const uint64_t* keys, int n_q, int q_offset, int k, int dtype,

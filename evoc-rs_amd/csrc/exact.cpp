@@ -53,7 +53,6 @@ struct Plan {
   int stride = 1, n_sample = 0, ng = 0;
   int p1_tiles_per_split = 1, p1_splits = 1;
   int group_q = 1, p1_group_q = 1;  // query blocks per XCD group (tile_coords)
-  int max_entries = 0;
   // device bytes exact_search allocates for a slab of nq queries (each buffer
   // rounded to 256 bytes by DevBuf)
   size_t scratch_bytes(int64_t nq, int64_t nqb) const {
@@ -81,7 +80,6 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
     // every candidate fits one sorted list: no threshold pass
     p.pass1 = false;
     p.caps = p.tiles_per_split * TC;
-    p.max_entries = (int)std::min<int64_t>(n, (int64_t)p.n_splits * p.caps);
     return p;
   }
   p.pass1 = true;
@@ -105,7 +103,6 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   // the kept subset must hold 2K keys so an overflow round tightens the bound
   caps = std::max<int64_t>(caps, pow2ceil(cdiv(2LL * K, p.n_splits)));
   p.caps = (int)std::min<int64_t>(caps, MAX_SORT);
-  p.max_entries = (int)std::min<int64_t>((int64_t)p.n_splits * p.caps, MAX_SORT);
   const int64_t s_tiles = cdiv(p.n_sample, TC);
   p.p1_tiles_per_split = best_split(nqb, s_tiles, slots);
   p.p1_splits = (int)cdiv(s_tiles, p.p1_tiles_per_split);
@@ -261,7 +258,7 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
     (void)hipEventRecord(e0, s);
     e = launch_pass2(kind, a2, (int)nqb, p.n_splits, s);
     (void)hipEventRecord(e1, s);
-    if (e == hipSuccess) e = launch_select(sa, p.max_entries, s);
+    if (e == hipSuccess) e = launch_select(sa, s);
     if (e != hipSuccess) {
       rc = ctx->hip_fail(e, "scoring pass");
       break;
@@ -308,7 +305,7 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       SelectArgs sr = sa;
       sr.query_list = d_q;
       sr.n_list = (int32_t)qlist.size();
-      if (e == hipSuccess) e = launch_select(sr, p.max_entries, s);
+      if (e == hipSuccess) e = launch_select(sr, s);
       if (e != hipSuccess) {
         rc = ctx->hip_fail(e, "re-scoring pass");
         break;

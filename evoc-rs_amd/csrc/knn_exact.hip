@@ -134,11 +134,14 @@ struct Engine {
   }
 
   // Accumulate the TC x TQ score tile (candidates c0.., queries q0..) over all
-  // stages.  Stage st lives in ring buffer st % MFMA_NS; loads run
-  // MFMA_NS - 1 stages ahead of the MFMAs.  Iteration st waits for its own
-  // stage's loads only (newer ones stay in flight), then one barrier both
-  // publishes stage st and retires stage st - 1, whose buffer the loads of
-  // stage st + MFMA_NS - 1 then refill.
+  // stages.  Stage st lives in ring buffer st % MFMA_NS.  The stage boundary
+  // is crossed inside the previous stage: once a wave has issued (and waited
+  // for) its LDS reads of the last K step of stage s, it waits for its own
+  // loads of stage s + 1, and one barrier then publishes stage s + 1 and
+  // retires buffer s % MFMA_NS, which the loads of stage s + MFMA_NS refill
+  // (loads run MFMA_NS stages ahead).  The first reads of stage s + 1 then
+  // overlap the last MFMAs of stage s, so no stage starts on an LDS bubble.
+  // Within a K step, the reads of the next step alternate with the MFMAs.
   __device__ __forceinline__ static void score_tile(acc_t (&acc)[BM][BN], uint8_t* lds, const TileArgs& a, int c0,
                                                     int q0) {
     const int lane = threadIdx.x & 63;
@@ -153,59 +156,74 @@ struct Engine {
     const int arow = (wm * 32 * BM + (lane & 31)) * MFMA_SB;
     const int brow = TC * MFMA_SB + (wn * 32 * BN + (lane & 31)) * MFMA_SB;
     const int sw = lds_chunk_key(lane & 31);
-    // uniform row bases, opaque to the compiler per tile: per-lane 64-bit
-    // addresses derived from them are then rebuilt from SGPRs + 32-bit offsets
-    // (saddr loads) instead of living in VGPR pairs across the tile loop
     const uint8_t* c_rows = opaque_uniform_ptr(a.c_rows);
     const uint8_t* q_rows = opaque_uniform_ptr(a.q_rows);
+    constexpr int KS = MFMA_SB / M::KSTEP_BYTES;
+    static_assert(KS % 2 == 0, "fragment double buffer parity must carry across stages");
+    typename M::frag_t fa[2][BM], fb[2][BN];
+    auto read_frags = [&](const uint8_t* buf, int ks, int par) __attribute__((always_inline)) {
+      const int kb = ks * M::KSTEP_BYTES + (lane >> 5) * M::LANE_BYTES;
+      const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
+#pragma unroll
+      for (int b = 0; b < BM; b++) fa[par][b] = M::load(buf + arow + off + b * 32 * MFMA_SB);
+#pragma unroll
+      for (int b = 0; b < BN; b++) fb[par][b] = M::load(buf + brow + off + b * 32 * MFMA_SB);
+    };
+    // wait until this wave's loads of `stage` have landed; `newer` later stages may stay in flight
+    auto wait_stage = [&](int newer) __attribute__((always_inline)) {
+      if (newer >= 3) wait_vmcnt<3 * DMA_PER_STAGE>();
+      else if (newer == 2) wait_vmcnt<2 * DMA_PER_STAGE>();
+      else if (newer == 1) wait_vmcnt<DMA_PER_STAGE>();
+      else wait_vmcnt<0>();
+    };
     // the ring may still be read by waves finishing the previous tile
     __syncthreads();
     // (kept rolled: unrolled, the compiler precomputes each stage's 64-bit
     // addresses, runs out of registers and spills them)
 #pragma unroll 1
-    for (int st = 0; st < MFMA_NS - 1; st++)
+    for (int st = 0; st < MFMA_NS; st++)
       if (st < nst) issue_stage(lds, c_rows, q_rows, a, c0, q0, st);
+    wait_stage(min(MFMA_NS - 1, nst - 1));
+    ring_barrier();
+    read_frags(lds, 0, 0);
+    // MFMAs of one K step from fragment buffer `par`, each followed by one of
+    // the fragment reads issued just before (an MFMA first: its operand wait
+    // then covers the previous step's reads only), then the remaining MFMAs
+    auto mma_step = [&](int par) __attribute__((always_inline)) {
+#pragma unroll
+      for (int bm = 0; bm < BM; bm++)
+#pragma unroll
+        for (int bn = 0; bn < BN; bn++) M::mma(acc[bm][bn], fa[par][bm], fb[par][bn]);
+#pragma unroll
+      for (int i = 0; i < BM + BN; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, BM * BN, 0);
+      __builtin_amdgcn_sched_barrier(0);
+    };
     for (int s = 0; s < nst; s++) {
-      // stages issued after s: min(MFMA_NS - 2, nst - 1 - s)
-      const int newer = min(MFMA_NS - 2, nst - 1 - s);
-      if (newer >= 2) wait_vmcnt<2 * DMA_PER_STAGE>();
-      else if (newer == 1) wait_vmcnt<DMA_PER_STAGE>();
-      else wait_vmcnt<0>();
-      ring_barrier();
-      if (s + MFMA_NS - 1 < nst) issue_stage(lds, c_rows, q_rows, a, c0, q0, s + MFMA_NS - 1);
       const uint8_t* cur = lds + (s % MFMA_NS) * STAGE_LDS;
-      // fragments of K step ks + 1 are read from LDS while the MFMAs of step ks run
-      constexpr int KS = MFMA_SB / M::KSTEP_BYTES;
-      typename M::frag_t fa[2][BM], fb[2][BN];
-      {
-        const int kb = (lane >> 5) * M::LANE_BYTES;
-        const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
 #pragma unroll
-        for (int b = 0; b < BM; b++) fa[0][b] = M::load(cur + arow + off + b * 32 * MFMA_SB);
-#pragma unroll
-        for (int b = 0; b < BN; b++) fb[0][b] = M::load(cur + brow + off + b * 32 * MFMA_SB);
-      }
-#pragma unroll
-      for (int ks = 0; ks < KS; ks++) {
+      for (int ks = 0; ks + 1 < KS; ks++) {
         const int p = ks & 1;
-        if (ks + 1 < KS) {
-          const int kb = (ks + 1) * M::KSTEP_BYTES + (lane >> 5) * M::LANE_BYTES;
-          const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
-#pragma unroll
-          for (int b = 0; b < BM; b++) fa[p ^ 1][b] = M::load(cur + arow + off + b * 32 * MFMA_SB);
-#pragma unroll
-          for (int b = 0; b < BN; b++) fb[p ^ 1][b] = M::load(cur + brow + off + b * 32 * MFMA_SB);
-        }
-        // the reads above stay ahead of these MFMAs: their wait lands at the next step
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int bm = 0; bm < BM; bm++)
-#pragma unroll
-          for (int bn = 0; bn < BN; bn++) M::mma(acc[bm][bn], fa[p][bm], fb[p][bn]);
-        __builtin_amdgcn_sched_barrier(0);
+        read_frags(cur, ks + 1, p ^ 1);
+        mma_step(p);
       }
+      constexpr int PL = (KS - 1) & 1;  // buffer of the last K step
+      // cross into stage s + 1: this wave's reads of stage s are complete
+      // (lgkmcnt) and its loads of stage s + 1 have landed (vmcnt).  After the
+      // last stage this is a harmless extra round (every load has landed, no
+      // load is issued, and the reads fill a fragment buffer no MFMA uses);
+      // one unconditional path keeps the compiler from merging waits.
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0), vmcnt / expcnt untouched
+      wait_stage(min(MFMA_NS - 2, nst - 2 - s));
+      ring_barrier();
+      if (s + MFMA_NS < nst) issue_stage(lds, c_rows, q_rows, a, c0, q0, s + MFMA_NS);
+      read_frags(lds + ((s + 1) % MFMA_NS) * STAGE_LDS, 0, PL ^ 1);
+      mma_step(PL);
     }
-    // every stage load has landed (the last iteration waited for all); saying
+    // every stage load has landed (the last crossing waited for all); saying
     // so lets the epilogue's LDS accesses go without further vmcnt waits
     wait_vmcnt<0>();
   }
@@ -556,32 +574,47 @@ __global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Pass 3: per query, gather the buffered keys of all splits, sort, keep K.
-// Overflow (a split dropped keys, or more than 64*E keys in total) marks the
-// query and stores the K-th kept key as a tighter bound for the next round:
+// Pass 3: per query, stream the buffered keys of all splits through a wave
+// sort, keep K.  Overflow (a split's buffer dropped keys) marks the query and
+// stores the K-th kept key as a tighter bound for the next round:
 // the K-th best of any subset is <= the true K-th best.
 // ---------------------------------------------------------------------------
-// gather the first 64*E buffered keys of query q, sort, write the K best
+// Keys of query q from all splits, streamed through a 64*E-key wave sort: each
+// round loads as many unread keys as fit beside the K best so far, sorts, and
+// keeps the K best.  Writes the K best.
 template <int E>
-__device__ __forceinline__ void select_body(const SelectArgs& a, int q, bool overflow) {
+__device__ __forceinline__ void select_body(const SelectArgs& a, int q, bool overflow, int total) {
   const int lane = threadIdx.x & 63;
   constexpr int CAP = 64 * E;
   uint64_t v[E];
 #pragma unroll
   for (int e = 0; e < E; e++) v[e] = EVOC_KEY_NONE;
-  int filled = 0;
-  for (int s = 0; s < a.n_splits; s++) {
-    const int c = min(a.counts[(int64_t)s * a.n_q + q], a.caps);
-    const uint64_t* src = a.bufs + ((int64_t)s * a.n_q + q) * a.caps;
+  int keep = 0;  // best keys so far, at positions [0, keep)
+  int g0 = 0;    // index (over all splits, in split order) of the next unread key
+  do {
+    const int take = min(CAP - keep, total - g0);
+    // keys g0 .. g0 + take - 1 go to positions keep .. keep + take - 1
+    int base = 0;  // index of split s's first key
+    for (int s = 0; s < a.n_splits; s++) {
+      const int c = min(a.counts[(int64_t)s * a.n_q + q], a.caps);
+      const int lo = max(base, g0), hi = min(base + c, g0 + take);
+      if (lo < hi) {
+        const uint64_t* src = a.bufs + ((int64_t)s * a.n_q + q) * a.caps;
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-      const int f = e * 64 + lane;
-      if (f >= filled && f < filled + c) v[e] = src[f - filled];
+        for (int e = 0; e < E; e++) {
+          const int gi = g0 + (e * 64 + lane - keep);
+          if (e * 64 + lane >= keep && gi >= lo && gi < hi) v[e] = src[gi - base];
+        }
+      }
+      base += c;
     }
-    filled += c;
-  }
-  if (filled > CAP) overflow = true;
-  wave_sort_desc<E>(v);
+#pragma unroll
+    for (int e = 0; e < E; e++)
+      if (e * 64 + lane >= keep + take) v[e] = EVOC_KEY_NONE;
+    wave_sort_desc<E>(v);
+    keep = min(a.K, keep + take);
+    g0 += take;
+  } while (g0 < total);
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int i = e * 64 + lane;
@@ -614,13 +647,15 @@ __global__ __launch_bounds__(256) void select_topk_kernel(SelectArgs a) {
     overflow |= c > a.caps;
     total += min(c, a.caps);
   }
-  const int need = max(total, a.K);
-  if (need <= 64 || EMAX == 1) select_body<1>(a, q, overflow);
-  else if (need <= 128 || EMAX == 2) select_body<2>(a, q, overflow);
-  else if (need <= 256 || EMAX == 4) select_body<4>(a, q, overflow);
-  else if (need <= 512 || EMAX == 8) select_body<8>(a, q, overflow);
-  else if (need <= 1024 || EMAX == 16) select_body<16>(a, q, overflow);
-  else select_body<EMAX>(a, q, overflow);
+  // sort width: the query's own key count, or the widest sort streamed over
+  // its keys (which must leave room beside the K kept)
+  const int need = max(total, 2 * a.K);
+  if (need <= 64 || EMAX == 1) select_body<1>(a, q, overflow, total);
+  else if (need <= 128 || EMAX == 2) select_body<2>(a, q, overflow, total);
+  else if (need <= 256 || EMAX == 4) select_body<4>(a, q, overflow, total);
+  else if (need <= 512 || EMAX == 8) select_body<8>(a, q, overflow, total);
+  else if (need <= 1024 || EMAX == 16) select_body<16>(a, q, overflow, total);
+  else select_body<EMAX>(a, q, overflow, total);
 }
 
 // ---------------------------------------------------------------------------
@@ -871,10 +906,9 @@ hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int K, float* t
   return hipGetLastError();
 }
 
-hipError_t launch_select(const SelectArgs& a, int max_entries, hipStream_t s) {
+hipError_t launch_select(const SelectArgs& a, hipStream_t s) {
   const int nq = a.query_list ? a.n_list : a.n_q;
   const dim3 g((unsigned)((nq + 3) / 4)), b(256);
-  (void)max_entries;
   hipLaunchKernelGGL(select_topk_kernel<32>, g, b, 0, s, a);
   return hipGetLastError();
 }
