@@ -23,6 +23,11 @@ for p in (ROOT, os.path.join(ROOT, "evoc-rs_amd")):
 METRIC = "kNN-graph points/sec at recall@k≥0.99 (BF16+rescore vs FP32); p50/p99 latency"
 PEAK = {"bf16": 2500.0, "fp32": 157.3, "int8": 5000.0}  # dense TFLOP/s (TOP/s for int8)
 
+# HBM bytes per pass-2 launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each
+# scaled by what the counter reads for a 1 GiB device copy (profiles/pass2_bf16_100k_768_pmc.md);
+# keyed by (n, d, element kind): the counters cannot be read from inside this process
+MEASURED_TRAFFIC = {(100000, 768, "bf16"): 37.3e9}
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -501,8 +506,11 @@ def main():
         "latency_ms": {"p50": lat[len(lat) // 2] * 1e3, "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))] * 1e3},
         "roofline": {"bound": "mfma", "kernel": "pass2_append_kernel<bf16>", "achieved": achieved,
                      "peak": PEAK["bf16"], "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"],
-                     "traffic": None, "algorithmic_flop_per_launch": flops,
-                     "avg_launch_ms": score_s * 1e3},
+                     "traffic": MEASURED_TRAFFIC.get((n, d, "bf16")),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated "
+                                       "(profiles/pass2_bf16_100k_768_pmc.md)"
+                                       if (n, d, "bf16") in MEASURED_TRAFFIC else None,
+                     "algorithmic_flop_per_launch": flops, "avg_launch_ms": score_s * 1e3},
     }
     if not args.no_fp32:
         log("[bench] timing fp32")

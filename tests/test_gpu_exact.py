@@ -175,6 +175,39 @@ def test_zero_rows_and_negative_similarity(ctx):
     assert boundary_mismatch_rows(idx, dist, ei, et, 1e-6) == 0
 
 
+@pytest.mark.parametrize("dtype", ["f32", "i8"])
+def test_ragged_ld_device_view(ctx, dtype):
+    """A row-major view with row pitch ld > d (a column slice of a wider device matrix) gives
+    the same graph as the contiguous copy of the same rows."""
+    torch = pytest.importorskip("torch")
+    n, d, pad = 1500, 100, 13
+    if dtype == "f32":
+        base = synth.blobs_f32(n, d + pad, 8, 0.3, seed=31)
+    else:
+        base = synth.blobs_i8(n, d + pad, 8, 0.3, seed=31)
+    wide = torch.from_numpy(base).cuda()
+    view = wide[:, :d]
+    assert view.stride(0) == d + pad and view.stride(1) == 1
+    a_idx, a_dist = run(ctx, view, 3, n_neighbors=12, precision=E.PREC_FP32)
+    b_idx, b_dist = run(ctx, view.contiguous().cpu().numpy(), 3, n_neighbors=12, precision=E.PREC_FP32)
+    np.testing.assert_array_equal(a_idx, b_idx)
+    np.testing.assert_array_equal(a_dist, b_dist)
+
+
+@pytest.mark.parametrize("dtype", ["i8", "u8"])
+def test_single_column(ctx, dtype):
+    """d = 1: every score is one product (or one byte's Jaccard), so almost every candidate
+    ties; the lower-index rule decides, bit-exact against the oracle."""
+    rs = np.random.default_rng(8)
+    if dtype == "i8":
+        x = rs.integers(-127, 128, size=(600, 1)).astype(np.int8)
+    else:
+        x = rs.integers(0, 256, size=(600, 1)).astype(np.uint8)
+    idx, dist = run(ctx, x, 1, n_neighbors=9)
+    ei, ed = O.exact_knn(x, 9)
+    np.testing.assert_array_equal(idx, ei)
+
+
 @pytest.mark.parametrize("dtype", ["i8", "u8"])
 def test_massive_exact_ties(ctx, dtype):
     """3000 rows that are 30 distinct rows repeated: every candidate list overflows and is
