@@ -23,10 +23,13 @@ for p in (ROOT, os.path.join(ROOT, "evoc-rs_amd")):
 METRIC = "kNN-graph points/sec at recall@k≥0.99 (BF16+rescore vs FP32); p50/p99 latency"
 PEAK = {"bf16": 2500.0, "fp32": 157.3, "int8": 5000.0}  # dense TFLOP/s (TOP/s for int8)
 
-# HBM bytes per pass-2 launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each
-# scaled by what the counter reads for a 1 GiB device copy (profiles/pass2_bf16_100k_768_pmc.md);
-# keyed by (n, d, element kind): the counters cannot be read from inside this process
-MEASURED_TRAFFIC = {(100000, 768, "bf16"): 37.3e9}
+# HBM bytes per scoring launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each
+# scaled by what the counter reads for a 1 GiB device copy; keyed by (n, d, kernel), with the
+# profile summary they come from: the counters cannot be read from inside this process
+MEASURED_TRAFFIC = {
+    (100000, 768, "pass2_append_kernel<bf16>"): (37.3e9, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, "
+                                                         "copy-calibrated (profiles/pass2_bf16_100k_768_pmc.md)"),
+}
 
 
 def log(msg):
@@ -226,6 +229,15 @@ def _timed(torch, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps, stats
 
 
+def _scored(stats, n, kind_tag):
+    """Row pairs one scoring launch scored (the library reports them: n (n + 1) / 2 in the
+    symmetric pass, which scores each unordered pair once, n^2 in the per-split pass) and
+    the kernel's name."""
+    pairs = float(np.mean([s_.score_pairs for s_ in stats])) or float(n) * n
+    name = "pass2_sym_kernel" if pairs < float(n) * n else "pass2_append_kernel"
+    return pairs, f"{name}<{kind_tag}>"
+
+
 def _base_line(args, metric_value, sec, workload, dtype, extra):
     out = {"metric": METRIC, "value": metric_value, "unit": "points/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -247,7 +259,7 @@ def run_million(args, torch, E, kind):
         x = synth.blobs_i8_device(n, d, 200, 0.2, seed=42)
         desc = (f"{n}x{d} int8 (200 Gaussian blobs, std 0.2, L2-normalised, x127, rounded), "
                 f"inner-product kNN k={k}, exact i8 MFMA into i32")
-        dtype, peak, unit, work = "int8 (i32 accumulate)", PEAK["int8"], "TOP/s", 2.0 * n * n * d
+        dtype, peak, unit, per_pair = "int8 (i32 accumulate)", PEAK["int8"], "TOP/s", 2.0 * d
     else:
         nbytes = 128
         log(f"[bench] generating {n}x{nbytes * 8}-bit rows on the GPU")
@@ -255,7 +267,7 @@ def run_million(args, torch, E, kind):
         desc = (f"{n}x{nbytes * 8}-bit rows (500 Bernoulli(0.3) prototypes, bit flips p=0.08), "
                 f"bitwise Jaccard k={k}, VALU popcount")
         # AND + popcount-accumulate per 32-bit word; peak = 256 CU x 64 lanes x 2.4 GHz int32 ops
-        dtype, peak, unit, work = "u8 bits (i32 popcount)", 256 * 64 * 2.4e9 / 1e12, "TOP/s", 2.0 * n * n * (nbytes / 4)
+        dtype, peak, unit, per_pair = "u8 bits (i32 popcount)", 256 * 64 * 2.4e9 / 1e12, "TOP/s", 2.0 * (nbytes / 4)
     ctx = E.Context(0)
     idx = torch.empty((n, k), dtype=torch.int32, device="cuda")
     dist = torch.empty((n, k), dtype=torch.float32, device="cuda")
@@ -277,11 +289,14 @@ def run_million(args, torch, E, kind):
     log(f"[bench] {exact_rows}/256 sampled rows bit-exact")
     sec, stats = _timed(torch, step, args.steps, args.warmup)
     score_s = float(np.mean([s.score_seconds for s in stats]))
+    pairs, kern = _scored(stats, n, "i8")
+    work = per_pair * pairs
     achieved = work / score_s / 1e12
     res = _base_line(args, n / sec, sec, {"workload": desc, "n": n, "k": k, "method": "exact"}, dtype, {
         "sampled_rows_bit_exact": f"{exact_rows}/256",
+        "rescore_rounds": max(s_.rescore_rounds for s_ in stats),
         "roofline": {"bound": "mfma" if kind == "i8" else "valu", "kernel":
-                     "pass2_append_kernel<i8>" if kind == "i8" else "jac_pass2_kernel",
+                     kern if kind == "i8" else "jac_pass2_kernel",
                      "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak, "traffic": None,
                      "algorithmic_ops_per_launch": work, "avg_launch_ms": score_s * 1e3},
     })
@@ -340,8 +355,11 @@ def run_north_star(args, torch, E):
         rec = float((got.unsqueeze(2) == truth.unsqueeze(1)).any(dim=2).float().mean().item())
         sec, stats = _timed(torch, step, steps, 0)
         score_s = float(np.mean([s_.score_seconds for s_ in stats]))
-        flops = 2.0 * n * n * d
+        pairs, kern = _scored(stats, n, name)
+        flops = 2.0 * pairs * d
         out[name] = {"points_per_s": n / sec, "ms_per_step": sec * 1e3, "recall_at_k_sampled": rec,
+                     "kernel": kern, "flop_per_launch": flops,
+                     "rescore_rounds": max(s_.rescore_rounds for s_ in stats),
                      "score_tflops": flops / score_s / 1e12,
                      "score_frac": flops / score_s / 1e12 / (PEAK["bf16"] if name == "bf16" else PEAK["fp32"]),
                      "score_ms": score_s * 1e3, "steps": steps}
@@ -355,10 +373,11 @@ def run_north_star(args, torch, E):
                        "n": n, "d": d, "k": k, "method": "exact_bf16_rescore"},
                       "bf16 (fp32 accumulate) + fp32 rescore",
                       {"recall_at_k": b["recall_at_k_sampled"], "recall_sample_rows": 1000,
-                       "roofline": {"bound": "mfma", "kernel": "pass2_append_kernel<bf16>",
+                       "roofline": {"bound": "mfma", "kernel": b["kernel"],
                                     "achieved": b["score_tflops"], "peak": PEAK["bf16"], "unit": "TFLOP/s",
                                     "frac": b["score_frac"], "traffic": None,
-                                    "algorithmic_flop_per_launch": 2.0 * n * n * d, "avg_launch_ms": b["score_ms"]},
+                                    "algorithmic_flop_per_launch": b["flop_per_launch"],
+                                    "avg_launch_ms": b["score_ms"]},
                        "fp32": f32, "bf16_over_fp32": b["points_per_s"] / f32["points_per_s"] if f32 else None})
 
 
@@ -483,7 +502,8 @@ def main():
     sec, stats = timed(E.PREC_BF16_RESCORE)
     lat = sorted(s.seconds for s in stats)
     score_s = float(np.mean([s.score_seconds for s in stats]))
-    flops = 2.0 * n * n * d
+    pairs, kern = _scored(stats, n, "bf16")
+    flops = 2.0 * pairs * d
     achieved = flops / score_s / 1e12
     result = {
         "metric": METRIC,
@@ -504,12 +524,10 @@ def main():
                    "n": n, "d": d, "k": k, "method": "exact_bf16_rescore"},
         "recall_at_k": rec_bf16,
         "latency_ms": {"p50": lat[len(lat) // 2] * 1e3, "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))] * 1e3},
-        "roofline": {"bound": "mfma", "kernel": "pass2_append_kernel<bf16>", "achieved": achieved,
+        "roofline": {"bound": "mfma", "kernel": kern, "achieved": achieved,
                      "peak": PEAK["bf16"], "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"],
-                     "traffic": MEASURED_TRAFFIC.get((n, d, "bf16")),
-                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated "
-                                       "(profiles/pass2_bf16_100k_768_pmc.md)"
-                                       if (n, d, "bf16") in MEASURED_TRAFFIC else None,
+                     "traffic": MEASURED_TRAFFIC.get((n, d, kern), (None, None))[0],
+                     "traffic_source": MEASURED_TRAFFIC.get((n, d, kern), (None, None))[1],
                      "algorithmic_flop_per_launch": flops, "avg_launch_ms": score_s * 1e3},
     }
     if not args.no_fp32:
@@ -519,9 +537,11 @@ def main():
         rec_fp32 = recall_gpu(torch, idx, truth)
         sec32, st32 = timed(E.PREC_FP32)
         s32 = float(np.mean([s.score_seconds for s in st32]))
+        pairs32, kern32 = _scored(st32, n, "fp32")
+        f32flops = 2.0 * pairs32 * d
         result["fp32"] = {"points_per_s": n / sec32, "ms_per_step": sec32 * 1e3, "recall_at_k": rec_fp32,
-                          "score_kernel_tflops": flops / s32 / 1e12,
-                          "score_kernel_frac": flops / s32 / 1e12 / PEAK["fp32"]}
+                          "score_kernel": kern32, "score_kernel_tflops": f32flops / s32 / 1e12,
+                          "score_kernel_frac": f32flops / s32 / 1e12 / PEAK["fp32"]}
         result["bf16_over_fp32"] = (n / sec) / (n / sec32)
     if args.serving_requests > 0:
         log(f"[bench] serving: {args.serving_requests} requests at {args.serving_rate}/s")

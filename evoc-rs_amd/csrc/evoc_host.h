@@ -29,6 +29,23 @@ struct evoc_mi355x_ctx {
   void* stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   size_t stage_bytes = 0;
+  // Non-serving contexts keep the scratch blocks of their last call for the
+  // next one: a released block goes to `cache` instead of back to the pool,
+  // and an allocation of about its size takes it again.  Every use is on this
+  // context's stream, so stream order makes the reuse safe.  (Releasing tens
+  // of GiB to the pool and taking them back costs hundreds of milliseconds
+  // per call.)  A call frees the cached blocks it did not reuse when it
+  // returns (CallScope); evoc_mi355x_trim frees all of them.
+  struct Cached {
+    void* ptr;
+    size_t bytes;
+    uint64_t gen;  // call that released it
+  };
+  std::vector<Cached> cache;
+  uint64_t cached = 0;  // bytes in `cache` (not in in_use)
+  uint64_t gen = 0;     // calls begun
+  void free_cached(size_t i);
+  void free_stale();    // free cached blocks released before the current call
   std::mutex mu;
   std::string last_error;
 
@@ -37,6 +54,15 @@ struct evoc_mi355x_ctx {
   ~evoc_mi355x_ctx();
 };
 typedef evoc_mi355x_ctx Ctx;
+
+// One top-level call on a context: blocks cached by earlier calls and not
+// reused by this one are freed when it ends.  Construct it before the call's
+// first DevBuf, so that it ends after their release.
+struct CallScope {
+  Ctx* c;
+  explicit CallScope(Ctx* ctx) : c(ctx) { c->gen++; }
+  ~CallScope() { c->free_stale(); }
+};
 
 // Device allocation accounted against the context's HBM budget, from the
 // context's memory pool in the order of its stream.
@@ -64,6 +90,7 @@ struct ExactStats {
   int64_t extra_rounds = 0;  // re-scoring rounds caused by buffer overflow
   int64_t slabs = 0;
   double pass2_ms = 0.0;     // device time of the scoring pass (all slabs)
+  int64_t score_pairs = 0;   // row pairs scored by the scoring pass (all slabs, first round)
 };
 
 // Host -> device copy on the context stream that returns only when the copy is

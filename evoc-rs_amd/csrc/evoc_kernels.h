@@ -42,7 +42,16 @@ struct TileArgs {
   uint64_t* bufs;            // [splits][n_q][caps]
   int32_t* counts;           // [splits][n_q]
   int32_t caps;
-  int32_t pad2;
+  int32_t ccap;              // symmetric pass: candidate-side slots per row
+  // symmetric pass 2 (query and candidate rows are the same matrix, q_offset 0):
+  // each workgroup scores query block work[i].x against candidate tiles
+  // [work[i].y, work[i].z) (all >= the query block) as split work[i].w, and
+  // also appends every pair's key to the candidate row's list in cbufs
+  const int4* work;          // [n_work]
+  int32_t n_work;
+  int32_t pad3;
+  uint64_t* cbufs;           // [n_q][ccap] candidate-side keys
+  int32_t* ccounts;          // [n_q] candidate-side key counts (may exceed ccap: overflow)
 };
 
 struct SelectArgs {
@@ -58,6 +67,11 @@ struct SelectArgs {
   uint64_t* out_keys;     // [n_q][K]
   int32_t* overflow;      // [n_q]
   uint64_t* tau_key_out;  // [n_q]
+  // symmetric pass: the candidate-side list of each row, read as one more split
+  const uint64_t* cbufs;  // [n_q][ccap] or null
+  const int32_t* ccounts;
+  int32_t ccap;
+  int32_t pad0;
 };
 
 // ---- launchers (knn_exact.hip) ----
@@ -71,6 +85,10 @@ hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, int n_splits,
 hipError_t launch_pass2(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);
 hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int K, float* tau, hipStream_t s);
 hipError_t launch_select(const SelectArgs& a, hipStream_t s);
+// symmetric pass 2 over a.work (square tiles only: tile_rows_c == tile_rows_q)
+hipError_t launch_pass2_sym(int kind, const TileArgs& a, hipStream_t s);
+// zero ccounts of the queries in list[0..n)
+hipError_t launch_zero_list(int32_t* ccounts, const int32_t* list, int n, hipStream_t s);
 hipError_t launch_rescore(const float* X, int64_t ld, int d, int n_q, int q_offset,
                           const uint64_t* cand, int Kc, int k, uint64_t* out, hipStream_t s);
 hipError_t launch_finalize(const uint64_t* keys, int n_q, int q_offset, int k, int dtype,

@@ -144,6 +144,7 @@ evoc_mi355x_ctx::~evoc_mi355x_ctx() {
     (void)hipSetDevice(device);
     if (ev_begin) (void)hipEventDestroy(ev_begin);
     if (ev_end) (void)hipEventDestroy(ev_end);
+    while (!cache.empty()) free_cached(cache.size() - 1);
     if (stream) {
       (void)hipStreamSynchronize(stream);
       (void)hipStreamDestroy(stream);
@@ -170,17 +171,59 @@ DevBuf::~DevBuf() { release(); }
 
 void DevBuf::release() {
   if (ptr) {
-    (void)hipFreeAsync(ptr, owner->stream);
-    if (owner) owner->in_use -= bytes;
+    if (owner->serving) {
+      (void)hipFreeAsync(ptr, owner->stream);
+    } else {
+      owner->cache.push_back({ptr, bytes, owner->gen});
+      owner->cached += bytes;
+    }
+    owner->in_use -= bytes;
     ptr = nullptr;
     bytes = 0;
   }
+}
+
+void Ctx::free_cached(size_t i) {
+  (void)hipFreeAsync(cache[i].ptr, stream);
+  cached -= cache[i].bytes;
+  cache[i] = cache.back();
+  cache.pop_back();
+}
+
+void Ctx::free_stale() {
+  for (size_t i = cache.size(); i-- > 0;)
+    if (cache[i].gen < gen) free_cached(i);
 }
 
 int DevBuf::alloc(Ctx* c, size_t nbytes, const char* what) {
   release();
   if (nbytes == 0) nbytes = 256;
   nbytes = (nbytes + 255) & ~(size_t)255;
+  // a cached block of about this size (at most 1/8 + 1 MiB larger)
+  size_t best = SIZE_MAX;
+  for (size_t i = 0; i < c->cache.size(); i++) {
+    const size_t b = c->cache[i].bytes;
+    if (b >= nbytes && b <= nbytes + nbytes / 8 + (1u << 20) && c->in_use + b <= c->budget &&
+        (best == SIZE_MAX || b < c->cache[best].bytes))
+      best = i;
+  }
+  if (best != SIZE_MAX) {
+    ptr = c->cache[best].ptr;
+    bytes = c->cache[best].bytes;
+    owner = c;
+    c->cached -= bytes;
+    c->cache[best] = c->cache.back();
+    c->cache.pop_back();
+    c->in_use += bytes;
+    return EVOC_OK;
+  }
+  // room for a new block: cached blocks go first, largest first
+  while (!c->cache.empty() && c->in_use + c->cached + nbytes > c->budget) {
+    size_t big = 0;
+    for (size_t i = 1; i < c->cache.size(); i++)
+      if (c->cache[i].bytes > c->cache[big].bytes) big = i;
+    c->free_cached(big);
+  }
   if (c->in_use + nbytes > c->budget) {
     char buf[256];
     snprintf(buf, sizeof(buf), "HBM budget exceeded allocating %s: %zu bytes requested, %zu of %zu in use",
@@ -262,7 +305,11 @@ hipError_t staged_download(Ctx* c, void* dst, size_t dst_pitch, const void* src,
 
 extern "C" {
 
-const char* evoc_mi355x_version(void) { return "evoc_mi355x 0.1.0 (gfx950, abi 1)"; }
+#define EVOC_STR2(x) #x
+#define EVOC_STR(x) EVOC_STR2(x)
+const char* evoc_mi355x_version(void) {
+  return "evoc_mi355x 0.1.0 (gfx950, abi " EVOC_STR(EVOC_MI355X_ABI_VERSION) ")";
+}
 
 int evoc_mi355x_device_count(int32_t* count) {
   int c = 0;
@@ -317,6 +364,15 @@ int evoc_mi355x_create(int32_t device, uint64_t hbm_budget_bytes, evoc_mi355x_ct
 }
 
 void evoc_mi355x_destroy(evoc_mi355x_ctx* ctx) { delete ctx; }
+
+void evoc_mi355x_trim(evoc_mi355x_ctx* ctx) {
+  if (!ctx) return;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  while (!ctx->cache.empty()) ctx->free_cached(ctx->cache.size() - 1);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipMemPoolTrimTo(ctx->pool, 0);
+}
 
 const char* evoc_mi355x_last_error(const evoc_mi355x_ctx* ctx) {
   return ctx ? ctx->last_error.c_str() : "null context";
@@ -476,6 +532,7 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
                                             evoc_knn_output* out) {
   if (!ctx || !in || !o || !out) return EVOC_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(ctx->mu);
+  CallScope call(ctx);
   const auto t_start = std::chrono::steady_clock::now();
   ctx->last_error.clear();
   if (o->n_neighbors <= 0) return ctx->fail(EVOC_ERR_INVALID_NEIGHBORS, "n_neighbors must be positive");
@@ -616,6 +673,7 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
     if (rc) return rc;
     out->rescore_rounds = xs.extra_rounds;
     out->score_seconds = xs.pass2_ms * 1e-3;
+    out->score_pairs = xs.score_pairs;
   }
 
   const uint64_t* final_keys = (const uint64_t*)keys.ptr;

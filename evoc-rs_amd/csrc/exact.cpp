@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "evoc_host.h"
@@ -110,6 +111,66 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   return p;
 }
 
+// Symmetric plan (query rows = candidate rows, the whole matrix one slab).
+// Candidate tiles are cut into splits of tiles_per_split as for pass 2, and
+// query block Q takes split s's tiles from max(Q, s * tiles_per_split) on, so
+// each pair of tiles Q <= t is scored once; a row's keys from query blocks
+// before its own arrive on the candidate side (ccap slots per row).  Splits
+// are chosen so that about 16 work items run per resident workgroup slot (the
+// last dispatch round then costs little), at least 4 when there are enough
+// tiles.  Items run in groups of group_q query blocks, split-major inside a
+// group, as tile_coords orders pass 2: the items resident on an XCD at once
+// share candidate tiles (same split) and query tiles (same block).
+struct SymPlan {
+  Plan p;
+  int ccap = 0;
+  std::vector<int4> work;
+  size_t scratch_bytes(int64_t n) const {
+    auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return p.scratch_bytes(n, nqb) + r((size_t)n * ccap * 8) + r((size_t)n * 4) + r(work.size() * sizeof(int4));
+  }
+  int64_t nqb = 0;
+};
+
+SymPlan make_sym_plan(int kind, int64_t n, int K, int n_cu) {
+  SymPlan sp;
+  const int T = tile_rows_c(kind);
+  const int64_t nt = cdiv(n, T);
+  sp.nqb = nt;
+  sp.p = make_plan(kind, n, n, K, n_cu);
+  Plan& p = sp.p;
+  if (!p.pass1 || tile_rows_q(kind) != T || kind == EVOC_ELEM_U8BITS) return sp;
+  const int64_t per_cu = tile_lds_bytes(kind, 3) <= 80 * 1024 ? 2 : 1;
+  const int64_t slots = (int64_t)n_cu * per_cu;
+  const int64_t pairs = nt * (nt + 1) / 2;
+  int64_t tpi = std::min<int64_t>(nt, std::max<int64_t>(1, pairs / (16 * slots)));
+  if (nt >= 64 && cdiv(nt, tpi) < 4) tpi = cdiv(nt, 4);
+  p.tiles_per_split = (int)tpi;
+  p.n_splits = (int)cdiv(nt, tpi);
+  p.group_q = (int)std::max<int64_t>(1, std::max<int64_t>(slots / 8, 1) / p.n_splits);
+  const int64_t expect = (int64_t)p.stride * 2 * K;
+  int64_t caps = std::max<int64_t>(pow2ceil(cdiv(4 * expect, p.n_splits)), 128);
+  caps = std::max<int64_t>(caps, pow2ceil(cdiv(2LL * K, p.n_splits)));
+  p.caps = (int)std::min<int64_t>(caps, MAX_SORT);
+  // the candidate side of the last rows holds about all of their keys: 2x headroom
+  sp.ccap = (int)std::max<int64_t>(std::max<int64_t>(pow2ceil(2 * expect), pow2ceil(2LL * K)), 256);
+  const int64_t G = p.group_q;
+  for (int64_t g0 = 0; g0 < nt; g0 += G)
+    for (int64_t s = 0; s < p.n_splits; s++)
+      for (int64_t Q = g0; Q < std::min(nt, g0 + G); Q++) {
+        const int64_t t0 = std::max(Q, s * tpi), t1 = std::min(nt, (s + 1) * tpi);
+        if (t0 < t1) sp.work.push_back(make_int4((int)Q, (int)t0, (int)t1, (int)s));
+      }
+  return sp;
+}
+
+// The symmetric pass is on unless EVOC_MI355X_SYMMETRIC=0 (both passes give
+// the same graph; the switch lets tests run either).
+bool symmetric_enabled() {
+  const char* e = std::getenv("EVOC_MI355X_SYMMETRIC");
+  return !(e && e[0] == '0');
+}
+
 }  // namespace
 
 namespace {
@@ -161,6 +222,10 @@ uint64_t exact_scratch_bytes(int kind, int64_t n, int64_t row_bytes, int K, int 
   // the plan's bytes per query are not monotone in the slab size (more splits
   // for smaller slabs): take the largest over the slabs the search may use
   uint64_t b = slab_scratch(kind, n, n, K, n_cu);
+  if (!serving && symmetric_enabled()) {
+    const SymPlan sp = make_sym_plan(kind, n, K, n_cu);
+    if (!sp.work.empty()) b = std::max<uint64_t>(b, sp.scratch_bytes(n));
+  }
   if (serving)
     for (double ms : {8.0, 2.0}) b = std::max(b, slab_scratch(kind, n, time_slab(kind, n, row_bytes, ms), K, n_cu));
   return b;
@@ -178,13 +243,20 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
   const uint64_t avail = ctx->budget > ctx->in_use ? ctx->budget - ctx->in_use : 0;
   const int64_t mem_slab = exact_slab(kind, n, K, ctx->n_cu, avail);
   const int64_t n_ctiles = cdiv(n, TC);
+  // the symmetric pass when the whole matrix fits one slab with its buffers
+  SymPlan sp;
+  bool sym = false;
+  if (!ctx->serving && symmetric_enabled()) {
+    sp = make_sym_plan(kind, n, K, ctx->n_cu);
+    sym = !sp.work.empty() && sp.scratch_bytes(n) <= avail;
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   int rc = EVOC_OK;
   int64_t nq = 0;
   for (int64_t q0 = 0; q0 < n && rc == EVOC_OK; q0 += nq) {
-    int64_t slab = mem_slab;
+    int64_t slab = sym ? n : mem_slab;
     if (ctx->serving) {
       // about 8 ms of scoring per slab, 2 ms while other requests wait or run
       const bool busy = ctx->contention && ctx->contention->load(std::memory_order_relaxed) > 0;
@@ -192,12 +264,12 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
     }
     // a slab whose plan needs more than what is left is cut further
     const uint64_t left = ctx->budget > ctx->in_use ? ctx->budget - ctx->in_use : 0;
-    while (slab > 4 * TQ && slab_scratch(kind, n, std::min(slab, n - q0), K, ctx->n_cu) > left)
+    while (!sym && slab > 4 * TQ && slab_scratch(kind, n, std::min(slab, n - q0), K, ctx->n_cu) > left)
       slab = std::max<int64_t>(4 * TQ, cdiv(slab / 2, TQ) * TQ);
     nq = std::min(slab, n - q0);
     const int64_t nqb = cdiv(nq, TQ);
-    const Plan p = make_plan(kind, n, nq, K, ctx->n_cu);
-    DevBuf gmax, tau, bufs, counts, ovf, tkey, list;
+    const Plan p = sym ? sp.p : make_plan(kind, n, nq, K, ctx->n_cu);
+    DevBuf gmax, tau, bufs, counts, ovf, tkey, list, cbufs, ccounts, work;
     if (p.pass1) {
       if ((rc = gmax.alloc(ctx, (size_t)nq * p.ng * 4, "group maxima"))) break;
       if ((rc = tau.alloc(ctx, (size_t)nq * 4, "score bounds"))) break;
@@ -207,6 +279,11 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
     if ((rc = ovf.alloc(ctx, (size_t)nq * 4, "overflow flags"))) break;
     if ((rc = tkey.alloc(ctx, (size_t)nq * 8, "key bounds"))) break;
     if ((rc = list.alloc(ctx, (size_t)(nqb + nq) * 4, "re-score lists"))) break;
+    if (sym) {
+      if ((rc = cbufs.alloc(ctx, (size_t)n * sp.ccap * 8, "candidate-side buffers"))) break;
+      if ((rc = ccounts.alloc(ctx, (size_t)n * 4, "candidate-side counts"))) break;
+      if ((rc = work.alloc(ctx, sp.work.size() * sizeof(int4), "work items"))) break;
+    }
 
     TileArgs a = {};
     a.q_rows = rows + q0 * row_bytes;
@@ -255,8 +332,26 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
     sa.overflow = (int32_t*)ovf.ptr;
     sa.tau_key_out = (uint64_t*)tkey.ptr;
 
+    if (sym) {
+      // query blocks skip the splits before their own tile: those counts stay 0
+      e = upload_sync(work.ptr, sp.work.data(), sp.work.size() * sizeof(int4), s);
+      if (e == hipSuccess) e = hipMemsetAsync(counts.ptr, 0, (size_t)p.n_splits * nq * 4, s);
+      if (e == hipSuccess) e = hipMemsetAsync(ccounts.ptr, 0, (size_t)n * 4, s);
+      if (e != hipSuccess) {
+        rc = ctx->hip_fail(e, "scoring pass");
+        break;
+      }
+      a2.work = (const int4*)work.ptr;
+      a2.n_work = (int32_t)sp.work.size();
+      a2.cbufs = (uint64_t*)cbufs.ptr;
+      a2.ccounts = (int32_t*)ccounts.ptr;
+      a2.ccap = sp.ccap;
+      sa.cbufs = (const uint64_t*)cbufs.ptr;
+      sa.ccounts = (const int32_t*)ccounts.ptr;
+      sa.ccap = sp.ccap;
+    }
     (void)hipEventRecord(e0, s);
-    e = launch_pass2(kind, a2, (int)nqb, p.n_splits, s);
+    e = sym ? launch_pass2_sym(kind, a2, s) : launch_pass2(kind, a2, (int)nqb, p.n_splits, s);
     (void)hipEventRecord(e1, s);
     if (e == hipSuccess) e = launch_select(sa, s);
     if (e != hipSuccess) {
@@ -274,7 +369,10 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       if (round == 0) {
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        if (stats) stats->pass2_ms += ms;
+        if (stats) {
+          stats->pass2_ms += ms;
+          stats->score_pairs += sym ? n * (n + 1) / 2 : nq * n;
+        }
       }
       std::vector<int32_t> blocks;
       for (int64_t b = 0; b < nqb; b++) {
@@ -297,7 +395,10 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       int32_t* d_q = d_blocks + nqb;
       e = upload_sync(d_blocks, blocks.data(), blocks.size() * 4, s);
       if (e == hipSuccess) e = upload_sync(d_q, qlist.data(), qlist.size() * 4, s);
+      // the re-scored queries' candidate-side lists are covered by pass 2 now
+      if (e == hipSuccess && sym) e = launch_zero_list((int32_t*)ccounts.ptr, d_q, (int)qlist.size(), s);
       TileArgs ar = a2;
+      ar.work = nullptr;
       ar.block_list = d_blocks;
       ar.tau = nullptr;
       ar.tau_key = (const uint64_t*)tkey.ptr;

@@ -409,6 +409,179 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
 }
 
 // ---------------------------------------------------------------------------
+// Symmetric pass 2.  Every score here is symmetric (s(q, c) = s(c, q): the
+// same products in the same K order), so each unordered pair of tiles is
+// scored once: the workgroup of work item (Q, [t0, t1), split) scores query
+// block Q against candidate tiles t >= Q and tests every value twice, against
+// the query row's bound (appended to the workgroup's own split buffer, LDS
+// counters, as pass 2) and, off the diagonal tile, against the candidate row's
+// bound (the key of the pair seen from the candidate row, with the query as
+// neighbour).  Candidate-side keys are staged per candidate row in LDS and
+// flushed once per tile with one global atomicAdd per row that has any; a row
+// with more than SYM_SLOTS keys in a tile appends the rest directly.
+// ---------------------------------------------------------------------------
+constexpr int SYM_SLOTS = 8;
+
+template <int KIND, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  using En = Engine<KIND, BM, BN, WM, WN>;
+  using M = Mfma<KIND>;
+  static_assert(En::TC == En::TQ, "the symmetric pass needs square tiles");
+  constexpr int T = En::TC;
+  // LDS after the ring: query rows (counters, bounds, fast bounds, key bounds),
+  // candidate rows (bounds, fast bounds, key bounds, staged counts), staged keys
+  int* cnt = (int*)(lds + En::LDS);
+  float* tau_l = (float*)(cnt + T);
+  float* tfast_l = tau_l + T;
+  uint64_t* tkey_l = (uint64_t*)(tfast_l + T);
+  float* ctau_l = (float*)(tkey_l + T);
+  float* cfast_l = ctau_l + T;
+  uint64_t* ckey_l = (uint64_t*)(cfast_l + T);
+  int* scnt = (int*)(ckey_l + T);
+  uint64_t* stage = (uint64_t*)(scnt + T);
+  typename En::acc_t acc[BM][BN];
+  const int lane = threadIdx.x & 63;
+  const int wm = En::wave_m(), wn = En::wave_n();
+  const int per_xcd = (int)gridDim.x >> 3;
+  const int logical = ((int)blockIdx.x & 7) * per_xcd + ((int)blockIdx.x >> 3);
+  if (logical >= a.n_work) return;
+  const int4 w = a.work[logical];
+  const int q0 = w.x * T, split = w.w;
+  auto bounds = [&](int r, float& tau, float& fast, uint64_t& key) __attribute__((always_inline)) {
+    tau = (r < a.n_q && a.tau) ? a.tau[r] : -INFINITY;
+    key = (r < a.n_q && a.tau_key) ? a.tau_key[r] : 0ull;
+    float tb = tau;
+    if (a.tau_key && key != EVOC_KEY_NONE) tb = fmaxf(tb, key_score(key));
+    fast = fast_bound<KIND>(tb);
+  };
+  for (int i = threadIdx.x; i < T; i += En::THREADS) {
+    cnt[i] = 0;
+    bounds(q0 + i, tau_l[i], tfast_l[i], tkey_l[i]);
+  }
+  for (int t = w.y; t < w.z; t++) {
+    const int c0 = t * T;
+    const bool diag = t == w.x;  // the query block's own tile: query side only
+    __syncthreads();             // the last tile's epilogue and flush are done with these
+    if (!diag)
+      for (int i = threadIdx.x; i < T; i += En::THREADS) {
+        scnt[i] = 0;
+        bounds(c0 + i, ctau_l[i], cfast_l[i], ckey_l[i]);
+      }
+    En::score_tile(acc, lds, a, c0, q0);  // (begins with a barrier, which publishes them)
+    int lane_e = lane, wm_e = wm, wn_e = wn;
+    asm volatile("" : "+v"(lane_e), "+v"(wm_e), "+v"(wn_e));
+    static_for<BM>([&](auto bmc) __attribute__((always_inline)) {
+      constexpr int bm = decltype(bmc)::value;
+      __builtin_amdgcn_sched_barrier(0);
+      const int cb = c0 + wm_e * 32 * BM + bm * 32;  // first candidate of the block row
+      // candidate-side fast bounds of this lane's 16 rows 8*(j/4) + 4*(lane/32) + j%4
+      float cf[16];
+      const int rl = cb - c0 + 4 * (lane_e >> 5);
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const float4 v = diag ? make_float4(INFINITY, INFINITY, INFINITY, INFINITY)
+                              : *(const float4*)(cfast_l + rl + 8 * g);
+        cf[4 * g] = v.x;
+        cf[4 * g + 1] = v.y;
+        cf[4 * g + 2] = v.z;
+        cf[4 * g + 3] = v.w;
+      }
+      static_for<BN>([&](auto bnc) __attribute__((always_inline)) {
+        constexpr int bn = decltype(bnc)::value;
+        __builtin_amdgcn_sched_barrier(0);
+        const int ql = wn_e * 32 * BN + bn * 32 + (lane_e & 31);
+        const int q = q0 + ql;
+        const float tfast = tfast_l[ql];
+        const bool dblk = diag && cb <= q + 31 - (lane_e & 31) && q - (lane_e & 31) <= cb + 31;
+        uint32_t m = 0, mc = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const float raw = (float)acc[bm][bn][j];
+          m |= (raw >= tfast ? 1u : 0u) << j;
+          mc |= (raw >= cf[j] ? 1u : 0u) << j;
+        }
+        if (dblk) m = 0xffffu;  // the self pair takes the exact path
+        if (__ballot((m | mc) != 0) == 0) return;
+        if (m) {  // query side, as pass 2
+          const float tau = tau_l[ql];
+          const uint64_t tkey = tkey_l[ql];
+          uint32_t take = 0;
+#pragma unroll
+          for (int j = 0; j < 16; j++) {
+            if (m & (1u << j)) {
+              const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+              const bool self = (c == q);
+              const float sc = self ? 1.0f : M::value(acc[bm][bn], j);
+              bool tk = (c < a.n_c) && (q < a.n_q) && (sc >= tau) && !(self && a.skip_self);
+              if (tk && a.tau_key) tk = make_key(sc, c, self) >= tkey;
+              take |= (tk ? 1u : 0u) << j;
+            }
+          }
+          if (take) {
+            int pos = atomicAdd(&cnt[ql], __popc(take));
+            uint64_t* dst = a.bufs + ((int64_t)split * a.n_q + q) * a.caps;
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+              if (take & (1u << j)) {
+                const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+                const bool self = (c == q);
+                if (pos < a.caps) dst[pos] = make_key(self ? 1.0f : M::value(acc[bm][bn], j), c, self);
+                pos++;
+              }
+            }
+          }
+        }
+        if (mc && q < a.n_q) {  // candidate side: the pair seen from row c (never self here)
+#pragma unroll
+          for (int j = 0; j < 16; j++) {
+            if (mc & (1u << j)) {
+              const int cr = cb - c0 + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+              const int c = c0 + cr;
+              const float sc = M::value(acc[bm][bn], j);
+              const uint64_t key = make_key(sc, q, false);
+              bool tk = (c < a.n_c) && (sc >= ctau_l[cr]);
+              if (tk && a.tau_key) tk = key >= ckey_l[cr];
+              if (tk) {
+                const int slot = atomicAdd(&scnt[cr], 1);
+                if (slot < SYM_SLOTS) {
+                  stage[cr * SYM_SLOTS + slot] = key;
+                } else {
+                  const int pos = atomicAdd(&a.ccounts[c], 1);
+                  if (pos < a.ccap) a.cbufs[(int64_t)c * a.ccap + pos] = key;
+                }
+              }
+            }
+          }
+        }
+      });
+    });
+    if (!diag) {
+      __syncthreads();  // every wave's staged keys are in
+      for (int i = threadIdx.x; i < T; i += En::THREADS) {
+        const int k = min(scnt[i], SYM_SLOTS);
+        const int c = c0 + i;
+        if (k > 0 && c < a.n_c) {
+          const int pos = atomicAdd(&a.ccounts[c], k);
+          for (int j = 0; j < k; j++)
+            if (pos + j < a.ccap) a.cbufs[(int64_t)c * a.ccap + pos + j] = stage[i * SYM_SLOTS + j];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < T; i += En::THREADS) {
+    const int q = q0 + i;
+    if (q < a.n_q) a.counts[(int64_t)split * a.n_q + q] = cnt[i];
+  }
+}
+
+__global__ void zero_list_kernel(int32_t* __restrict__ p, const int32_t* __restrict__ list, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[list[i]] = 0;
+}
+
+// ---------------------------------------------------------------------------
 // Binary Jaccard tiles (uint8.rs:21-36) on the VALU: 16 x 16 threads, each
 // owning 8 candidates x 8 queries; popcount of AND per 32-bit word, union from
 // the per-row popcounts.  Same two epilogues as the MFMA kernels.
@@ -594,12 +767,10 @@ __device__ __forceinline__ void select_body(const SelectArgs& a, int q, bool ove
   do {
     const int take = min(CAP - keep, total - g0);
     // keys g0 .. g0 + take - 1 go to positions keep .. keep + take - 1
-    int base = 0;  // index of split s's first key
-    for (int s = 0; s < a.n_splits; s++) {
-      const int c = min(a.counts[(int64_t)s * a.n_q + q], a.caps);
+    int base = 0;  // index of the current segment's first key
+    auto segment = [&](const uint64_t* src, int c) __attribute__((always_inline)) {
       const int lo = max(base, g0), hi = min(base + c, g0 + take);
       if (lo < hi) {
-        const uint64_t* src = a.bufs + ((int64_t)s * a.n_q + q) * a.caps;
 #pragma unroll
         for (int e = 0; e < E; e++) {
           const int gi = g0 + (e * 64 + lane - keep);
@@ -607,7 +778,10 @@ __device__ __forceinline__ void select_body(const SelectArgs& a, int q, bool ove
         }
       }
       base += c;
-    }
+    };
+    for (int s = 0; s < a.n_splits; s++)
+      segment(a.bufs + ((int64_t)s * a.n_q + q) * a.caps, min(a.counts[(int64_t)s * a.n_q + q], a.caps));
+    if (a.cbufs) segment(a.cbufs + (int64_t)q * a.ccap, min(a.ccounts[q], a.ccap));
 #pragma unroll
     for (int e = 0; e < E; e++)
       if (e * 64 + lane >= keep + take) v[e] = EVOC_KEY_NONE;
@@ -646,6 +820,11 @@ __global__ __launch_bounds__(256) void select_topk_kernel(SelectArgs a) {
     const int c = a.counts[(int64_t)s * a.n_q + q];
     overflow |= c > a.caps;
     total += min(c, a.caps);
+  }
+  if (a.cbufs) {
+    const int c = a.ccounts[q];
+    overflow |= c > a.ccap;
+    total += min(c, a.ccap);
   }
   // sort width: the query's own key count, or the widest sort streamed over
   // its keys (which must leave room beside the K kept)
@@ -814,8 +993,12 @@ size_t tile_lds_bytes(int kind, int pass) {
   if (kind == EVOC_ELEM_U8BITS) return 2 * TILE * JAC_PITCH + (pass == 2 ? TILE * sizeof(int) : 0);
   int tc, tq, g;
   shape_dims(kind, tc, tq, g);
-  // pass 2 adds per-query counters, score bounds, fast bounds and key bounds
-  return (size_t)MFMA_NS * (tc + tq) * MFMA_SB + (pass == 2 ? (size_t)tq * (4 + 4 + 4 + 8) : 0);
+  const size_t ring = (size_t)MFMA_NS * (tc + tq) * MFMA_SB;
+  // pass 2 adds per-query counters, score bounds, fast bounds and key bounds;
+  // the symmetric pass (3) adds the candidate rows' bounds, staged counts and
+  // SYM_SLOTS staged keys per candidate row
+  if (pass == 3) return ring + (size_t)tq * 40 + (size_t)tc * SYM_SLOTS * 8;
+  return ring + (pass == 2 ? (size_t)tq * (4 + 4 + 4 + 8) : 0);
 }
 
 template <class Kern>
@@ -884,6 +1067,36 @@ hipError_t launch_pass2(int kind, const TileArgs& a0, int n_blocks, int n_splits
     case EVOC_ELEM_U8BITS: return launch_tile(jac_pass2_kernel, lds, g, s, a);
   }
   return hipErrorInvalidValue;
+}
+
+template <int KIND>
+static hipError_t launch_pass2_sym_kind(const TileArgs& a, dim3 g, size_t lds, hipStream_t s) {
+  switch (tile_shape(KIND)) {
+    case 2: return launch_tile(pass2_sym_kernel<KIND, 2, 4, 4, 2>, lds, g, s, a, 512);
+    case 3: return launch_tile(pass2_sym_kernel<KIND, 4, 4, 2, 2>, lds, g, s, a);
+    case 0: return launch_tile(pass2_sym_kernel<KIND, 2, 2, 2, 2>, lds, g, s, a);
+    default: return hipErrorInvalidValue;  // 128 x 256 tiles are not square
+  }
+}
+
+hipError_t launch_pass2_sym(int kind, const TileArgs& a, hipStream_t s) {
+  int tc, tq, g;
+  shape_dims(kind, tc, tq, g);
+  if (tc != tq || kind == EVOC_ELEM_U8BITS) return hipErrorInvalidValue;
+  const size_t lds = tile_lds_bytes(kind, 3);
+  const dim3 grid((unsigned)(((int64_t)a.n_work + 7) / 8 * 8));
+  switch (kind) {
+    case EVOC_ELEM_BF16: return launch_pass2_sym_kind<EVOC_ELEM_BF16>(a, grid, lds, s);
+    case EVOC_ELEM_F32: return launch_pass2_sym_kind<EVOC_ELEM_F32>(a, grid, lds, s);
+    case EVOC_ELEM_I8: return launch_pass2_sym_kind<EVOC_ELEM_I8>(a, grid, lds, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_zero_list(int32_t* p, const int32_t* list, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_list_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, list, n);
+  return hipGetLastError();
 }
 
 static int pow2_per_lane(int entries) {

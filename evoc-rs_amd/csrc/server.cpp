@@ -180,6 +180,18 @@ void evoc_mi355x_server_destroy(evoc_mi355x_server* s) {
   delete s;
 }
 
+int evoc_mi355x_server_request_bytes(const evoc_mi355x_server* s, const evoc_knn_input* in,
+                                     const evoc_knn_options* opts, int32_t out_location, uint64_t* full_bytes,
+                                     uint64_t* min_bytes) {
+  if (!s || !full_bytes || !min_bytes) return EVOC_ERR_INVALID_ARGUMENT;
+  const int rc = evoc_mi355x_request_bytes(s->slots[0], in, opts, out_location, full_bytes, min_bytes);
+  if (rc != EVOC_OK) return rc;
+  // DevBuf rounding and small buffers of the search: keep 1% + 4 MiB headroom
+  *full_bytes += *full_bytes / 100 + (4u << 20);
+  *min_bytes += *min_bytes / 100 + (4u << 20);
+  return EVOC_OK;
+}
+
 int evoc_mi355x_submit(evoc_mi355x_server* s, const evoc_knn_input* in, const evoc_knn_options* opts,
                        const int64_t tau_state[3], const int64_t* tree_states, int64_t n_tree_rows,
                        evoc_knn_output* out, uint64_t* ticket) {
@@ -194,11 +206,8 @@ int evoc_mi355x_submit(evoc_mi355x_server* s, const evoc_knn_input* in, const ev
   std::memcpy(r->tau, tau_state, sizeof(r->tau));
   if (tree_states && n_tree_rows > 0) r->trees.assign(tree_states, tree_states + 3 * n_tree_rows);
   r->out = out;
-  const int rc = evoc_mi355x_request_bytes(s->slots[0], in, opts, out->location, &r->need_full, &r->need_min);
+  const int rc = evoc_mi355x_server_request_bytes(s, in, opts, out->location, &r->need_full, &r->need_min);
   if (rc != EVOC_OK) return rc;
-  // DevBuf rounding and small buffers of the search: keep 1% + 4 MiB headroom
-  r->need_full += r->need_full / 100 + (4u << 20);
-  r->need_min += r->need_min / 100 + (4u << 20);
   if (r->need_min > s->budget) return EVOC_ERR_OUT_OF_BUDGET;
   r->t_submit = Clock::now();
   std::lock_guard<std::mutex> lk(s->mu);

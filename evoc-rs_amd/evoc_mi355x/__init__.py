@@ -86,7 +86,8 @@ class _Output(C.Structure):
                 ("distances", C.c_void_p), ("capacity_k", C.c_int64), ("k_out", C.c_int64),
                 ("seconds", C.c_double), ("device_seconds", C.c_double), ("score_seconds", C.c_double),
                 ("iterations", C.c_int64), ("rescore_rounds", C.c_int64), ("offers_dropped", C.c_int64),
-                ("queue_seconds", C.c_double)]
+                ("queue_seconds", C.c_double),
+                ("score_pairs", C.c_int64)]
 
 
 _lib = None
@@ -99,6 +100,7 @@ SIGNATURES = {
     "evoc_mi355x_create": ([C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
     "evoc_mi355x_destroy": ([_P], None),
     "evoc_mi355x_last_error": ([_P], C.c_char_p),
+    "evoc_mi355x_trim": ([_P], None),
     "evoc_knn_options_default": ([C.POINTER(_Options)], None),
     "evoc_knn_effective_k": ([C.c_int64, C.c_int64], C.c_int64),
     "evoc_knn_rng_draws": ([C.c_int32, C.c_int64, C.POINTER(_Options)], C.c_int64),
@@ -116,6 +118,8 @@ SIGNATURES = {
                             C.POINTER(_Output), C.POINTER(C.c_uint64)], C.c_int),
     "evoc_mi355x_poll": ([_P, C.c_uint64], C.c_int),
     "evoc_mi355x_wait": ([_P, C.c_uint64, C.c_double, C.c_char_p, C.c_int64], C.c_int),
+    "evoc_mi355x_server_request_bytes": ([_P, C.POINTER(_Input), C.POINTER(_Options), C.c_int32,
+                                          C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
     "evoc_mi355x_server_stats": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)], None),
     "evoc_numpy_rng_new": ([C.c_uint32], _P),
@@ -296,6 +300,10 @@ class Context:
     def last_error(self) -> str:
         return lib().evoc_mi355x_last_error(self._h).decode()
 
+    def trim(self) -> None:
+        """Return the device scratch kept from the last call (evoc_mi355x_trim)."""
+        lib().evoc_mi355x_trim(self._h)
+
 
 _default_ctx: Optional[Context] = None
 
@@ -316,6 +324,7 @@ class CallStats:
     rescore_rounds: int = 0
     offers_dropped: int = 0
     queue_seconds: float = 0.0
+    score_pairs: int = 0
     extra: dict = field(default_factory=dict)
 
 
@@ -340,6 +349,7 @@ def _fill_stats(stats, res) -> None:
     stats.rescore_rounds = res.rescore_rounds
     stats.offers_dropped = res.offers_dropped
     stats.queue_seconds = res.queue_seconds
+    stats.score_pairs = res.score_pairs
 
 
 def _input_from(data, dtype_code=None):
@@ -577,6 +587,20 @@ class Server:
         if rc != OK:
             raise BackendError(rc, f"submit failed with status {rc}")
         return Ticket(self, t.value, res, (keep, res, tau, trees, o), result)
+
+    def request_bytes(self, data, options: KnnGraphOptions, out_on_device: bool = False):
+        """(full, minimum) device bytes this server's admission reserves for such a request
+        (evoc_mi355x_server_request_bytes)."""
+        inp, keep = _input_from(data)
+        o = options._c()
+        full, low = C.c_uint64(), C.c_uint64()
+        rc = lib().evoc_mi355x_server_request_bytes(self._h, C.byref(inp), C.byref(o),
+                                                    MEM_DEVICE if out_on_device else MEM_HOST,
+                                                    C.byref(full), C.byref(low))
+        del keep
+        if rc != OK:
+            raise BackendError(rc, "invalid request")
+        return int(full.value), int(low.value)
 
     def stats(self):
         q, r, g, b = C.c_int64(), C.c_int64(), C.c_uint64(), C.c_uint64()

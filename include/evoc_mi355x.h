@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define EVOC_MI355X_ABI_VERSION 2
+#define EVOC_MI355X_ABI_VERSION 3
 
 /* EmbeddingData::{Float32, Int8, UInt8}, src/knn.rs:16-20.  UInt8 rows are
  * packed bits compared with bitwise Jaccard; d counts bytes. */
@@ -103,6 +103,8 @@ typedef struct evoc_knn_output {
   int64_t rescore_rounds;  /* exact: extra scoring rounds for overflowing buffers */
   int64_t offers_dropped;  /* NN-descent: candidate offers lost to full per-vertex buffers */
   double queue_seconds;    /* serving: time from submit until a slot started the request */
+  int64_t score_pairs;     /* exact: row pairs the scoring kernel scored (one per unordered pair
+                              in the symmetric pass, one per query and candidate otherwise) */
 } evoc_knn_output;
 
 typedef struct evoc_mi355x_ctx evoc_mi355x_ctx;
@@ -115,6 +117,10 @@ int evoc_mi355x_device_count(int32_t* count);
 int evoc_mi355x_create(int32_t device, uint64_t hbm_budget_bytes, evoc_mi355x_ctx** out);
 void evoc_mi355x_destroy(evoc_mi355x_ctx* ctx);
 const char* evoc_mi355x_last_error(const evoc_mi355x_ctx* ctx);
+/* A context keeps the device scratch of its last call for the next one (calls
+ * of the same shape then allocate nothing); this returns it to the device.
+ * No counterpart in the reference (its CPU path has no device memory). */
+void evoc_mi355x_trim(evoc_mi355x_ctx* ctx);
 
 /* ---- options and contract helpers ---- */
 void evoc_knn_options_default(evoc_knn_options* opts);           /* KnnGraphOptions::default() */
@@ -185,6 +191,11 @@ int evoc_mi355x_poll(evoc_mi355x_server* srv, uint64_t ticket);
  * (err_len bytes, may be NULL) and releases the ticket */
 int evoc_mi355x_wait(evoc_mi355x_server* srv, uint64_t ticket, double timeout_seconds, char* err,
                      int64_t err_len);
+/* the (full, minimum) device bytes the server's admission reserves for such a
+ * request: evoc_mi355x_request_bytes on a serving slot plus the server's headroom */
+int evoc_mi355x_server_request_bytes(const evoc_mi355x_server* srv, const evoc_knn_input* in,
+                                     const evoc_knn_options* opts, int32_t out_location,
+                                     uint64_t* full_bytes, uint64_t* min_bytes);
 /* requests queued / running and bytes granted to running requests */
 void evoc_mi355x_server_stats(const evoc_mi355x_server* srv, int64_t* queued, int64_t* running,
                               uint64_t* granted_bytes, uint64_t* budget_bytes);

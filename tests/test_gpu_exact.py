@@ -224,6 +224,49 @@ def test_massive_exact_ties(ctx, dtype):
     np.testing.assert_array_equal(idx, ei)
 
 
+@pytest.mark.parametrize("dtype,prec_name", [("f32", "bf16"), ("f32", "fp32"), ("i8", "-")])
+@pytest.mark.parametrize("n", [9000, 5137])
+def test_symmetric_pass_matches_split_pass(ctx, dtype, prec_name, n):
+    """The symmetric scoring pass (each pair of tiles scored once, keys appended to both rows)
+    and the per-split pass (every query against every candidate) give identical graphs: the
+    score of a pair is the same products summed in the same order either way.  n = 5137
+    leaves a ragged last tile; int8 is also checked against the oracle."""
+    print(f"[gpu] symmetric vs split {dtype} {prec_name} n={n}", flush=True)
+    prec = dict(PRECS).get(prec_name, E.PREC_FP32)
+    x = (synth.blobs_f32(n, 192, 30, 0.3, seed=n) if dtype == "f32"
+         else synth.blobs_i8(n, 192, 30, 0.3, seed=n))
+    out = {}
+    for sym in ("1", "0"):
+        os.environ["EVOC_MI355X_SYMMETRIC"] = sym
+        try:
+            out[sym] = run(ctx, x, 5, n_neighbors=25, precision=prec)
+        finally:
+            del os.environ["EVOC_MI355X_SYMMETRIC"]
+    np.testing.assert_array_equal(out["1"][0], out["0"][0])
+    np.testing.assert_array_equal(out["1"][1].view(np.uint32), out["0"][1].view(np.uint32))
+    if dtype == "i8":
+        ei, _ = O.exact_knn(x, 25)
+        np.testing.assert_array_equal(out["1"][0], ei)
+
+
+def test_scratch_kept_between_calls_and_trimmed(ctx):
+    """A context reuses its last call's scratch blocks (same shape: same graph), frees the
+    ones a call of another shape does not reuse, and gives all back on trim."""
+    xa = synth.blobs_f32(7000, 256, 20, 0.3, seed=11)
+    xb = synth.blobs_i8(3000, 96, 10, 0.3, seed=12)
+    a1 = run(ctx, xa, 2, n_neighbors=20)
+    a2 = run(ctx, xa, 2, n_neighbors=20)
+    b1 = run(ctx, xb, 2, n_neighbors=10)
+    ctx.trim()
+    a3 = run(ctx, xa, 2, n_neighbors=20)
+    ctx.trim()
+    b2 = run(ctx, xb, 2, n_neighbors=10)
+    for u, v in ((a1, a2), (a1, a3), (b1, b2)):
+        np.testing.assert_array_equal(u[0], v[0])
+        np.testing.assert_array_equal(u[1].view(np.uint32), v[1].view(np.uint32))
+    np.testing.assert_array_equal(b1[0], O.exact_knn(xb, 10)[0])
+
+
 def test_errors(ctx):
     with pytest.raises(E.InvalidNeighbors):
         run(ctx, np.ones((4, 4), np.float32), n_neighbors=0)

@@ -67,7 +67,9 @@ def test_admission_by_budget():
     never grants more than the budget, and refuses a request that can never fit."""
     xs = [synth.blobs_f32(6000, 256, 20, 0.3, seed=s) for s in range(6)]
     o = E.KnnGraphOptions(n_neighbors=15)
-    full, low = E.request_bytes(xs[0], o)
+    probe = E.Server(0, slots=1)
+    full, low = probe.request_bytes(xs[0], o)
+    probe.close()
     budget = int(full * 1.5)
     print(f"[gpu] admission: request {full / 2**20:.1f} MiB (min {low / 2**20:.1f}), "
           f"budget {budget / 2**20:.1f} MiB", flush=True)
