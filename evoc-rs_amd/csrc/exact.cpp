@@ -147,7 +147,11 @@ SymPlan make_sym_plan(int kind, int64_t n, int K, int n_cu) {
   if (nt >= 64 && cdiv(nt, tpi) < 4) tpi = cdiv(nt, 4);
   p.tiles_per_split = (int)tpi;
   p.n_splits = (int)cdiv(nt, tpi);
-  p.group_q = (int)std::max<int64_t>(1, std::max<int64_t>(slots / 8, 1) / p.n_splits);
+  // the items resident on one XCD at a time form a block of about group_q query
+  // blocks x 8 splits: each candidate tile of a step is then fetched once for
+  // group_q items, each query tile once for its whole row of splits
+  p.group_q = (int)std::max<int64_t>(1, std::max<int64_t>(slots / 8, 1) / std::min<int64_t>(p.n_splits, 8));
+  if (const char* e = std::getenv("EVOC_MI355X_SYM_GROUP")) p.group_q = std::max(1, std::atoi(e));
   const int64_t expect = (int64_t)p.stride * 2 * K;
   int64_t caps = std::max<int64_t>(pow2ceil(cdiv(4 * expect, p.n_splits)), 128);
   caps = std::max<int64_t>(caps, pow2ceil(cdiv(2LL * K, p.n_splits)));
