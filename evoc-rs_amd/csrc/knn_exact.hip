@@ -261,6 +261,27 @@ __device__ __forceinline__ float fast_bound(float tau) {
   return tau > -1.17549435e-38f ? tau : -INFINITY;
 }
 
+// Whether the cheap bound test of a row is its exact test for every value
+// outside the self pair: no key bound (first round), and integer scores (the
+// score is the accumulator as float) or a positive bound (an f32 score is
+// dot > 0 ? dot : -FLT_MIN, so for tau > 0 the raw dot reaches tau exactly
+// when the score does).  Rows past the end carry an infinite bound: no value
+// passes on their side.
+template <int KIND>
+__device__ __forceinline__ bool bound_is_exact(float tau, bool key_bound) {
+  return !key_bound && (KIND == EVOC_ELEM_I8 || tau > 0.0f);
+}
+
+// Bits j of a lane's 16 accumulator rows (candidates cb + 8 (j / 4) +
+// 4 (lane / 32) + j % 4) that are real candidates (< n_c).
+__device__ __forceinline__ uint32_t valid_rows(int cb, int lane, int n_c) {
+  if (cb + 32 <= n_c) return 0xffffu;
+  uint32_t v = 0;
+#pragma unroll
+  for (int j = 0; j < 16; j++) v |= (cb + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3) < n_c ? 1u : 0u) << j;
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // Pass 1: group maxima over a strided candidate sample (GROUPS per tile:
 // wave row half x BM blocks x lane half, 16 candidates each).
@@ -332,7 +353,7 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
   const int q0 = qb * En::TQ;
   for (int i = threadIdx.x; i < En::TQ; i += En::THREADS) {
     const int q = q0 + i;
-    const float tau = (q < a.n_q && a.tau) ? a.tau[q] : -INFINITY;
+    const float tau = q < a.n_q ? (a.tau ? a.tau[q] : -INFINITY) : INFINITY;
     const uint64_t tkey = (q < a.n_q && a.tau_key) ? a.tau_key[q] : 0ull;
     float tb = tau;
     if (a.tau_key && tkey != EVOC_KEY_NONE) tb = fmaxf(tb, key_score(tkey));
@@ -369,19 +390,24 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
         for (int j = 0; j < 16; j++) m |= ((float)acc[bm][bn][j] >= tfast ? 1u : 0u) << j;
         if (diag) m = 0xffffu;  // the self pair takes the exact path
         if (__ballot(m != 0) == 0) return;
-        // the exact test on those values
         const float tau = tau_l[ql];
-        const uint64_t tkey = tkey_l[ql];
-        uint32_t take = 0;
+        uint32_t take;
+        if (!diag && bound_is_exact<KIND>(tau, a.tau_key != nullptr)) {
+          take = m & valid_rows(cb, lane_e, a.n_c);
+        } else {
+          // the exact test on those values
+          const uint64_t tkey = tkey_l[ql];
+          take = 0;
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-          if (m & (1u << j)) {
-            const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
-            const bool self = (c == qg);
-            const float sc = self ? 1.0f : M::value(acc[bm][bn], j);
-            bool t = (c < a.n_c) && (q < a.n_q) && (sc >= tau) && !(self && a.skip_self);
-            if (t && a.tau_key) t = make_key(sc, c, self) >= tkey;
-            take |= (t ? 1u : 0u) << j;
+          for (int j = 0; j < 16; j++) {
+            if (m & (1u << j)) {
+              const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+              const bool self = (c == qg);
+              const float sc = self ? 1.0f : M::value(acc[bm][bn], j);
+              bool t = (c < a.n_c) && (q < a.n_q) && (sc >= tau) && !(self && a.skip_self);
+              if (t && a.tau_key) t = make_key(sc, c, self) >= tkey;
+              take |= (t ? 1u : 0u) << j;
+            }
           }
         }
         // one slot reservation per lane and block, then the appends
@@ -429,8 +455,11 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
   using M = Mfma<KIND>;
   static_assert(En::TC == En::TQ, "the symmetric pass needs square tiles");
   constexpr int T = En::TC;
+  static_assert(En::THREADS >= T && T % 64 == 0, "one candidate row per thread of the first T / 64 waves");
   // LDS after the ring: query rows (counters, bounds, fast bounds, key bounds),
-  // candidate rows (bounds, fast bounds, key bounds, staged counts), staged keys
+  // candidate rows (bounds, fast bounds, key bounds, staged counts), staged
+  // keys, and one flag per wave of the candidate-row loaders (all of its rows'
+  // bounds exact)
   int* cnt = (int*)(lds + En::LDS);
   float* tau_l = (float*)(cnt + T);
   float* tfast_l = tau_l + T;
@@ -440,6 +469,7 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
   uint64_t* ckey_l = (uint64_t*)(cfast_l + T);
   int* scnt = (int*)(ckey_l + T);
   uint64_t* stage = (uint64_t*)(scnt + T);
+  int* cexact = (int*)(stage + T * SYM_SLOTS);
   typename En::acc_t acc[BM][BN];
   const int lane = threadIdx.x & 63;
   const int wm = En::wave_m(), wn = En::wave_n();
@@ -448,29 +478,63 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
   if (logical >= a.n_work) return;
   const int4 w = a.work[logical];
   const int q0 = w.x * T, split = w.w;
-  auto bounds = [&](int r, float& tau, float& fast, uint64_t& key) __attribute__((always_inline)) {
-    tau = (r < a.n_q && a.tau) ? a.tau[r] : -INFINITY;
-    key = (r < a.n_q && a.tau_key) ? a.tau_key[r] : 0ull;
+  const bool key_bound = a.tau_key != nullptr;
+  // bounds of row r (queries and candidates are the same rows); past the end
+  // the bound is infinite, so nothing passes on that row's side
+  auto load_bound = [&](int r, float& tau, uint64_t& key) __attribute__((always_inline)) {
+    tau = r < a.n_q ? (a.tau ? a.tau[r] : -INFINITY) : INFINITY;
+    key = (r < a.n_q && key_bound) ? a.tau_key[r] : 0ull;
+  };
+  auto fast_of = [&](float tau, uint64_t key) __attribute__((always_inline)) {
     float tb = tau;
-    if (a.tau_key && key != EVOC_KEY_NONE) tb = fmaxf(tb, key_score(key));
-    fast = fast_bound<KIND>(tb);
+    if (key_bound && key != EVOC_KEY_NONE) tb = fmaxf(tb, key_score(key));
+    return fast_bound<KIND>(tb);
   };
   for (int i = threadIdx.x; i < T; i += En::THREADS) {
+    float tau;
+    uint64_t key;
+    load_bound(q0 + i, tau, key);
     cnt[i] = 0;
-    bounds(q0 + i, tau_l[i], tfast_l[i], tkey_l[i]);
+    tau_l[i] = tau;
+    tfast_l[i] = fast_of(tau, key);
+    tkey_l[i] = key;
   }
+  // candidate-row bounds of a tile (row threadIdx.x of it), into LDS with the
+  // tile's exactness flags
+  auto store_cbounds = [&](int c0, float tau, uint64_t key) __attribute__((always_inline)) {
+    const int i = threadIdx.x;
+    scnt[i] = 0;
+    ctau_l[i] = tau;
+    cfast_l[i] = fast_of(tau, key);
+    ckey_l[i] = key;
+    const bool ex = c0 + i >= a.n_q || bound_is_exact<KIND>(tau, key_bound);
+    const uint64_t all = __ballot(ex);
+    if (lane == 0) cexact[i >> 6] = all == ~0ull ? 1 : 0;
+  };
+  // the first tile's bounds now (the diagonal tile has no candidate side);
+  // each later tile's are loaded before the previous tile's MFMA loop and
+  // stored after its epilogue, so no tile waits on a load of them
+  if (w.y != w.x && threadIdx.x < T) {
+    float tau;
+    uint64_t key;
+    load_bound(w.y * T + (int)threadIdx.x, tau, key);
+    store_cbounds(w.y * T, tau, key);
+  }
+  float pf_tau = INFINITY;
+  uint64_t pf_key = 0;
   for (int t = w.y; t < w.z; t++) {
     const int c0 = t * T;
     const bool diag = t == w.x;  // the query block's own tile: query side only
     __syncthreads();             // the last tile's epilogue and flush are done with these
-    if (!diag)
-      for (int i = threadIdx.x; i < T; i += En::THREADS) {
-        scnt[i] = 0;
-        bounds(c0 + i, ctau_l[i], cfast_l[i], ckey_l[i]);
-      }
+    if (t != w.y && threadIdx.x < T) store_cbounds(c0, pf_tau, pf_key);  // (t > w.y >= w.x: never diagonal)
+    if (t + 1 < w.z && threadIdx.x < T) load_bound(c0 + T + (int)threadIdx.x, pf_tau, pf_key);
     En::score_tile(acc, lds, a, c0, q0);  // (begins with a barrier, which publishes them)
     int lane_e = lane, wm_e = wm, wn_e = wn;
     asm volatile("" : "+v"(lane_e), "+v"(wm_e), "+v"(wn_e));
+    // every candidate row of the tile has an exact cheap test
+    bool c_fast = !diag;
+#pragma unroll
+    for (int g = 0; g < T / 64; g++) c_fast = c_fast && cexact[g] != 0;
     static_for<BM>([&](auto bmc) __attribute__((always_inline)) {
       constexpr int bm = decltype(bmc)::value;
       __builtin_amdgcn_sched_barrier(0);
@@ -502,20 +566,26 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
           mc |= (raw >= cf[j] ? 1u : 0u) << j;
         }
         if (dblk) m = 0xffffu;  // the self pair takes the exact path
+        if (q >= a.n_q) mc = 0;  // a query row past the end is nobody's neighbour
         if (__ballot((m | mc) != 0) == 0) return;
         if (m) {  // query side, as pass 2
           const float tau = tau_l[ql];
-          const uint64_t tkey = tkey_l[ql];
-          uint32_t take = 0;
+          uint32_t take;
+          if (!dblk && bound_is_exact<KIND>(tau, key_bound)) {
+            take = m & valid_rows(cb, lane_e, a.n_c);
+          } else {
+            const uint64_t tkey = tkey_l[ql];
+            take = 0;
 #pragma unroll
-          for (int j = 0; j < 16; j++) {
-            if (m & (1u << j)) {
-              const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
-              const bool self = (c == q);
-              const float sc = self ? 1.0f : M::value(acc[bm][bn], j);
-              bool tk = (c < a.n_c) && (q < a.n_q) && (sc >= tau) && !(self && a.skip_self);
-              if (tk && a.tau_key) tk = make_key(sc, c, self) >= tkey;
-              take |= (tk ? 1u : 0u) << j;
+            for (int j = 0; j < 16; j++) {
+              if (m & (1u << j)) {
+                const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+                const bool self = (c == q);
+                const float sc = self ? 1.0f : M::value(acc[bm][bn], j);
+                bool tk = (c < a.n_c) && (q < a.n_q) && (sc >= tau) && !(self && a.skip_self);
+                if (tk && key_bound) tk = make_key(sc, c, self) >= tkey;
+                take |= (tk ? 1u : 0u) << j;
+              }
             }
           }
           if (take) {
@@ -532,24 +602,35 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
             }
           }
         }
-        if (mc && q < a.n_q) {  // candidate side: the pair seen from row c (never self here)
+        if (mc) {  // candidate side: the pair seen from row c (never self here)
+          uint32_t takec;
+          if (c_fast) {
+            takec = mc;  // rows past the end have infinite bounds
+          } else {
+            takec = 0;
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+              if (mc & (1u << j)) {
+                const int cr = cb - c0 + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+                const float sc = M::value(acc[bm][bn], j);
+                bool tk = (c0 + cr < a.n_c) && (sc >= ctau_l[cr]);
+                if (tk && key_bound) tk = make_key(sc, q, false) >= ckey_l[cr];
+                takec |= (tk ? 1u : 0u) << j;
+              }
+            }
+          }
 #pragma unroll
           for (int j = 0; j < 16; j++) {
-            if (mc & (1u << j)) {
+            if (takec & (1u << j)) {
               const int cr = cb - c0 + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
-              const int c = c0 + cr;
-              const float sc = M::value(acc[bm][bn], j);
-              const uint64_t key = make_key(sc, q, false);
-              bool tk = (c < a.n_c) && (sc >= ctau_l[cr]);
-              if (tk && a.tau_key) tk = key >= ckey_l[cr];
-              if (tk) {
-                const int slot = atomicAdd(&scnt[cr], 1);
-                if (slot < SYM_SLOTS) {
-                  stage[cr * SYM_SLOTS + slot] = key;
-                } else {
-                  const int pos = atomicAdd(&a.ccounts[c], 1);
-                  if (pos < a.ccap) a.cbufs[(int64_t)c * a.ccap + pos] = key;
-                }
+              const uint64_t key = make_key(M::value(acc[bm][bn], j), q, false);
+              const int slot = atomicAdd(&scnt[cr], 1);
+              if (slot < SYM_SLOTS) {
+                stage[cr * SYM_SLOTS + slot] = key;
+              } else {
+                const int c = c0 + cr;
+                const int pos = atomicAdd(&a.ccounts[c], 1);
+                if (pos < a.ccap) a.cbufs[(int64_t)c * a.ccap + pos] = key;
               }
             }
           }
@@ -995,9 +1076,9 @@ size_t tile_lds_bytes(int kind, int pass) {
   shape_dims(kind, tc, tq, g);
   const size_t ring = (size_t)MFMA_NS * (tc + tq) * MFMA_SB;
   // pass 2 adds per-query counters, score bounds, fast bounds and key bounds;
-  // the symmetric pass (3) adds the candidate rows' bounds, staged counts and
-  // SYM_SLOTS staged keys per candidate row
-  if (pass == 3) return ring + (size_t)tq * 40 + (size_t)tc * SYM_SLOTS * 8;
+  // the symmetric pass (3) adds the candidate rows' bounds, staged counts,
+  // SYM_SLOTS staged keys per candidate row and four exactness flags
+  if (pass == 3) return ring + (size_t)tq * 40 + (size_t)tc * SYM_SLOTS * 8 + 16;
   return ring + (pass == 2 ? (size_t)tq * (4 + 4 + 4 + 8) : 0);
 }
 
