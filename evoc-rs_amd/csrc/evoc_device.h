@@ -115,6 +115,26 @@ __device__ __forceinline__ void wave_sort_desc(T (&v)[E]) {
   }
 }
 
+// K-th largest of the 64*E values of a wave (register-major, as above; K >= 1),
+// counting repeats: the largest t with #(values >= t) >= K, built bit by bit
+// from the top.  Each count is one ballot per register (no data movement).
+// Registers e with 64 e >= n_used are taken to hold only values below every
+// candidate t (0), and are skipped.  Returns 0 when fewer than K values are >= 1.
+template <int E, typename T>
+__device__ __forceinline__ T wave_kth_largest(const T (&v)[E], int K, int n_used) {
+  T t = 0;
+#pragma unroll 1
+  for (int b = (int)sizeof(T) * 8 - 1; b >= 0; b--) {
+    const T cand = t | ((T)1 << b);
+    int cnt = 0;
+#pragma unroll
+    for (int e = 0; e < E; e++)
+      if (e * 64 < n_used) cnt += __popcll(__ballot(v[e] >= cand));
+    if (cnt >= K) t = cand;
+  }
+  return t;
+}
+
 // Element i of a register-major wave array, broadcast to every lane.
 template <int E>
 __device__ __forceinline__ uint64_t wave_elem(const uint64_t (&v)[E], int i) {

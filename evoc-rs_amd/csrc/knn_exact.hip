@@ -807,7 +807,7 @@ __global__ __launch_bounds__(256) void jac_pass2_kernel(TileArgs a) {
 
 // ---------------------------------------------------------------------------
 // Threshold from group maxima: tau_q = K-th largest of gmax[q, 0..ng).
-// One wave per query, ng <= 64 * E.
+// One wave per query, ng <= 64 * E, found by bisection (no sort).
 // ---------------------------------------------------------------------------
 template <int E>
 __global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict__ gmax, int n_q,
@@ -822,8 +822,8 @@ __global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict
     const float m = g < ng ? gmax[(int64_t)q * ng + g] : -INFINITY;
     v[e] = ord_f32(m);
   }
-  wave_sort_desc<E>(v);
-  const uint32_t kth = wave_elem<E>(v, K - 1);
+  // the -inf padding past ng decides no count that fixes the K-th when K <= ng
+  const uint32_t kth = wave_kth_largest<E>(v, K, ng);
   if (lane == 0) tau[q] = (K <= ng) ? unord_f32(kth) : -INFINITY;
 }
 
@@ -882,18 +882,144 @@ __device__ __forceinline__ void select_body(const SelectArgs& a, int q, bool ove
   if (lane == 0) a.tau_key_out[q] = kth;
 }
 
-// One wave per query; the sort width follows the query's own key count
-// (wave-uniform branch), capped at 64*EMAX keys.
-template <int EMAX>
+// Keys of query q from all its segments (the split buffers, then the
+// candidate-side buffer), total <= 64 * E, held register-major.  Lane s holds
+// segment s's kept count `kept`, its first key's position `base` and its
+// address `src`.  The K-th largest key is found by bisection, the keys above it
+// are compacted into the wave's LDS slice (with the K-th key repeated up to K:
+// keys are distinct, so it repeats only as the empty key when fewer than K
+// exist) and only those K are sorted.  Same output as sorting all keys and
+// keeping K.
+template <int E, int EK>
+__device__ __forceinline__ void select_bisect(const SelectArgs& a, int q, bool overflow, int total, int nseg,
+                                              int kept, int base, const uint64_t* src, uint64_t* sel) {
+  const int lane = threadIdx.x & 63;
+  uint64_t v[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) v[e] = EVOC_KEY_NONE;
+  for (int sg = 0; sg < nseg; sg++) {
+    const int c = __builtin_amdgcn_readlane(kept, sg);
+    if (c == 0) continue;
+    const int b0 = __builtin_amdgcn_readlane(base, sg);
+    // (both halves zero-extended: readlane returns int)
+    const uint64_t p = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)src, sg) |
+                       ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)src >> 32), sg) << 32);
+    const uint64_t* sp = (const uint64_t*)p;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int i = e * 64 + lane;
+      if (e * 64 < b0 + c && e * 64 + 64 > b0 && i >= b0 && i < b0 + c) v[e] = sp[i - b0];
+    }
+  }
+  const uint64_t kth = wave_kth_largest<E>(v, a.K, total);
+  // keys above the K-th, in lane order, then the K-th up to K
+  int above = 0;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const bool up = v[e] > kth;
+    const uint64_t m = __ballot(up);
+    if (up) sel[above + __popcll(m & ((1ull << lane) - 1ull))] = v[e];
+    above += __popcll(m);
+  }
+  for (int i = above + lane; i < a.K; i += 64) sel[i] = kth;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  uint64_t w[EK];
+#pragma unroll
+  for (int e = 0; e < EK; e++) {
+    const int i = e * 64 + lane;
+    w[e] = i < a.K ? sel[i] : EVOC_KEY_NONE;
+  }
+  wave_sort_desc<EK>(w);
+#pragma unroll
+  for (int e = 0; e < EK; e++) {
+    const int i = e * 64 + lane;
+    if (i < a.K) a.out_keys[(int64_t)q * a.K + i] = w[e];
+  }
+  if (lane == 0) {
+    a.overflow[q] = overflow ? 1 : 0;
+    // the K-th kept key bounds the true K-th key from below (kept keys are a
+    // subset of all candidates); it drives the next round for this query block
+    a.tau_key_out[q] = kth;
+  }
+}
+
+// Segment counts of query q in one vector load: lane s holds segment s (split
+// buffer s, then the candidate-side buffer), with its kept count, position of
+// its first key among all kept keys, and address.  nseg <= 64.
+struct SegLane {
+  int kept, base, total;
+  bool overflow;
+  const uint64_t* src;
+};
+__device__ __forceinline__ SegLane segment_lanes(const SelectArgs& a, int q) {
+  const int lane = threadIdx.x & 63;
+  int c = 0, cap = 0;
+  const uint64_t* src = nullptr;
+  if (lane < a.n_splits) {
+    c = a.counts[(int64_t)lane * a.n_q + q];
+    cap = a.caps;
+    src = a.bufs + ((int64_t)lane * a.n_q + q) * a.caps;
+  } else if (lane == a.n_splits && a.cbufs) {
+    c = a.ccounts[q];
+    cap = a.ccap;
+    src = a.cbufs + (int64_t)q * a.ccap;
+  }
+  SegLane r;
+  r.overflow = __ballot(c > cap) != 0;
+  r.kept = min(c, cap);
+  int incl = r.kept;  // inclusive prefix over the lanes (segment order)
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  r.base = incl - r.kept;
+  r.total = __shfl(incl, 63, 64);
+  r.src = src;
+  return r;
+}
+
+constexpr int SELECT_REG_KEYS = 2048;  // keys a query may have for the register path
+
+__device__ __forceinline__ int select_query(const SelectArgs& a, int qi) {
+  if (a.query_list) return qi < a.n_list ? a.query_list[qi] : -1;
+  return qi < a.n_q ? qi : -1;
+}
+
+// One wave per query, K <= 64 * EK: queries with at most 64 segments and
+// SELECT_REG_KEYS kept keys (the usual case), their keys held in registers
+// (width from the query's own key count, a wave-uniform branch) and selected
+// by bisection.  select_stream_kernel takes the others.
+template <int EK>
 __global__ __launch_bounds__(256) void select_topk_kernel(SelectArgs a) {
-  const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
-  int q;
-  if (a.query_list) {
-    if (qi >= a.n_list) return;
-    q = a.query_list[qi];
-  } else {
-    q = qi;
-    if (q >= a.n_q) return;
+  extern __shared__ uint64_t sel_lds[];
+  const int q = select_query(a, blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (q < 0) return;
+  const int nseg = a.n_splits + (a.cbufs ? 1 : 0);
+  if (nseg > 64) return;
+  const SegLane g = segment_lanes(a, q);
+  const int total = g.total;
+  uint64_t* sel = sel_lds + (threadIdx.x >> 6) * 64 * EK;
+  if (total <= 64) select_bisect<1, EK>(a, q, g.overflow, total, nseg, g.kept, g.base, g.src, sel);
+  else if (total <= 128) select_bisect<2, EK>(a, q, g.overflow, total, nseg, g.kept, g.base, g.src, sel);
+  else if (total <= 256) select_bisect<4, EK>(a, q, g.overflow, total, nseg, g.kept, g.base, g.src, sel);
+  else if (total <= 512) select_bisect<8, EK>(a, q, g.overflow, total, nseg, g.kept, g.base, g.src, sel);
+  else if (total <= 1024) select_bisect<16, EK>(a, q, g.overflow, total, nseg, g.kept, g.base, g.src, sel);
+  else if (total <= SELECT_REG_KEYS) select_bisect<32, EK>(a, q, g.overflow, total, nseg, g.kept, g.base, g.src, sel);
+}
+
+// The queries select_topk_kernel leaves: their keys streamed through a
+// 2048-key wave sort.
+__global__ __launch_bounds__(256) void select_stream_kernel(SelectArgs a) {
+  const int q = select_query(a, blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (q < 0) return;
+  const int nseg = a.n_splits + (a.cbufs ? 1 : 0);
+  if (nseg <= 64) {
+    const SegLane g = segment_lanes(a, q);
+    if (g.total > SELECT_REG_KEYS) select_body<32>(a, q, g.overflow, g.total);
+    return;
   }
   int total = 0;
   bool overflow = false;
@@ -907,15 +1033,7 @@ __global__ __launch_bounds__(256) void select_topk_kernel(SelectArgs a) {
     overflow |= c > a.ccap;
     total += min(c, a.ccap);
   }
-  // sort width: the query's own key count, or the widest sort streamed over
-  // its keys (which must leave room beside the K kept)
-  const int need = max(total, 2 * a.K);
-  if (need <= 64 || EMAX == 1) select_body<1>(a, q, overflow, total);
-  else if (need <= 128 || EMAX == 2) select_body<2>(a, q, overflow, total);
-  else if (need <= 256 || EMAX == 4) select_body<4>(a, q, overflow, total);
-  else if (need <= 512 || EMAX == 8) select_body<8>(a, q, overflow, total);
-  else if (need <= 1024 || EMAX == 16) select_body<16>(a, q, overflow, total);
-  else select_body<EMAX>(a, q, overflow, total);
+  select_body<32>(a, q, overflow, total);
 }
 
 // ---------------------------------------------------------------------------
@@ -1203,7 +1321,19 @@ hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int K, float* t
 hipError_t launch_select(const SelectArgs& a, hipStream_t s) {
   const int nq = a.query_list ? a.n_list : a.n_q;
   const dim3 g((unsigned)((nq + 3) / 4)), b(256);
-  hipLaunchKernelGGL(select_topk_kernel<32>, g, b, 0, s, a);
+  const int ek = pow2_per_lane(a.K);
+  const size_t lds = (size_t)4 * 64 * ek * sizeof(uint64_t);  // one K-key slice per wave
+  switch (ek) {
+    case 1: hipLaunchKernelGGL(select_topk_kernel<1>, g, b, lds, s, a); break;
+    case 2: hipLaunchKernelGGL(select_topk_kernel<2>, g, b, lds, s, a); break;
+    case 4: hipLaunchKernelGGL(select_topk_kernel<4>, g, b, lds, s, a); break;
+    case 8: hipLaunchKernelGGL(select_topk_kernel<8>, g, b, lds, s, a); break;
+    case 16: hipLaunchKernelGGL(select_topk_kernel<16>, g, b, lds, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(select_stream_kernel, g, b, 0, s, a);
   return hipGetLastError();
 }
 

@@ -5,7 +5,8 @@
 // every row by its norm (a zero norm becomes 1).  The row norm is computed here
 // by one thread per row in the same serial order with non-contracted IEEE
 // operations, so the renormalisation decision and the divided values are the
-// reference's bit for bit.  The element-wise pass is coalesced and also writes
+// reference's bit for bit (rows staged through LDS so the loads coalesce).
+// The element-wise pass is coalesced and also writes
 // the zero-padded BF16 copy used by the MFMA scorer.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -14,16 +15,32 @@
 
 namespace evoc {
 
-__global__ void row_norm_serial_kernel(const float* __restrict__ X, int64_t ld, int64_t n, int d,
-                                       float* __restrict__ norms, int32_t* __restrict__ flag) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  const float* x = X + r * ld;
+// One 64-thread block per 64 rows.  Columns are staged 64 at a time through
+// LDS with row-contiguous (coalesced) loads; thread t then adds row t's staged
+// values in column order, so each row is still one serial f32 sum.
+constexpr int NORM_ROWS = 64;
+__global__ __launch_bounds__(NORM_ROWS) void row_norm_serial_kernel(const float* __restrict__ X, int64_t ld,
+                                                                    int64_t n, int d,
+                                                                    float* __restrict__ norms,
+                                                                    int32_t* __restrict__ flag) {
+  __shared__ float tile[NORM_ROWS][NORM_ROWS + 1];
+  const int t = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * NORM_ROWS;
+  const int rows = (int)(n - r0 < NORM_ROWS ? n - r0 : NORM_ROWS);
   float s = 0.0f;
-  for (int t = 0; t < d; t++) s = __fadd_rn(s, __fmul_rn(x[t], x[t]));
+  for (int c0 = 0; c0 < d; c0 += NORM_ROWS) {
+    const int cols = min(NORM_ROWS, d - c0);
+    __syncthreads();  // the last chunk's sums are done with the tile
+    if (t < cols)
+      for (int i = 0; i < rows; i++) tile[i][t] = X[(r0 + i) * ld + c0 + t];
+    __syncthreads();
+    if (t < rows)
+      for (int j = 0; j < cols; j++) s = __fadd_rn(s, __fmul_rn(tile[t][j], tile[t][j]));
+  }
+  if (t >= rows) return;
   const float nr = __fsqrt_rn(s);
   if (fabsf(__fsub_rn(nr, 1.0f)) > 1e-5f) atomicOr(flag, 1);
-  norms[r] = nr == 0.0f ? 1.0f : nr;
+  norms[r0 + t] = nr == 0.0f ? 1.0f : nr;
 }
 
 __global__ void zero_i32_kernel(int32_t* __restrict__ p, int64_t n) {
@@ -62,12 +79,17 @@ __global__ void pad_rows_kernel(const uint8_t* __restrict__ X, int64_t ld, int64
   out[r * out_ld + c] = c < row_bytes ? X[r * ld + c] : (uint8_t)0;
 }
 
+// rows padded to a multiple of 128 bytes (16-byte aligned, zero padded)
 __global__ void row_popcount_kernel(const uint8_t* __restrict__ X, int64_t ld, int64_t n,
                                     int row_bytes, uint32_t* __restrict__ pop) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
+  const uint4* x = (const uint4*)(X + r * ld);
   uint32_t s = 0;
-  for (int t = 0; t < row_bytes; t++) s += __popc((uint32_t)X[r * ld + t]);
+  for (int t = 0; t < row_bytes / 16; t++) {
+    const uint4 w = x[t];
+    s += __popc(w.x) + __popc(w.y) + __popc(w.z) + __popc(w.w);
+  }
   pop[r] = s;
 }
 
@@ -79,8 +101,8 @@ hipError_t launch_prepare_float(const float* X, int64_t ld, int64_t n, int d, fl
                                 int64_t out_ld, uint16_t* out_bf16, int64_t bf_ld, float* norms,
                                 int32_t* flag, hipStream_t s) {
   hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, flag, (int64_t)1);
-  hipLaunchKernelGGL(row_norm_serial_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X,
-                     ld, n, d, norms, flag);
+  hipLaunchKernelGGL(row_norm_serial_kernel, dim3((unsigned)((n + NORM_ROWS - 1) / NORM_ROWS)), dim3(NORM_ROWS),
+                     0, s, X, ld, n, d, norms, flag);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int64_t cols = out_ld > bf_ld ? out_ld : bf_ld;
