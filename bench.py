@@ -166,6 +166,47 @@ def serving_bench(torch, E, n_req, rate, slots, budget_gb, seed=42):
                         "device-resident; exact BF16 + FP32 re-score"}
 
 
+def fuzzy_bench(torch, E, idx, dist, k, ctx, steps):
+    """Fuzzy graph (symmetrized) of the n x k device kNN graph: device time per
+    call with the kNN arrays and the CSR in HBM (and the wall time of one call
+    that copies the CSR to the host), checked against the oracle's CSR of the
+    same arrays (same layout, values within 1e-6), which is timed as the CPU
+    baseline of this step (1 thread)."""
+    import oracle as O
+
+    n = idx.shape[0]
+    g = E.neighbor_graph_matrix(float(k), idx, dist, True, ctx=ctx)
+    # HBM-resident: kNN arrays in, CSR out on the device
+    cap = 2 * n * k
+    dout = (torch.empty(n + 1, dtype=torch.int64, device="cuda"), torch.empty(cap, dtype=torch.int32, device="cuda"),
+            torch.empty(cap, dtype=torch.float32, device="cuda"))
+    sts = []
+    for _ in range(steps + 1):
+        st = E.CallStats()
+        gd = E.neighbor_graph_matrix(float(k), idx, dist, True, ctx=ctx, stats=st, out=dout)
+        sts.append(st)
+    sts = sts[1:]
+    dev_ms = float(np.median([s.device_seconds for s in sts])) * 1e3
+    wall_ms = float(np.median([s.seconds for s in sts])) * 1e3
+    same_dev = bool(torch.equal(gd[0].cpu(), torch.from_numpy(g[0])) and torch.equal(gd[1].cpu(), torch.from_numpy(g[1]))
+                    and torch.equal(gd[2].cpu(), torch.from_numpy(g[2])))
+    host_st = E.CallStats()
+    E.neighbor_graph_matrix(float(k), idx, dist, True, ctx=ctx, stats=host_st)
+    ind_h, dst_h = idx.cpu().numpy(), dist.cpu().numpy()
+    t0 = time.perf_counter()
+    o = O.neighbor_graph_matrix(float(k), ind_h, dst_h, True)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    same = bool(np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1]))
+    err = float(np.abs(g[2] - o[2]).max()) if same and g[2].size else None
+    log(f"[bench] fuzzy graph: {dev_ms:.2f} ms device, {wall_ms:.2f} ms wall, host output "
+        f"{host_st.seconds * 1e3:.2f} ms, oracle {cpu_ms:.0f} ms, layout equal {same}, max |delta| {err}")
+    return {"device_ms": dev_ms, "wall_ms": wall_ms, "host_output_wall_ms": host_st.seconds * 1e3,
+            "nnz": int(g[0][-1]), "n": int(n), "k": int(k), "device_output_equals_host_output": same_dev,
+            "layout_equals_oracle": same, "max_abs_diff_vs_oracle": err, "oracle_ms_1_thread": cpu_ms,
+            "what": "neighbor_graph_matrix (sigma/rho bisection, membership strengths, fuzzy union, CSR) "
+                    "of the BF16 + re-score kNN graph above"}
+
+
 def _unpack_bits(torch, xb):
     """[m, B] uint8 -> [m, 8B] float32 bits, MSB first (numpy.packbits order)."""
     shifts = torch.arange(7, -1, -1, device=xb.device, dtype=torch.int32)
@@ -532,6 +573,11 @@ def main():
                      "traffic_source": MEASURED_TRAFFIC.get((n, d, kern), (None, None))[1],
                      "algorithmic_flop_per_launch": flops, "avg_launch_ms": score_s * 1e3},
     }
+    # the next step of the pipeline on the same device-resident graph: the fuzzy
+    # union (src/graph_construction.rs:175-226), against the oracle at full size
+    step(E.PREC_BF16_RESCORE)
+    torch.cuda.synchronize()
+    result["fuzzy_graph"] = fuzzy_bench(torch, E, idx, dist, k, ctx, args.steps)
     if not args.no_fp32:
         log("[bench] timing fp32")
         step(E.PREC_FP32)

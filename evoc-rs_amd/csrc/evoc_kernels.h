@@ -105,4 +105,25 @@ hipError_t launch_pad_rows(const uint8_t* X, int64_t ld_bytes, int64_t n, int ro
 hipError_t launch_row_popcount(const uint8_t* X, int64_t ld, int64_t n, int row_bytes,
                                uint32_t* pop, hipStream_t s);
 
+// ---- launchers (fuzzy.hip): the fuzzy graph and find_duplicates ----
+size_t fuzzy_partial_doubles();  // scratch doubles of launch_fuzzy_sigma's mean
+hipError_t launch_fuzzy_sigma(const float* dist, int64_t n, int k, float n_neighbors, double* partial,
+                              float* sigma, float* rho, hipStream_t s);
+// 2 n k (key, value) slots; keys row * n + col, empty slots all ones
+hipError_t launch_fuzzy_emit(const int32_t* inds, const float* dist, int64_t n, int k, const float* sigma,
+                             const float* rho, int symmetrize, uint64_t* keys, float* vals, int32_t* bad,
+                             hipStream_t s);
+unsigned fuzzy_key_bits(int64_t n);  // key bits the sorts cover
+hipError_t fuzzy_sort_pairs(void* temp, size_t& temp_bytes, const uint64_t* keys_in, uint64_t* keys_out,
+                            const float* vals_in, float* vals_out, int64_t m, unsigned bits, hipStream_t s);
+hipError_t launch_fuzzy_csr(const uint64_t* sorted_keys, int64_t m, int64_t n, int64_t* indptr, hipStream_t s);
+hipError_t launch_fuzzy_cols(const uint64_t* sorted_keys, int64_t nnz, int64_t n, int32_t* cols, hipStream_t s);
+// n k keys min * n + max of duplicate pairs, empty slots all ones; dmax: one zeroed uint32
+hipError_t launch_dup_emit(const int32_t* inds, const float* dist, int64_t n, int k, uint32_t* dmax,
+                           uint64_t* keys, int32_t* bad, hipStream_t s);
+// m distinct keys a * n + b -> pairs (a << 32 | b)
+hipError_t launch_dup_pairs(const uint64_t* keys, int64_t m, int64_t n, uint64_t* pairs, hipStream_t s);
+hipError_t dup_sort_unique(void* temp, size_t& temp_bytes, const uint64_t* keys_in, uint64_t* keys_sorted,
+                           uint64_t* keys_unique, int64_t* n_unique, int64_t m, unsigned bits, hipStream_t s);
+
 }  // namespace evoc

@@ -90,6 +90,19 @@ class _Output(C.Structure):
                 ("score_pairs", C.c_int64)]
 
 
+class _FuzzyInput(C.Structure):
+    _fields_ = [("location", C.c_int32), ("symmetrize", C.c_int32), ("knn_indices", C.c_void_p),
+                ("knn_dists", C.c_void_p), ("n", C.c_int64), ("k", C.c_int64), ("n_neighbors", C.c_float),
+                ("reserved0", C.c_int32)]
+
+
+class _FuzzyOutput(C.Structure):
+    _fields_ = [("location", C.c_int32), ("reserved0", C.c_int32), ("indptr", C.c_void_p),
+                ("indices", C.c_void_p), ("data", C.c_void_p), ("capacity", C.c_int64),
+                ("sigmas", C.c_void_p), ("rhos", C.c_void_p), ("nnz", C.c_int64),
+                ("seconds", C.c_double), ("device_seconds", C.c_double)]
+
+
 _lib = None
 
 # Every function include/evoc_mi355x.h declares, with its ctypes signature.
@@ -122,6 +135,9 @@ SIGNATURES = {
                                           C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
     "evoc_mi355x_server_stats": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)], None),
+    "evoc_mi355x_fuzzy_graph": ([_P, C.POINTER(_FuzzyInput), C.POINTER(_FuzzyOutput)], C.c_int),
+    "evoc_mi355x_find_duplicates": ([_P, C.c_int32, _P, _P, C.c_int64, C.c_int64, C.c_int32, _P, C.c_int64,
+                                     C.POINTER(C.c_int64)], C.c_int),
     "evoc_numpy_rng_new": ([C.c_uint32], _P),
     "evoc_numpy_rng_free": ([_P], None),
     "evoc_numpy_rng_randint": ([_P, C.c_int64, C.c_int64], C.c_int64),
@@ -615,3 +631,99 @@ class Server:
     def __del__(self):
         if _lib is not None:
             self.close()
+
+
+# ---------------------------------------------------------------------------
+# The steps after knn_graph that consume its output
+# ---------------------------------------------------------------------------
+def _knn_arrays(knn_indices, knn_dists):
+    """(indices, distances) as numpy arrays or GPU tensors -> (location, ptrs, n, k, keepalive)."""
+    is_torch = type(knn_indices).__module__.startswith("torch")
+    if is_torch:
+        if str(knn_indices.dtype) != "torch.int32" or str(knn_dists.dtype) != "torch.float32":
+            raise BackendError(ERR_INVALID_ARGUMENT, "expected int32 indices and float32 distances")
+        ind, dst = knn_indices.contiguous(), knn_dists.contiguous()
+        loc = MEM_DEVICE if ind.is_cuda else MEM_HOST
+        return loc, ind.data_ptr(), dst.data_ptr(), ind.shape[0], ind.shape[1], (ind, dst)
+    ind = np.ascontiguousarray(knn_indices, dtype=np.int32)
+    dst = np.ascontiguousarray(knn_dists, dtype=np.float32)
+    if ind.ndim != 2 or ind.shape != dst.shape:
+        raise BackendError(ERR_INVALID_ARGUMENT, "knn_indices and knn_dists must be n x k")
+    return MEM_HOST, ind.ctypes.data, dst.ctypes.data, ind.shape[0], ind.shape[1], (ind, dst)
+
+
+def neighbor_graph_matrix(n_neighbors: float, knn_indices, knn_dists, symmetrize: bool = True,
+                          ctx: Optional[Context] = None, stats: Optional[CallStats] = None,
+                          return_sigmas: bool = False, out=None):
+    """src/graph_construction.rs:162-173 (neighbor_graph_matrix) on the MI355X: the
+    fuzzy graph of a kNN graph as CSR (indptr int64 [n + 1], indices int32,
+    data float32), each row's columns ascending.  With return_sigmas also the
+    per-row (sigma, rho) of smooth_knn_dist.  Inputs: numpy arrays or torch
+    tensors (host or GPU).  Outputs are numpy arrays, or with `out` = (indptr,
+    indices, data) GPU tensors (indices / data of capacity 2 n k) those are
+    filled in place and (indptr, indices[:nnz], data[:nnz]) views returned."""
+    ctx = ctx or default_context()
+    loc, pi, pd, n, k, keep = _knn_arrays(knn_indices, knn_dists)
+    if n <= 0:
+        _raise(ERR_EMPTY_DATA, None)
+    sig = np.empty(n, np.float32)
+    rho = np.empty(n, np.float32)
+    if out is not None:
+        indptr, indices, data = out
+        cap = min(indices.numel(), data.numel())
+        ptrs = (indptr.data_ptr(), indices.data_ptr(), data.data_ptr())
+        oloc = MEM_DEVICE if indptr.is_cuda else MEM_HOST
+        sig_p = rho_p = None  # (host sigma / rho would need a host copy per call)
+    else:
+        cap = max(1, 2 * n * k)
+        indptr = np.empty(n + 1, np.int64)
+        indices = np.empty(cap, np.int32)
+        data = np.empty(cap, np.float32)
+        ptrs = (indptr.ctypes.data, indices.ctypes.data, data.ctypes.data)
+        oloc = MEM_HOST
+        sig_p, rho_p = sig.ctypes.data, rho.ctypes.data
+    fin = _FuzzyInput(loc, 1 if symmetrize else 0, C.c_void_p(pi), C.c_void_p(pd), n, k, float(n_neighbors), 0)
+    fo = _FuzzyOutput(oloc, 0, C.c_void_p(ptrs[0]), C.c_void_p(ptrs[1]), C.c_void_p(ptrs[2]), cap,
+                      C.c_void_p(sig_p), C.c_void_p(rho_p), 0, 0.0, 0.0)
+    _sync_torch(*keep, *(out or ()))
+    rc = lib().evoc_mi355x_fuzzy_graph(ctx.handle, C.byref(fin), C.byref(fo))
+    del keep
+    if rc != OK:
+        _raise(rc, ctx.handle)
+    if stats is not None:
+        stats.seconds = fo.seconds
+        stats.device_seconds = fo.device_seconds
+    nnz = fo.nnz
+    if out is not None:
+        res = (indptr, indices[:nnz], data[:nnz])
+    else:
+        res = (indptr, indices[:nnz].copy(), data[:nnz].copy())
+    return res + ((sig, rho),) if return_sigmas else res
+
+
+def smooth_knn_dist(distances, k: float, ctx: Optional[Context] = None):
+    """src/graph_construction.rs:15-81 on the MI355X: per-row (sigma, rho)."""
+    d = np.ascontiguousarray(distances, dtype=np.float32)
+    ind = np.full(d.shape, -1, np.int32)  # no edges: only the per-row bisection runs
+    _, _, _, (sig, rho) = neighbor_graph_matrix(k, ind, d, True, ctx=ctx, return_sigmas=True)
+    return sig, rho
+
+
+def find_duplicates(knn_inds, knn_dists, ctx: Optional[Context] = None):
+    """src/cluster_util.rs:430-449 on the MI355X: the set of pairs (i, j), i < j,
+    listed at a distance <= the largest first-column distance."""
+    ctx = ctx or default_context()
+    loc, pi, pd, n, k, keep = _knn_arrays(knn_inds, knn_dists)
+    if n <= 0:
+        _raise(ERR_EMPTY_DATA, None)
+    cap = max(1, n * k)
+    pairs = np.empty(cap, np.uint64)
+    cnt = C.c_int64(0)
+    _sync_torch(*keep)
+    rc = lib().evoc_mi355x_find_duplicates(ctx.handle, loc, C.c_void_p(pi), C.c_void_p(pd), n, k, MEM_HOST,
+                                           C.c_void_p(pairs.ctypes.data), cap, C.byref(cnt))
+    del keep
+    if rc != OK:
+        _raise(rc, ctx.handle)
+    p = pairs[: cnt.value]
+    return {(int(a), int(b)) for a, b in zip(p >> np.uint64(32), p & np.uint64(0xFFFFFFFF))}

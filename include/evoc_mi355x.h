@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define EVOC_MI355X_ABI_VERSION 3
+#define EVOC_MI355X_ABI_VERSION 4
 
 /* EmbeddingData::{Float32, Int8, UInt8}, src/knn.rs:16-20.  UInt8 rows are
  * packed bits compared with bitwise Jaccard; d counts bytes. */
@@ -199,6 +199,52 @@ int evoc_mi355x_server_request_bytes(const evoc_mi355x_server* srv, const evoc_k
 /* requests queued / running and bytes granted to running requests */
 void evoc_mi355x_server_stats(const evoc_mi355x_server* srv, int64_t* queued, int64_t* running,
                               uint64_t* granted_bytes, uint64_t* budget_bytes);
+
+/* ---- the steps after knn_graph that consume its output ----
+ * Fuzzy simplicial set: neighbor_graph_matrix_with_coo(n_neighbors,
+ * knn_indices, knn_dists, symmetrize) (src/graph_construction.rs:175-226,
+ * without its golden-value override): sigma / rho by smooth_knn_dist (:15-81),
+ * membership strengths (:83-115) and, with symmetrize, the fuzzy union
+ * A + A^T - A o A^T (src/csr_matmul.rs:62-147), returned as CSR with each row's
+ * columns ascending and exact zeros dropped (:228-240).  Evoc::fit_predict calls
+ * it on knn_graph's output with n_neighbors = neighbor_scale * n_neighbors
+ * (src/clustering.rs:427-433).  -1 entries are skipped; a row must not list a
+ * column twice; other indices outside [0, n) are EVOC_ERR_INVALID_ARGUMENT. */
+typedef struct evoc_fuzzy_input {
+  int32_t location;            /* enum evoc_location of knn_indices / knn_dists */
+  int32_t symmetrize;          /* nonzero: the fuzzy union (the reference's default) */
+  const int32_t* knn_indices;  /* n x k, row-major */
+  const float* knn_dists;      /* n x k, as knn_graph returns them */
+  int64_t n;
+  int64_t k;
+  float n_neighbors;           /* k of log2(k) in smooth_knn_dist */
+  int32_t reserved0;
+} evoc_fuzzy_input;
+
+typedef struct evoc_fuzzy_output {
+  int32_t location;   /* enum evoc_location of every array below */
+  int32_t reserved0;
+  int64_t* indptr;    /* n + 1 */
+  int32_t* indices;   /* capacity */
+  float* data;        /* capacity */
+  int64_t capacity;   /* entries allocated in indices / data; 2 n k always suffices */
+  float* sigmas;      /* n, may be NULL */
+  float* rhos;        /* n, may be NULL */
+  int64_t nnz;        /* set: entries of the result (also when capacity is too small) */
+  double seconds;     /* wall time of the call */
+  double device_seconds;
+} evoc_fuzzy_output;
+
+int evoc_mi355x_fuzzy_graph(evoc_mi355x_ctx* ctx, const evoc_fuzzy_input* in, evoc_fuzzy_output* out);
+
+/* find_duplicates(knn_inds, knn_dists) (src/cluster_util.rs:430-449): the pairs
+ * (i, j), i < j, that the kNN graph lists at a distance <= the largest distance
+ * of its first column, as ascending keys (i << 32 | j) (the reference returns
+ * them as a HashSet).  -1 entries are skipped.  *n_pairs is set also when
+ * `capacity` is too small (EVOC_ERR_INVALID_ARGUMENT); n k always suffices. */
+int evoc_mi355x_find_duplicates(evoc_mi355x_ctx* ctx, int32_t location, const int32_t* knn_indices,
+                                const float* knn_dists, int64_t n, int64_t k, int32_t out_location,
+                                uint64_t* pairs, int64_t capacity, int64_t* n_pairs);
 
 /* ---- NumPy RandomState (src/numpy_rng.rs:12-226), for C / Python callers ---- */
 evoc_numpy_rng* evoc_numpy_rng_new(uint32_t seed);        /* check_random_state(Some(seed)) */

@@ -298,3 +298,64 @@ def knn_graph(data: np.ndarray, options: KnnGraphOptions, rng: NumpyRandomState,
     if rc:
         raise RuntimeError(f"oracle knn_graph failed ({rc})")
     return idx, dist, int(it.value)
+
+
+def _declare_fuzzy(L):
+    L.orc_smooth_knn_dist.argtypes = [_P, _I64, _I64, C.c_float, _P, _P]
+    L.orc_fuzzy_graph.argtypes = [_P, _P, _I64, _I64, C.c_float, C.c_int, _P, _P, _P, _P, _P, _I64]
+    L.orc_fuzzy_graph.restype = _I64
+
+
+def smooth_knn_dist(distances: np.ndarray, k: float):
+    """sigma, rho per row (src/graph_construction.rs:15-81)."""
+    d = np.ascontiguousarray(distances, dtype=np.float32)
+    n, kk = d.shape
+    L = lib()
+    _declare_fuzzy(L)
+    sigma = np.empty(n, np.float32)
+    rho = np.empty(n, np.float32)
+    L.orc_smooth_knn_dist(_ptr(d), n, kk, float(k), _ptr(sigma), _ptr(rho))
+    return sigma, rho
+
+
+def neighbor_graph_matrix(n_neighbors: float, knn_indices: np.ndarray, knn_dists: np.ndarray,
+                          symmetrize: bool = True):
+    """Fuzzy graph as CSR (indptr int64, indices int32, data float32), rows sorted
+    by column, plus (sigma, rho): src/graph_construction.rs:162-226."""
+    ind = np.ascontiguousarray(knn_indices, dtype=np.int32)
+    dst = np.ascontiguousarray(knn_dists, dtype=np.float32)
+    n, k = ind.shape
+    L = lib()
+    _declare_fuzzy(L)
+    cap = max(1, 2 * n * k)
+    sigma = np.empty(n, np.float32)
+    rho = np.empty(n, np.float32)
+    indptr = np.empty(n + 1, np.int64)
+    indices = np.empty(cap, np.int32)
+    data = np.empty(cap, np.float32)
+    nnz = L.orc_fuzzy_graph(_ptr(ind), _ptr(dst), n, k, float(n_neighbors), int(bool(symmetrize)), _ptr(sigma),
+                            _ptr(rho), _ptr(indptr), _ptr(indices), _ptr(data), cap)
+    if nnz < 0:
+        raise MemoryError("orc_fuzzy_graph failed")
+    return indptr, indices[:nnz].copy(), data[:nnz].copy(), sigma, rho
+
+
+def find_duplicates(knn_inds: np.ndarray, knn_dists: np.ndarray) -> set:
+    """src/cluster_util.rs:430-449: threshold = max of the first distance column
+    (NaN ignored, as f32::max folds from NaN); every listed pair (i, j), i != j,
+    at a distance <= threshold, as (min, max).  -1 entries are skipped (the
+    reference would cast them to usize::MAX)."""
+    ind = np.asarray(knn_inds)
+    dst = np.asarray(knn_dists, dtype=np.float32)
+    col = dst[:, 0]
+    col = col[~np.isnan(col)]
+    if col.size == 0:
+        return set()
+    thr = col.max()
+    out = set()
+    for i in range(ind.shape[0]):
+        for j in range(ind.shape[1]):
+            kk = int(ind[i, j])
+            if kk >= 0 and dst[i, j] <= thr and kk != i:
+                out.add((min(i, kk), max(i, kk)))
+    return out

@@ -1519,3 +1519,252 @@ int orc_knn_graph(int dtype, void *x, int64_t n, int64_t d, const orc_knn_option
     orc_transform(dtype, out_dist, n * k);
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Fuzzy simplicial set (the step after knn_graph): src/graph_construction.rs */
+/* ------------------------------------------------------------------------- */
+#define ORC_SMOOTH_K_TOLERANCE 1e-5f /* src/graph_construction.rs:12 */
+#define ORC_MIN_K_DIST_SCALE 1e-3f   /* src/graph_construction.rs:13 */
+
+/* smooth_knn_dist, src/graph_construction.rs:15-81.  Per row, in f64 and in row
+ * order: rho = the first positive distance; sigma from at most 64 bisection
+ * steps on sum_{j >= 1} exp(-(d_j - rho) / sigma) = log2(n_neighbors) (stop
+ * when within 1e-5); floored at 1e-3 x the row mean (rho > 0) or 1e-3 x the
+ * mean of all distances (rho = 0).  The reference takes that global mean with
+ * ndarray's f32 mean; here it is an f64 sum rounded to f32 (it only enters the
+ * floor of rows without a positive distance). */
+void orc_smooth_knn_dist(const float *dist, int64_t n, int64_t k, float n_neighbors, float *sigma,
+                         float *rho)
+{
+    const double target = log2((double)n_neighbors);
+    double tot = 0.0;
+    for (int64_t i = 0; i < n * k; i++) tot += (double)dist[i];
+    const double mean_all = n * k > 0 ? (double)(float)(tot / (double)(n * k)) : 0.0;
+    const double tol = (double)ORC_SMOOTH_K_TOLERANCE;
+    for (int64_t i = 0; i < n; i++) {
+        const float *row = dist + i * k;
+        double rho_i = 0.0;
+        for (int64_t j = 0; j < k; j++)
+            if ((double)row[j] > 0.0) {
+                rho_i = (double)row[j];
+                break;
+            }
+        double lo = 0.0, hi = INFINITY, mid = 1.0;
+        for (int it = 0; it < 64; it++) {
+            double psum = 0.0;
+            for (int64_t j = 1; j < k; j++) {
+                const double d = (double)row[j] - rho_i;
+                psum += d > 0.0 ? exp(-(d / mid)) : 1.0;
+            }
+            if (fabs(psum - target) < tol) break;
+            if (psum > target) {
+                hi = mid;
+                mid = (lo + hi) / 2.0;
+            } else {
+                lo = mid;
+                mid = isinf(hi) ? mid * 2.0 : (lo + hi) / 2.0;
+            }
+        }
+        double s = mid;
+        if (rho_i > 0.0) {
+            double m = 0.0;
+            for (int64_t j = 0; j < k; j++) m += (double)row[j];
+            m /= (double)k;
+            const double fl = (double)ORC_MIN_K_DIST_SCALE * m;
+            if (s < fl) s = fl;
+        } else {
+            const double fl = (double)ORC_MIN_K_DIST_SCALE * mean_all;
+            if (s < fl) s = fl;
+        }
+        sigma[i] = (float)s;
+        rho[i] = (float)rho_i;
+    }
+}
+
+/* compute_membership_strengths, src/graph_construction.rs:83-115 (f32):
+ * self -> 0, d - rho <= 0 or sigma == 0 -> 1, else exp(-(d - rho) / sigma). */
+static float membership(int64_t i, int32_t idx, float d, float sg, float rh)
+{
+    if (idx == (int32_t)i) return 0.0f;
+    if (d - rh <= 0.0f || sg == 0.0f) return 1.0f;
+    return expf(-((d - rh) / sg));
+}
+
+typedef struct {
+    int64_t n;
+    int64_t *ptr;
+    int32_t *col;
+    float *val;
+} orc_csr;
+
+static int cmp_col_val(const void *a, const void *b)
+{
+    const int32_t ca = *(const int32_t *)a, cb = *(const int32_t *)b;
+    return (ca > cb) - (ca < cb);
+}
+
+/* COO triplets (row-major by construction) -> CSR with each row sorted by column;
+ * duplicates (none in a kNN graph) would be summed, as TriMat::to_csr does. */
+static int coo_to_csr(int64_t n, int64_t nnz, const int32_t *r, const int32_t *c, const float *v,
+                      orc_csr *out)
+{
+    out->n = n;
+    out->ptr = calloc((size_t)n + 1, sizeof(int64_t));
+    out->col = malloc(((size_t)nnz + 1) * sizeof(int32_t));
+    out->val = malloc(((size_t)nnz + 1) * sizeof(float));
+    typedef struct { int32_t c; float v; } cv;
+    cv *tmp = malloc(((size_t)nnz + 1) * sizeof(cv));
+    int64_t *fill = calloc((size_t)n + 1, sizeof(int64_t));
+    if (!out->ptr || !out->col || !out->val || !tmp || !fill) return -1;
+    for (int64_t e = 0; e < nnz; e++) out->ptr[r[e] + 1]++;
+    for (int64_t i = 0; i < n; i++) out->ptr[i + 1] += out->ptr[i];
+    for (int64_t e = 0; e < nnz; e++) {
+        const int64_t p = out->ptr[r[e]] + fill[r[e]]++;
+        tmp[p].c = c[e];
+        tmp[p].v = v[e];
+    }
+    int64_t w = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t a = out->ptr[i], b = out->ptr[i + 1];
+        qsort(tmp + a, (size_t)(b - a), sizeof(cv), cmp_col_val);
+        const int64_t start = w;
+        for (int64_t p = a; p < b; p++) {
+            if (w > start && out->col[w - 1] == tmp[p].c) {
+                out->val[w - 1] += tmp[p].v;
+            } else {
+                out->col[w] = tmp[p].c;
+                out->val[w] = tmp[p].v;
+                w++;
+            }
+        }
+        out->ptr[i] = start;
+    }
+    out->ptr[n] = w;
+    free(tmp);
+    free(fill);
+    return 0;
+}
+
+static void csr_free(orc_csr *m)
+{
+    free(m->ptr);
+    free(m->col);
+    free(m->val);
+}
+
+/* Row-wise sparse binary op over sorted rows, dropping exact zeros
+ * (scipy_csr_rowwise_binop, src/csr_matmul.rs:81-147): op 0 = a + b (union),
+ * 1 = a * b (intersection), 2 = a - b (union); a missing operand is 0.0. */
+static int csr_binop(const orc_csr *A, const orc_csr *B, int op, orc_csr *C)
+{
+    const int64_t n = A->n;
+    const int64_t cap = A->ptr[n] + B->ptr[n] + 1;
+    C->n = n;
+    C->ptr = calloc((size_t)n + 1, sizeof(int64_t));
+    C->col = malloc((size_t)cap * sizeof(int32_t));
+    C->val = malloc((size_t)cap * sizeof(float));
+    if (!C->ptr || !C->col || !C->val) return -1;
+    int64_t w = 0;
+    for (int64_t i = 0; i < n; i++) {
+        int64_t ja = A->ptr[i], jb = B->ptr[i];
+        const int64_t ea = A->ptr[i + 1], eb = B->ptr[i + 1];
+        while (ja < ea || jb < eb) {
+            const int64_t ca = ja < ea ? A->col[ja] : INT64_MAX;
+            const int64_t cb = jb < eb ? B->col[jb] : INT64_MAX;
+            const int64_t c = ca < cb ? ca : cb;
+            const int has_a = ca == c, has_b = cb == c;
+            const float va = has_a ? A->val[ja] : 0.0f, vb = has_b ? B->val[jb] : 0.0f;
+            int emit = op == 1 ? (has_a && has_b) : 1;
+            float v = op == 0 ? va + vb : (op == 1 ? va * vb : va - vb);
+            if (emit && v != 0.0f) {
+                C->col[w] = (int32_t)c;
+                C->val[w] = v;
+                w++;
+            }
+            if (has_a) ja++;
+            if (has_b) jb++;
+        }
+        C->ptr[i + 1] = w;
+    }
+    return 0;
+}
+
+static int csr_transpose(const orc_csr *A, orc_csr *T)
+{
+    const int64_t n = A->n, nnz = A->ptr[n];
+    int32_t *r = malloc(((size_t)nnz + 1) * sizeof(int32_t));
+    int32_t *c = malloc(((size_t)nnz + 1) * sizeof(int32_t));
+    if (!r || !c) return -1;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t p = A->ptr[i]; p < A->ptr[i + 1]; p++) {
+            r[p] = A->col[p];
+            c[p] = (int32_t)i;
+        }
+    const int rc = coo_to_csr(n, nnz, r, c, A->val, T);
+    free(r);
+    free(c);
+    return rc;
+}
+
+/* neighbor_graph_matrix_with_coo, src/graph_construction.rs:175-226 (without
+ * the reference-value override, which needs on-disk goldens): membership
+ * strengths of the kNN graph (nonzero only), then with `symmetrize` the fuzzy
+ * union A + A^T - A o A^T, rows sorted by column, zeros dropped
+ * (canonicalize_csr_rows, :228-240).  Writes sigma/rho (n), indptr (n + 1),
+ * indices / data (capacity cap) and returns nnz, or -1 (allocation, capacity). */
+int64_t orc_fuzzy_graph(const int32_t *inds, const float *dist, int64_t n, int64_t k, float n_neighbors,
+                        int symmetrize, float *sigma, float *rho, int64_t *indptr, int32_t *indices,
+                        float *data, int64_t cap)
+{
+    orc_smooth_knn_dist(dist, n, k, n_neighbors, sigma, rho);
+    int32_t *r = malloc(((size_t)(n * k) + 1) * sizeof(int32_t));
+    int32_t *c = malloc(((size_t)(n * k) + 1) * sizeof(int32_t));
+    float *v = malloc(((size_t)(n * k) + 1) * sizeof(float));
+    if (!r || !c || !v) return -1;
+    int64_t m = 0;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < k; j++) {
+            const int32_t idx = inds[i * k + j];
+            if (idx == -1) continue;
+            const float val = membership(i, idx, dist[i * k + j], sigma[i], rho[i]);
+            if (val != 0.0f) {
+                r[m] = (int32_t)i;
+                c[m] = idx;
+                v[m] = val;
+                m++;
+            }
+        }
+    orc_csr A, res;
+    if (coo_to_csr(n, m, r, c, v, &A)) return -1;
+    free(r);
+    free(c);
+    free(v);
+    if (symmetrize) {
+        orc_csr T, sum, prod;
+        if (csr_transpose(&A, &T) || csr_binop(&A, &T, 1, &prod) || csr_binop(&A, &T, 0, &sum) ||
+            csr_binop(&sum, &prod, 2, &res))
+            return -1;
+        csr_free(&T);
+        csr_free(&sum);
+        csr_free(&prod);
+        csr_free(&A);
+    } else {
+        res = A;
+    }
+    const int64_t nnz = res.ptr[n];
+    int64_t w = 0;
+    if (nnz <= cap) {
+        for (int64_t i = 0; i < n; i++) {
+            indptr[i] = w;
+            for (int64_t p = res.ptr[i]; p < res.ptr[i + 1]; p++)
+                if (res.val[p] != 0.0f) {
+                    indices[w] = res.col[p];
+                    data[w] = res.val[p];
+                    w++;
+                }
+        }
+        indptr[n] = w;
+    }
+    csr_free(&res);
+    return nnz <= cap ? w : -1;
+}
