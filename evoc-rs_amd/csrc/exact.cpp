@@ -65,6 +65,19 @@ struct Plan {
   }
 };
 
+// Candidate rows the threshold pass samples (every stride-th row): a larger
+// sample costs pass 1 more and tightens the bound, so fewer keys are appended
+// and selected.  EVOC_MI355X_SAMPLE_ROWS overrides the default (a tuning knob;
+// results do not depend on it).
+int64_t sample_rows() {
+  static const int64_t v = [] {
+    const char* e = std::getenv("EVOC_MI355X_SAMPLE_ROWS");
+    const long long r = e ? std::atoll(e) : 0;
+    return r > 0 ? (int64_t)r : (int64_t)16384;
+  }();
+  return v;
+}
+
 Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   Plan p;
   const int TC = tile_rows_c(kind), TQ = tile_rows_q(kind);
@@ -85,7 +98,7 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   }
   p.pass1 = true;
   const int gpt = groups_per_tile(kind);
-  int64_t stride = std::min<int64_t>(std::max<int64_t>(cdiv(n, 16384), 1), 16);
+  int64_t stride = std::min<int64_t>(std::max<int64_t>(cdiv(n, sample_rows()), 1), 16);
   for (;;) {
     const int64_t nsamp = cdiv(n, stride);
     const int64_t ng = cdiv(nsamp, TC) * gpt;

@@ -126,24 +126,37 @@ struct Engine {
   __device__ static int wave_m() { return (threadIdx.x >> 6) / WN; }
   __device__ static int wave_n() { return (threadIdx.x >> 6) % WN; }
 
-  __device__ __forceinline__ static void issue_stage(uint8_t* lds, const uint8_t* c_rows, const uint8_t* q_rows,
-                                     const TileArgs& a, int c0, int q0, int st) {
-    uint8_t* buf = lds + (st % MFMA_NS) * STAGE_LDS;
+  // stage `st` of the tile (c0, q0) into ring buffer `buf`
+  __device__ __forceinline__ static void issue_stage(uint8_t* buf, const uint8_t* c_rows, const uint8_t* q_rows,
+                                                     const TileArgs& a, int c0, int q0, int st) {
     dma_stage<TC, WAVES>(buf, c_rows, a.row_bytes, c0, a.n_c, a.c_stride, st * MFMA_SB);
     dma_stage<TQ, WAVES>(buf + TC * MFMA_SB, q_rows, a.row_bytes, q0, a.n_q, 1, st * MFMA_SB);
   }
 
+  // Whether a workgroup's next tile can be primed by this one: its first
+  // MFMA_NS stages are loaded during this tile's last MFMA_NS stages, which
+  // needs at least MFMA_NS stages per tile.
+  __device__ static bool can_prime(const TileArgs& a) { return a.row_bytes / MFMA_SB >= MFMA_NS; }
+
   // Accumulate the TC x TQ score tile (candidates c0.., queries q0..) over all
-  // stages.  Stage st lives in ring buffer st % MFMA_NS.  The stage boundary
-  // is crossed inside the previous stage: once a wave has issued (and waited
+  // stages.  Stage st of the tile lives in ring buffer (ph + st) % MFMA_NS;
+  // `ph` carries the ring phase from tile to tile.  The stage boundary is
+  // crossed inside the previous stage: once a wave has issued (and waited
   // for) its LDS reads of the last K step of stage s, it waits for its own
   // loads of stage s + 1, and one barrier then publishes stage s + 1 and
-  // retires buffer s % MFMA_NS, which the loads of stage s + MFMA_NS refill
-  // (loads run MFMA_NS stages ahead).  The first reads of stage s + 1 then
-  // overlap the last MFMAs of stage s, so no stage starts on an LDS bubble.
-  // Within a K step, the reads of the next step alternate with the MFMAs.
+  // retires buffer (ph + s) % MFMA_NS, which the loads of stage s + MFMA_NS
+  // refill (loads run MFMA_NS stages ahead).  The first reads of stage s + 1
+  // then overlap the last MFMAs of stage s, so no stage starts on an LDS
+  // bubble.  Within a K step, the reads of the next step alternate with the
+  // MFMAs.
+  // With `has_next` the loads of the last MFMA_NS crossings are the first
+  // MFMA_NS stages of the next tile (nc0, nq0): they land during this tile's
+  // epilogue, and the next call, `primed`, starts on them.  Memory operations
+  // issued between two tiles (the epilogue's) only make the vmcnt waits
+  // stricter: everything older than a stage's loads has landed with them.
   __device__ __forceinline__ static void score_tile(acc_t (&acc)[BM][BN], uint8_t* lds, const TileArgs& a, int c0,
-                                                    int q0) {
+                                                    int q0, int& ph, bool primed, bool has_next, int nc0,
+                                                    int nq0) {
     const int lane = threadIdx.x & 63;
     const int wm = wave_m(), wn = wave_n();
 #pragma unroll
@@ -161,6 +174,7 @@ struct Engine {
     constexpr int KS = MFMA_SB / M::KSTEP_BYTES;
     static_assert(KS % 2 == 0, "fragment double buffer parity must carry across stages");
     typename M::frag_t fa[2][BM], fb[2][BN];
+    auto ring = [&](int st) __attribute__((always_inline)) { return lds + ((ph + st) % MFMA_NS) * STAGE_LDS; };
     auto read_frags = [&](const uint8_t* buf, int ks, int par) __attribute__((always_inline)) {
       const int kb = ks * M::KSTEP_BYTES + (lane >> 5) * M::LANE_BYTES;
       const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
@@ -176,16 +190,23 @@ struct Engine {
       else if (newer == 1) wait_vmcnt<DMA_PER_STAGE>();
       else wait_vmcnt<0>();
     };
-    // the ring may still be read by waves finishing the previous tile
-    __syncthreads();
-    // (kept rolled: unrolled, the compiler precomputes each stage's 64-bit
-    // addresses, runs out of registers and spills them)
+    // stage st of this tile, or stage st - nst of the next one
+    auto issue = [&](int st) __attribute__((always_inline)) {
+      if (st < nst) issue_stage(ring(st), c_rows, q_rows, a, c0, q0, st);
+      else issue_stage(ring(st), c_rows, q_rows, a, nc0, nq0, st - nst);
+    };
+    if (!primed) {
+      // the ring may still be read by waves finishing the previous tile
+      __syncthreads();
+      // (kept rolled: unrolled, the compiler precomputes each stage's 64-bit
+      // addresses, runs out of registers and spills them)
 #pragma unroll 1
-    for (int st = 0; st < MFMA_NS; st++)
-      if (st < nst) issue_stage(lds, c_rows, q_rows, a, c0, q0, st);
+      for (int st = 0; st < MFMA_NS; st++)
+        if (st < nst) issue(st);
+    }
     wait_stage(min(MFMA_NS - 1, nst - 1));
     ring_barrier();
-    read_frags(lds, 0, 0);
+    read_frags(ring(0), 0, 0);
     // MFMAs of one K step from fragment buffer `par`, each followed by one of
     // the fragment reads issued just before (an MFMA first: its operand wait
     // then covers the previous step's reads only), then the remaining MFMAs
@@ -203,7 +224,7 @@ struct Engine {
       __builtin_amdgcn_sched_barrier(0);
     };
     for (int s = 0; s < nst; s++) {
-      const uint8_t* cur = lds + (s % MFMA_NS) * STAGE_LDS;
+      const uint8_t* cur = ring(s);
 #pragma unroll
       for (int ks = 0; ks + 1 < KS; ks++) {
         const int p = ks & 1;
@@ -211,21 +232,27 @@ struct Engine {
         mma_step(p);
       }
       constexpr int PL = (KS - 1) & 1;  // buffer of the last K step
-      // cross into stage s + 1: this wave's reads of stage s are complete
-      // (lgkmcnt) and its loads of stage s + 1 have landed (vmcnt).  After the
-      // last stage this is a harmless extra round (every load has landed, no
-      // load is issued, and the reads fill a fragment buffer no MFMA uses);
-      // one unconditional path keeps the compiler from merging waits.
+      // this wave's reads of stage s are complete (lgkmcnt)
       __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0), vmcnt / expcnt untouched
-      wait_stage(min(MFMA_NS - 2, nst - 2 - s));
-      ring_barrier();
-      if (s + MFMA_NS < nst) issue_stage(lds, c_rows, q_rows, a, c0, q0, s + MFMA_NS);
-      read_frags(lds + ((s + 1) % MFMA_NS) * STAGE_LDS, 0, PL ^ 1);
+      if (s + 1 < nst) {
+        // cross into stage s + 1: its loads have landed (vmcnt; two later
+        // stages stay in flight while the stream continues)
+        wait_stage(has_next ? MFMA_NS - 2 : min(MFMA_NS - 2, nst - 2 - s));
+        ring_barrier();
+        if (s + MFMA_NS < nst || has_next) issue(s + MFMA_NS);
+        read_frags(ring(s + 1), 0, PL ^ 1);
+      } else {
+        // last stage: every wave is done reading the ring
+        ring_barrier();
+        if (has_next) issue(s + MFMA_NS);
+      }
       mma_step(PL);
     }
-    // every stage load has landed (the last crossing waited for all); saying
-    // so lets the epilogue's LDS accesses go without further vmcnt waits
-    wait_vmcnt<0>();
+    // without a prefetch every stage load has landed (the last crossing
+    // waited for all); saying so lets the epilogue's LDS accesses go without
+    // further vmcnt waits
+    if (!has_next) wait_vmcnt<0>();
+    ph = (ph + nst) % MFMA_NS;
   }
 };
 
@@ -300,9 +327,11 @@ __global__ __launch_bounds__(64 * WM * WN) void pass1_groupmax_kernel(TileArgs a
   // candidate tiles of this split; group maxima are per tile
   const int t_begin = split * a.tiles_per_split;
   const int t_end = min(a.n_ctiles, t_begin + a.tiles_per_split);
+  const bool chain = En::can_prime(a);
+  int ph = 0;
   for (int t = t_begin; t < t_end; t++) {
     const int c0 = t * En::TC;
-    En::score_tile(acc, lds, a, c0, q0);
+    En::score_tile(acc, lds, a, c0, q0, ph, chain && t > t_begin, chain && t + 1 < t_end, c0 + En::TC, q0);
     // (see pass 2: indices opaque here so derived values are not held through the MFMA loop)
     int lane_e = lane, wm_e = wm, wn_e = wn;
     asm volatile("" : "+v"(lane_e), "+v"(wm_e), "+v"(wn_e));
@@ -365,9 +394,11 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
   // (score_tile begins with a barrier, which publishes these)
   const int t_begin = split * a.tiles_per_split;
   const int t_end = min(a.n_ctiles, t_begin + a.tiles_per_split);
+  const bool chain = En::can_prime(a);
+  int ph = 0;
   for (int t = t_begin; t < t_end; t++) {
     const int c0 = t * En::TC;
-    En::score_tile(acc, lds, a, c0, q0);
+    En::score_tile(acc, lds, a, c0, q0, ph, chain && t > t_begin, chain && t + 1 < t_end, c0 + En::TC, q0);
     // lane / wave indices made opaque here: what the epilogue derives from
     // them is recomputed per tile instead of living in registers through the
     // MFMA loop (which needs them all)
@@ -522,13 +553,16 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
   }
   float pf_tau = INFINITY;
   uint64_t pf_key = 0;
+  const bool chain = En::can_prime(a);
+  int ph = 0;
   for (int t = w.y; t < w.z; t++) {
     const int c0 = t * T;
     const bool diag = t == w.x;  // the query block's own tile: query side only
     __syncthreads();             // the last tile's epilogue and flush are done with these
     if (t != w.y && threadIdx.x < T) store_cbounds(c0, pf_tau, pf_key);  // (t > w.y >= w.x: never diagonal)
     if (t + 1 < w.z && threadIdx.x < T) load_bound(c0 + T + (int)threadIdx.x, pf_tau, pf_key);
-    En::score_tile(acc, lds, a, c0, q0);  // (begins with a barrier, which publishes them)
+    // (begins with a barrier, which publishes them)
+    En::score_tile(acc, lds, a, c0, q0, ph, chain && t > w.y, chain && t + 1 < w.z, c0 + T, q0);
     int lane_e = lane, wm_e = wm, wn_e = wn;
     asm volatile("" : "+v"(lane_e), "+v"(wm_e), "+v"(wn_e));
     // every candidate row of the tile has an exact cheap test
