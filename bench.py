@@ -339,7 +339,7 @@ def run_million(args, torch, E, kind):
         "sampled_rows_bit_exact": f"{exact_rows}/256",
         "rescore_rounds": max(s_.rescore_rounds for s_ in stats),
         "roofline": {"bound": "mfma" if kind == "i8" else "valu", "kernel":
-                     kern if kind == "i8" else "jac_pass2_kernel",
+                     kern if kind == "i8" else ("jac_pass2_sym_kernel" if pairs < float(n) * n else "jac_pass2_kernel"),
                      "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak, "traffic": None,
                      "algorithmic_ops_per_launch": work, "avg_launch_ms": score_s * 1e3},
     })

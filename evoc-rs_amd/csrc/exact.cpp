@@ -152,7 +152,7 @@ SymPlan make_sym_plan(int kind, int64_t n, int K, int n_cu) {
   sp.nqb = nt;
   sp.p = make_plan(kind, n, n, K, n_cu);
   Plan& p = sp.p;
-  if (!p.pass1 || tile_rows_q(kind) != T || kind == EVOC_ELEM_U8BITS) return sp;
+  if (!p.pass1 || tile_rows_q(kind) != T) return sp;
   const int64_t per_cu = tile_lds_bytes(kind, 3) <= 80 * 1024 ? 2 : 1;
   const int64_t slots = (int64_t)n_cu * per_cu;
   const int64_t pairs = nt * (nt + 1) / 2;

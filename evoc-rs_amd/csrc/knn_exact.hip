@@ -839,6 +839,105 @@ __global__ __launch_bounds__(256) void jac_pass2_kernel(TileArgs a) {
   }
 }
 
+// Symmetric Jaccard pass 2 (inter / union is symmetric: the same counts and
+// the same IEEE division for (q, c) and (c, q)): work items as pass2_sym_kernel,
+// each value tested against the query row's bound (its split buffer, as
+// jac_pass2_kernel) and, off the diagonal tile, against the candidate row's
+// bound (staged per candidate row in LDS, flushed once per tile).
+__global__ __launch_bounds__(256) void jac_pass2_sym_kernel(TileArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  int* cnt = (int*)(lds + 2 * TILE * JAC_PITCH);
+  float* ctau_l = (float*)(cnt + TILE);
+  int* scnt = (int*)(ctau_l + TILE);
+  uint64_t* ckey_l = (uint64_t*)(scnt + TILE);
+  uint64_t* stage = ckey_l + TILE;
+  uint32_t inter[8][8];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int per_xcd = (int)gridDim.x >> 3;
+  const int logical = ((int)blockIdx.x & 7) * per_xcd + ((int)blockIdx.x >> 3);
+  if (logical >= a.n_work) return;
+  const int4 w = a.work[logical];
+  const int q0 = w.x * TILE, split = w.w;
+  const bool key_bound = a.tau_key != nullptr;
+  if (threadIdx.x < TILE) cnt[threadIdx.x] = 0;
+  float tau[8];
+  uint64_t tkey[8];
+  uint32_t pq[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int q = q0 + tx + 16 * j;
+    tau[j] = (q < a.n_q && a.tau) ? a.tau[q] : -INFINITY;
+    tkey[j] = (q < a.n_q && key_bound) ? a.tau_key[q] : 0ull;
+    pq[j] = q < a.n_q ? a.row_pop[q] : 0u;
+  }
+  for (int t = w.y; t < w.z; t++) {
+    const int c0 = t * TILE;
+    const bool diag = t == w.x;  // the query block's own tile: query side only
+    __syncthreads();             // the last tile's flush is done with the staging area
+    if (!diag && threadIdx.x < TILE) {
+      const int c = c0 + threadIdx.x;
+      scnt[threadIdx.x] = 0;
+      ctau_l[threadIdx.x] = c < a.n_c ? (a.tau ? a.tau[c] : -INFINITY) : INFINITY;
+      ckey_l[threadIdx.x] = (c < a.n_c && key_bound) ? a.tau_key[c] : 0ull;
+    }
+    jac_tile(inter, lds, a, c0, q0);  // (its barriers publish the bounds)
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int cl = ty + 16 * i;
+      const int c = c0 + cl;
+      if (c >= a.n_c) continue;
+      const uint32_t pc = a.row_pop[c];
+      const float ctau = diag ? INFINITY : ctau_l[cl];
+      const uint64_t ckey = diag ? 0ull : ckey_l[cl];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int ql = tx + 16 * j;
+        const int q = q0 + ql;
+        if (q >= a.n_q) continue;
+        const bool self = (c == q);
+        const float s = self ? 1.0f : jac_score(inter[i][j], pc, pq[j]);
+        bool take = s >= tau[j] && !(self && a.skip_self);
+        if (take && key_bound) take = make_key(s, c, self) >= tkey[j];
+        if (take) {
+          int pos = atomicAdd(&cnt[ql], 1);
+          if (pos < a.caps) a.bufs[((int64_t)split * a.n_q + q) * a.caps + pos] = make_key(s, c, self);
+        }
+        // the pair seen from row c (never self off the diagonal tile)
+        bool takec = s >= ctau;
+        if (takec && key_bound) takec = make_key(s, q, false) >= ckey;
+        if (takec) {
+          const uint64_t key = make_key(s, q, false);
+          const int slot = atomicAdd(&scnt[cl], 1);
+          if (slot < SYM_SLOTS) {
+            stage[cl * SYM_SLOTS + slot] = key;
+          } else {
+            const int pos = atomicAdd(&a.ccounts[c], 1);
+            if (pos < a.ccap) a.cbufs[(int64_t)c * a.ccap + pos] = key;
+          }
+        }
+      }
+    }
+    if (!diag) {
+      __syncthreads();  // every thread's staged keys are in
+      if (threadIdx.x < TILE) {
+        const int i = threadIdx.x;
+        const int k = min(scnt[i], SYM_SLOTS);
+        const int c = c0 + i;
+        if (k > 0 && c < a.n_c) {
+          const int pos = atomicAdd(&a.ccounts[c], k);
+          for (int j = 0; j < k; j++)
+            if (pos + j < a.ccap) a.cbufs[(int64_t)c * a.ccap + pos + j] = stage[i * SYM_SLOTS + j];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < TILE) {
+    const int q = q0 + threadIdx.x;
+    if (q < a.n_q) a.counts[(int64_t)split * a.n_q + q] = cnt[threadIdx.x];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Threshold from group maxima: tau_q = K-th largest of gmax[q, 0..ng).
 // One wave per query, ng <= 64 * E, found by bisection (no sort).
@@ -1223,7 +1322,12 @@ int tile_rows_q(int kind) { int tc, tq, g; shape_dims(kind, tc, tq, g); return t
 int groups_per_tile(int kind) { int tc, tq, g; shape_dims(kind, tc, tq, g); return g; }
 
 size_t tile_lds_bytes(int kind, int pass) {
-  if (kind == EVOC_ELEM_U8BITS) return 2 * TILE * JAC_PITCH + (pass == 2 ? TILE * sizeof(int) : 0);
+  if (kind == EVOC_ELEM_U8BITS) {
+    // pass 2: per-query counters; symmetric pass (3): also the candidate rows'
+    // bounds, staged counts, key bounds and SYM_SLOTS staged keys per row
+    if (pass == 3) return 2 * TILE * JAC_PITCH + TILE * (4 + 4 + 4 + 8) + TILE * SYM_SLOTS * 8;
+    return 2 * TILE * JAC_PITCH + (pass == 2 ? TILE * sizeof(int) : 0);
+  }
   int tc, tq, g;
   shape_dims(kind, tc, tq, g);
   const size_t ring = (size_t)MFMA_NS * (tc + tq) * MFMA_SB;
@@ -1315,10 +1419,11 @@ static hipError_t launch_pass2_sym_kind(const TileArgs& a, dim3 g, size_t lds, h
 hipError_t launch_pass2_sym(int kind, const TileArgs& a, hipStream_t s) {
   int tc, tq, g;
   shape_dims(kind, tc, tq, g);
-  if (tc != tq || kind == EVOC_ELEM_U8BITS) return hipErrorInvalidValue;
+  if (tc != tq) return hipErrorInvalidValue;
   const size_t lds = tile_lds_bytes(kind, 3);
   const dim3 grid((unsigned)(((int64_t)a.n_work + 7) / 8 * 8));
   switch (kind) {
+    case EVOC_ELEM_U8BITS: return launch_tile(jac_pass2_sym_kernel, lds, grid, s, a);
     case EVOC_ELEM_BF16: return launch_pass2_sym_kind<EVOC_ELEM_BF16>(a, grid, lds, s);
     case EVOC_ELEM_F32: return launch_pass2_sym_kind<EVOC_ELEM_F32>(a, grid, lds, s);
     case EVOC_ELEM_I8: return launch_pass2_sym_kind<EVOC_ELEM_I8>(a, grid, lds, s);

@@ -224,17 +224,22 @@ def test_massive_exact_ties(ctx, dtype):
     np.testing.assert_array_equal(idx, ei)
 
 
-@pytest.mark.parametrize("dtype,prec_name", [("f32", "bf16"), ("f32", "fp32"), ("i8", "-")])
+@pytest.mark.parametrize("dtype,prec_name", [("f32", "bf16"), ("f32", "fp32"), ("i8", "-"), ("u8", "-")])
 @pytest.mark.parametrize("n", [9000, 5137])
 def test_symmetric_pass_matches_split_pass(ctx, dtype, prec_name, n):
     """The symmetric scoring pass (each pair of tiles scored once, keys appended to both rows)
     and the per-split pass (every query against every candidate) give identical graphs: the
-    score of a pair is the same products summed in the same order either way.  n = 5137
-    leaves a ragged last tile; int8 is also checked against the oracle."""
+    score of a pair is the same products summed in the same order (binary: the same counts and
+    the same division) either way.  n = 5137 leaves a ragged last tile; int8 and binary are
+    also checked against the oracle."""
     print(f"[gpu] symmetric vs split {dtype} {prec_name} n={n}", flush=True)
     prec = dict(PRECS).get(prec_name, E.PREC_FP32)
-    x = (synth.blobs_f32(n, 192, 30, 0.3, seed=n) if dtype == "f32"
-         else synth.blobs_i8(n, 192, 30, 0.3, seed=n))
+    if dtype == "f32":
+        x = synth.blobs_f32(n, 192, 30, 0.3, seed=n)
+    elif dtype == "i8":
+        x = synth.blobs_i8(n, 192, 30, 0.3, seed=n)
+    else:
+        x = synth.bits_u8(n, 512, 40, 0.3, 0.08, seed=n)
     out = {}
     for sym in ("1", "0"):
         os.environ["EVOC_MI355X_SYMMETRIC"] = sym
@@ -244,7 +249,7 @@ def test_symmetric_pass_matches_split_pass(ctx, dtype, prec_name, n):
             del os.environ["EVOC_MI355X_SYMMETRIC"]
     np.testing.assert_array_equal(out["1"][0], out["0"][0])
     np.testing.assert_array_equal(out["1"][1].view(np.uint32), out["0"][1].view(np.uint32))
-    if dtype == "i8":
+    if dtype in ("i8", "u8"):
         ei, _ = O.exact_knn(x, 25)
         np.testing.assert_array_equal(out["1"][0], ei)
 
