@@ -62,10 +62,11 @@ def recall_gpu(torch, idx, truth):
     return hits / (idx.shape[0] * k)
 
 
-def cpu_baseline(x_np, k, target_s, n_iters=8, delta=0.0):
+def cpu_baseline(x_np, k, target_s, n_iters=8, delta=0.0, keep=None):
     """The reference's CPU path (oracle restatement): non-deterministic NN-descent,
     T host threads, n_iters iterations (delta = delta_improv as given; 0 runs them
-    all), on the first m rows."""
+    all), on the first m rows.  With `keep` (a dict) the last run's indices are
+    stored in keep["indices"] (for recall against the same truth)."""
     import oracle as O
 
     try:
@@ -77,7 +78,9 @@ def cpu_baseline(x_np, k, target_s, n_iters=8, delta=0.0):
 
     def run(m):
         t0 = time.perf_counter()
-        O.knn_graph(x_np[:m], opts, O.NumpyRandomState(42), n_threads=cores, workers=cores)
+        idx, _, _ = O.knn_graph(x_np[:m], opts, O.NumpyRandomState(42), n_threads=cores, workers=cores)
+        if keep is not None:
+            keep["indices"] = idx
         return time.perf_counter() - t0
 
     m = min(2000, x_np.shape[0])
@@ -436,9 +439,14 @@ def run_nndescent(args, torch, E):
     idx = torch.empty((n, k), dtype=torch.int32, device="cuda")
     dist = torch.empty((n, k), dtype=torch.float32, device="cuda")
     out = {}
+    # the reference's options for the host threads its CPU run below uses (n_trees from them)
+    try:
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        threads = max(1, min(16, os.cpu_count() or 1))
     for name, prec in (("bf16", E.PREC_BF16_RESCORE), ("fp32", E.PREC_FP32)):
         opts = E.KnnGraphOptions(n_neighbors=k, n_iters=8, delta=0.0, delta_improv=0.0, mode=E.MODE_NNDESCENT,
-                                 precision=prec, margin=args.margin)
+                                 precision=prec, margin=args.margin, rayon_threads=threads)
 
         def step(st, opts=opts):
             E.knn_graph(x, opts, E.NumpyRandomState(42), ctx=ctx, stats=st, out=(idx, dist))
@@ -459,7 +467,13 @@ def run_nndescent(args, torch, E):
                      "bf16 (fp32 accumulate) + fp32 rescore", {"recall_at_k": best["recall_at_k"], "fp32": out["fp32"],
                                                                "bf16": out["bf16"]})
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(x.cpu().numpy(), k, args.cpu_seconds)
+        keep = {}
+        res["cpu_baseline"] = cpu_baseline(x.cpu().numpy(), k, args.cpu_seconds, keep=keep)
+        ref_idx = keep.get("indices")
+        if ref_idx is not None and ref_idx.shape[0] == n:
+            # the reference's NN-descent at the same iteration count, against the same FP64 truth
+            res["reference_recall_at_k"] = recall_gpu(torch, torch.from_numpy(ref_idx).cuda(), truth)
+            log(f"[bench] reference NN-descent recall@{k} at 8 iterations: {res['reference_recall_at_k']:.5f}")
     return res
 
 

@@ -25,10 +25,12 @@ def ctx():
     c.close()
 
 
-def nnd(ctx, x, k, iters, prec=E.PREC_FP32, seed=42, **kw):
+def nnd(ctx, x, k, iters, prec=E.PREC_FP32, seed=42, threads=8, **kw):
+    """Device NN-descent with the reference's options for `threads` rayon workers
+    (n_trees = clamp(threads, 4, 8), src/knn.rs:404-407), as reference_nnd runs it."""
     st = E.CallStats()
     opts = E.KnnGraphOptions(n_neighbors=k, n_iters=iters, delta=0.0, delta_improv=0.0,
-                             mode=E.MODE_NNDESCENT, precision=prec, **kw)
+                             mode=E.MODE_NNDESCENT, precision=prec, rayon_threads=threads, **kw)
     idx, dist = E.knn_graph(x, opts, E.NumpyRandomState(seed), ctx=ctx, stats=st)
     return idx, dist, st
 
@@ -63,7 +65,7 @@ def test_f32_recall_at_equal_iterations(ctx, prec_name, prec):
           f"iters={st.iterations} dropped={st.offers_dropped}", flush=True)
     # the reference also stops early once an iteration changes nothing (float.rs:847)
     assert 1 <= st.iterations <= iters
-    assert r_gpu >= r_ref - 0.002
+    assert r_gpu >= r_ref
     # every reported distance is the FP32 transform of the listed pair
     xs = O.prepare_float(x)
     sample = np.random.default_rng(0).integers(0, n, 200)
@@ -84,7 +86,7 @@ def test_int8_recall_at_equal_iterations(ctx):
     r_gpu = recall(idx, truth)
     r_ref = recall(reference_nnd(x, k, iters), truth)
     print(f"[gpu] nnd int8: recall gpu={r_gpu:.5f} reference={r_ref:.5f} dropped={st.offers_dropped}", flush=True)
-    assert r_gpu >= r_ref - 0.002
+    assert r_gpu >= r_ref
 
 
 def test_binary_recall_at_equal_iterations(ctx):
@@ -96,7 +98,7 @@ def test_binary_recall_at_equal_iterations(ctx):
     r_gpu = recall(idx, truth)
     r_ref = recall(reference_nnd(x, k, iters), truth)
     print(f"[gpu] nnd binary: recall gpu={r_gpu:.5f} reference={r_ref:.5f} dropped={st.offers_dropped}", flush=True)
-    assert r_gpu >= r_ref - 0.005
+    assert r_gpu >= r_ref
 
 
 def test_nndescent_advances_rng_like_reference(ctx):
@@ -130,8 +132,8 @@ def test_nndescent_c2_shape_20k(ctx):
         truth[s:e] = torch.topk(sim, k, dim=1).indices.cpu().numpy()
     r_ref = recall(reference_nnd(x, k, iters, threads=16), truth)
     for name, prec in [("fp32", E.PREC_FP32), ("bf16", E.PREC_BF16_RESCORE)]:
-        idx, dist, st = nnd(ctx, x, k, iters, prec)
+        idx, dist, st = nnd(ctx, x, k, iters, prec, threads=16)
         r = recall(idx, truth)
         print(f"[gpu] nnd c2-20k {name}: recall={r:.5f} reference={r_ref:.5f} "
               f"time={st.seconds*1e3:.1f} ms dropped={st.offers_dropped}", flush=True)
-        assert r >= r_ref - 0.002
+        assert r >= r_ref
