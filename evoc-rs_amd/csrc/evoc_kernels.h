@@ -89,8 +89,10 @@ hipError_t launch_select(const SelectArgs& a, hipStream_t s);
 hipError_t launch_pass2_sym(int kind, const TileArgs& a, hipStream_t s);
 // zero ccounts of the queries in list[0..n)
 hipError_t launch_zero_list(int32_t* ccounts, const int32_t* list, int n, hipStream_t s);
+// order: optional permutation of the n_q queries to process them in (null: index order)
 hipError_t launch_rescore(const float* X, int64_t ld, int d, int n_q, int q_offset,
-                          const uint64_t* cand, int Kc, int k, uint64_t* out, hipStream_t s);
+                          const uint64_t* cand, int Kc, int k, uint64_t* out, hipStream_t s,
+                          const int32_t* order = nullptr);
 hipError_t launch_finalize(const uint64_t* keys, int n_q, int q_offset, int k, int dtype,
                            int32_t* idx, float* dist, hipStream_t s);
 
@@ -105,7 +107,13 @@ hipError_t launch_pad_rows(const uint8_t* X, int64_t ld_bytes, int64_t n, int ro
 hipError_t launch_row_popcount(const uint8_t* X, int64_t ld, int64_t n, int row_bytes,
                                uint32_t* pop, hipStream_t s);
 
-// ---- launchers (fuzzy.hip): the fuzzy graph and find_duplicates ----
+// ---- launchers (fuzzy.hip): locality order, the fuzzy graph and find_duplicates ----
+// A permutation of the n rows ordered by their projection on a fixed +-1
+// vector (rows of one cluster come out together).  keys / keys_out: n uint32,
+// order_in: n int32 scratch, order: n int32 result; temp == nullptr queries
+// temp_bytes.
+hipError_t locality_order(void* temp, size_t& temp_bytes, const float* X, int64_t ld, int64_t n, int d,
+                          uint32_t* keys, uint32_t* keys_out, int32_t* order_in, int32_t* order, hipStream_t s);
 size_t fuzzy_partial_doubles();  // scratch doubles of launch_fuzzy_sigma's mean
 hipError_t launch_fuzzy_sigma(const float* dist, int64_t n, int k, float n_neighbors, double* partial,
                               float* sigma, float* rho, hipStream_t s);

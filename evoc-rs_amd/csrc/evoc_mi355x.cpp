@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -483,6 +484,13 @@ static CallShape call_shape(const evoc_knn_input* in, const evoc_knn_options* o,
   return c;
 }
 
+// The FP32 re-score takes its queries in a cluster-coherent order unless
+// EVOC_MI355X_RESCORE_ORDER=0 (the order changes speed only).
+static bool rescore_order_enabled() {
+  const char* e = std::getenv("EVOC_MI355X_RESCORE_ORDER");
+  return !(e && e[0] == '0');
+}
+
 // Device bytes of everything the call holds besides the search's own scratch.
 static uint64_t fixed_bytes(const CallShape& c) {
   uint64_t b = 0;
@@ -495,6 +503,11 @@ static uint64_t fixed_bytes(const CallShape& c) {
   if (c.dtype == EVOC_DTYPE_U8) b += r256((uint64_t)c.n * 4);
   b += r256((uint64_t)(c.n * c.K * 8));
   if (c.bf16 || c.K != c.k) b += r256((uint64_t)(c.n * c.k * 8));
+  if ((c.bf16 || c.K != c.k) && c.dtype == EVOC_DTYPE_F32 && rescore_order_enabled() && c.n >= 4096) {
+    size_t tb = 0;  // the re-score's query order: four n-entry arrays and the sort's scratch
+    (void)locality_order(nullptr, tb, nullptr, 0, c.n, (int)c.d, nullptr, nullptr, nullptr, nullptr, nullptr);
+    b += 4 * r256((uint64_t)c.n * 4) + r256(tb > 0 ? tb : 1);
+  }
   if (!c.dev_out) b += 2 * r256((uint64_t)(c.n * c.k * 4));
   return b;
 }
@@ -679,8 +692,26 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
   const uint64_t* final_keys = (const uint64_t*)keys.ptr;
   if (bf16 || K != k) {
     if (dtype == EVOC_DTYPE_F32) {
-      e = launch_rescore((const float*)(bf16 ? xf32.ptr : rows.ptr), f32_ld, (int)d, (int)n, 0,
-                         (const uint64_t*)keys.ptr, (int)K, (int)k, (uint64_t*)fkeys.ptr, st);
+      const float* xr = (const float*)(bf16 ? xf32.ptr : rows.ptr);
+      // queries in a cluster-coherent order, so the candidate rows they gather
+      // are shared by the queries in flight (cache hits instead of HBM reads)
+      DevBuf ok, ok2, oin, oord, otemp;
+      const int32_t* order = nullptr;
+      if (rescore_order_enabled() && n >= 4096) {
+        size_t tb = 0;
+        e = locality_order(nullptr, tb, xr, f32_ld, n, (int)d, nullptr, nullptr, nullptr, nullptr, st);
+        if (e != hipSuccess) return ctx->hip_fail(e, "rescore order");
+        if ((rc = ok.alloc(ctx, (size_t)n * 4, "order keys")) || (rc = ok2.alloc(ctx, (size_t)n * 4, "order keys")) ||
+            (rc = oin.alloc(ctx, (size_t)n * 4, "order")) || (rc = oord.alloc(ctx, (size_t)n * 4, "order")) ||
+            (rc = otemp.alloc(ctx, tb > 0 ? tb : 1, "order sort scratch")))
+          return rc;
+        e = locality_order(otemp.ptr, tb, xr, f32_ld, n, (int)d, (uint32_t*)ok.ptr, (uint32_t*)ok2.ptr,
+                           (int32_t*)oin.ptr, (int32_t*)oord.ptr, st);
+        if (e != hipSuccess) return ctx->hip_fail(e, "rescore order");
+        order = (const int32_t*)oord.ptr;
+      }
+      e = launch_rescore(xr, f32_ld, (int)d, (int)n, 0, (const uint64_t*)keys.ptr, (int)K, (int)k,
+                         (uint64_t*)fkeys.ptr, st, order);
     } else {
       e = hipMemcpy2DAsync(fkeys.ptr, (size_t)k * 8, keys.ptr, (size_t)K * 8, (size_t)k * 8, (size_t)n,
                            hipMemcpyDeviceToDevice, st);

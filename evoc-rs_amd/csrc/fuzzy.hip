@@ -225,6 +225,35 @@ __global__ __launch_bounds__(256) void fz_pairs_kernel(const uint64_t* __restric
 
 static inline unsigned blocks_for(int64_t m) { return (unsigned)((m + 255) / 256); }
 
+// Projection of each row on the vector whose element i is +1 or -1 by a hash
+// bit of i, as an order-preserving integer; one wave per row.
+__global__ __launch_bounds__(256) void proj_key_kernel(const float* __restrict__ X, int64_t ld, int64_t n, int d,
+                                                       uint32_t* __restrict__ keys, int32_t* __restrict__ iota) {
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= n) return;
+  float acc = 0.0f;
+  for (int i = lane; i < d; i += 64) {
+    const uint32_t h = (uint32_t)i * 0x9E3779B1u;
+    acc += (h & 0x80000000u) ? -X[r * ld + i] : X[r * ld + i];
+  }
+  acc = wave_sum_f32(acc);
+  if (lane == 0) {
+    keys[r] = ord_f32(acc);
+    iota[r] = (int32_t)r;
+  }
+}
+
+hipError_t locality_order(void* temp, size_t& temp_bytes, const float* X, int64_t ld, int64_t n, int d,
+                          uint32_t* keys, uint32_t* keys_out, int32_t* order_in, int32_t* order, hipStream_t s) {
+  if (!temp)
+    return rocprim::radix_sort_pairs(nullptr, temp_bytes, keys, keys_out, order_in, order, (size_t)n, 0, 32, s);
+  hipLaunchKernelGGL(proj_key_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, X, ld, n, d, keys, order_in);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_out, order_in, order, (size_t)n, 0, 32, s);
+}
+
 hipError_t launch_dup_pairs(const uint64_t* keys, int64_t m, int64_t n, uint64_t* pairs, hipStream_t s) {
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(fz_pairs_kernel, dim3(blocks_for(m)), dim3(256), 0, s, keys, m, n, pairs);

@@ -1178,14 +1178,18 @@ template <int E>
 __global__ __launch_bounds__(256) void rescore_kernel(const float* __restrict__ X, int64_t ld, int d,
                                                       int n_q, int q_offset,
                                                       const uint64_t* __restrict__ cand_keys, int Kc,
-                                                      int k, uint64_t* __restrict__ out_keys) {
+                                                      int k, uint64_t* __restrict__ out_keys,
+                                                      const int32_t* __restrict__ order) {
   // four candidates at a time, one 16-lane group each; lane t of a group sums
   // the float4 chunks t, t + 16, ... of the row with fmaf (fixed order), then a
-  // fixed xor-butterfly over the 16 lanes
+  // fixed xor-butterfly over the 16 lanes.  Queries are taken in `order` when
+  // given (a permutation of the slab's queries: only the order of the work
+  // changes, each query's result is its own)
   const int lane = threadIdx.x & 63;
   const int grp = lane >> 4, t = lane & 15;
-  const int ql = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (ql >= n_q) return;
+  const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (qi >= n_q) return;
+  const int ql = order ? order[qi] : qi;
   const int q = ql + q_offset;
   const float* xq = X + (int64_t)q * ld;
   const int nv4 = d >> 2;  // full float4 chunks; ld is a multiple of 32 and rows are zero padded
@@ -1477,12 +1481,13 @@ hipError_t launch_select(const SelectArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_rescore(const float* X, int64_t ld, int d, int n_q, int q_offset,
-                          const uint64_t* cand, int Kc, int k, uint64_t* out, hipStream_t s) {
+                          const uint64_t* cand, int Kc, int k, uint64_t* out, hipStream_t s,
+                          const int32_t* order) {
   const dim3 g((unsigned)((n_q + 3) / 4)), b(256);
   switch (pow2_per_lane(Kc)) {
-    case 1: hipLaunchKernelGGL(rescore_kernel<1>, g, b, 0, s, X, ld, d, n_q, q_offset, cand, Kc, k, out); break;
-    case 2: hipLaunchKernelGGL(rescore_kernel<2>, g, b, 0, s, X, ld, d, n_q, q_offset, cand, Kc, k, out); break;
-    case 4: hipLaunchKernelGGL(rescore_kernel<4>, g, b, 0, s, X, ld, d, n_q, q_offset, cand, Kc, k, out); break;
+    case 1: hipLaunchKernelGGL(rescore_kernel<1>, g, b, 0, s, X, ld, d, n_q, q_offset, cand, Kc, k, out, order); break;
+    case 2: hipLaunchKernelGGL(rescore_kernel<2>, g, b, 0, s, X, ld, d, n_q, q_offset, cand, Kc, k, out, order); break;
+    case 4: hipLaunchKernelGGL(rescore_kernel<4>, g, b, 0, s, X, ld, d, n_q, q_offset, cand, Kc, k, out, order); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
