@@ -86,7 +86,11 @@ def test_admission_by_budget():
         ia, da = E.knn_graph(x, o, E.NumpyRandomState(s), ctx=ctx)
         np.testing.assert_array_equal(ia, r[0])
     ctx.close()
-    big = synth.blobs_f32(40000, 256, 20, 0.3, seed=9)
+    big = synth.blobs_f32(200000, 256, 20, 0.3, seed=9)
+    probe = E.Server(0, slots=1)
+    _, big_low = probe.request_bytes(big, o)
+    probe.close()
+    assert big_low > budget  # the request can never fit, even in the smallest slabs
     with pytest.raises(E.BackendError) as ei:
         srv.submit(big, o, E.NumpyRandomState(0))
     assert ei.value.code == E.ERR_OUT_OF_BUDGET
