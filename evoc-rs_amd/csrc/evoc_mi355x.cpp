@@ -467,7 +467,10 @@ static CallShape call_shape(const evoc_knn_input* in, const evoc_knn_options* o,
   // NN-descent mode runs the device NN-descent; the brute-force branch
   // (src/knn.rs:418-422) and the exact mode run the tiled exact search.
   c.nnd = o->mode == EVOC_MODE_NNDESCENT && c.seed_self;
-  c.K = c.bf16 ? std::min<int64_t>(c.k + std::max(0, o->margin), c.n) : c.k;
+  // k + margin candidates: the BF16 form re-scores them in FP32; NN-descent
+  // keeps that many list entries per vertex in every precision (wider lists
+  // explore more per iteration; the best k are emitted)
+  c.K = (c.bf16 || c.nnd) ? std::min<int64_t>(c.k + std::max(0, o->margin), c.n) : c.k;
   if (c.nnd) c.K = std::min<int64_t>(c.K, 64);
   int64_t elem_bytes = 1;
   if (c.dtype == EVOC_DTYPE_F32) {

@@ -17,7 +17,7 @@ using namespace evoc;
 
 namespace {
 
-constexpr int UCAP = 192;  // offers buffered per vertex between merges
+constexpr int UCAP = NND_OFFER_CAP;  // offers buffered per vertex between merges
 
 uint64_t mix(uint64_t z) {
   z += 0x9e3779b97f4a7c15ull;
@@ -67,10 +67,10 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
   const int leaf_size = std::min<int64_t>(std::max<int64_t>(P.leaf_size, 2), 64);
   const int nb = (n + 1023) / 1024;
 
-  DevBuf keys, thr, ucnt, ubuf, newc, oldc, revcnt, revoff, revcur, rev, sums, counters, sumd;
+  DevBuf keys, thr, umark, ubuf, newc, oldc, revcnt, revoff, revcur, rev, sums, counters, sumd;
   int rc;
   if ((rc = keys.alloc(ctx, (size_t)n * Kg * 8, "neighbour lists")) || (rc = thr.alloc(ctx, (size_t)n * 4, "list bounds")) ||
-      (rc = ucnt.alloc(ctx, (size_t)n * 4, "offer counters")) ||
+      (rc = umark.alloc(ctx, (size_t)n * 4, "offer marks")) ||
       (rc = ubuf.alloc(ctx, (size_t)n * UCAP * 8, "offer buffers")) ||
       (rc = newc.alloc(ctx, (size_t)n * mc * 4, "new candidates")) ||
       (rc = oldc.alloc(ctx, (size_t)n * mc * 4, "old candidates")) ||
@@ -85,14 +85,15 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
   uint32_t* d_dropped = d_changed + 1;
   Check ck{ctx};
   ck(launch_fill_i32((int32_t*)counters.ptr, 4, 0, s), "counters");
-  ck(nnd_launch_init((uint64_t*)keys.ptr, (float*)thr.ptr, (int32_t*)ucnt.ptr, n, Kg, s), "list init");
+  ck(nnd_launch_init((uint64_t*)keys.ptr, (float*)thr.ptr, (int32_t*)umark.ptr, n, Kg, s), "list init");
+  ck(launch_fill_i32((int32_t*)ubuf.ptr, (int64_t)n * UCAP * 2, 0, s), "offer slots");  // EVOC_KEY_NONE
 
   JoinArgs ja = {};
   ja.rows = rows;
   ja.row_bytes = row_bytes;
   ja.row_pop = pop;
   ja.thr = (const float*)thr.ptr;
-  ja.ucnt = (int32_t*)ucnt.ptr;
+  ja.umark = (int32_t*)umark.ptr;
   ja.ubuf = (uint64_t*)ubuf.ptr;
   ja.ucap = UCAP;
   ja.n = n;
@@ -102,8 +103,8 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
   ja.mc = mc;
 
   auto merge_all = [&]() {
-    return ck(nnd_launch_merge((uint64_t*)keys.ptr, (float*)thr.ptr, (int32_t*)ucnt.ptr,
-                               (const uint64_t*)ubuf.ptr, UCAP, n, Kg, 0, n, d_changed, s),
+    return ck(nnd_launch_merge((uint64_t*)keys.ptr, (float*)thr.ptr, (int32_t*)umark.ptr,
+                               (uint64_t*)ubuf.ptr, UCAP, n, Kg, 0, n, d_changed, s),
               "merge");
   };
 
@@ -176,6 +177,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
       jl.n_items = nl;
       jl.leaves = (const int2*)leaves.ptr;
       jl.perm = d_perm;
+      jl.salt = 1;
       if (nl > 0) ck(nnd_launch_join(kind, jl, s), "leaf join");
       merge_all();
       if (st) st->leaves += nl;
@@ -191,6 +193,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
     jr.mode = 0;
     jr.v0 = 0;
     jr.n_items = n;
+    jr.salt = 2;
     ck(nnd_launch_join(kind, jr, s), "random join");
     merge_all();
   }
@@ -211,6 +214,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
       jj.mode = 0;
       jj.v0 = (int32_t)v0;
       jj.n_items = (int32_t)std::min<int64_t>(block, n - v0);
+      jj.salt = 3 + (uint32_t)it * 4096u + (uint32_t)(v0 / 1024);
       ck(nnd_launch_join(kind, jj, s), "local join");
       merge_all();
     }

@@ -5,6 +5,13 @@
 
 namespace evoc {
 
+// Offer slots per vertex between merges (a multiple of 64: the merge sorts the
+// list and the slots in one wave sort of 64 + NND_OFFER_CAP keys).  An offer
+// takes the 64-bit maximum into two slots picked by a hash of (target,
+// candidate, salt), so each slot ends as the best offer hashed to it whatever
+// the arrival order, and repeated offers of one pair land on the same slots.
+constexpr int NND_OFFER_CAP = 448;
+
 struct JoinArgs {
   int32_t mode;     // 0: candidate lists of vertices v0.., 1: forest leaves
   int32_t n_items;  // work items (one wave each)
@@ -18,11 +25,12 @@ struct JoinArgs {
   int64_t row_bytes;
   const uint32_t* row_pop;  // binary rows
   const float* thr;         // per-vertex score of the current Kg-th entry
-  int32_t* ucnt;            // per-vertex offer counters
-  uint64_t* ubuf;           // [n][ucap] offers
+  int32_t* umark;           // per-vertex flag: offer slots written since the last merge
+  uint64_t* ubuf;           // [n][ucap] offer slots, EVOC_KEY_NONE when empty
   int32_t ucap;
   int32_t n;
-  uint32_t* dropped;        // offers beyond ucap
+  uint32_t salt;            // varies the slot hash between launches
+  uint32_t* dropped;        // offers that met a better offer on both slots when they arrived
 };
 
 struct ForestArgs {
@@ -39,9 +47,9 @@ struct ForestArgs {
   uint64_t seed;
 };
 
-hipError_t nnd_launch_init(uint64_t* keys, float* thr, int32_t* ucnt, int n, int Kg, hipStream_t s);
+hipError_t nnd_launch_init(uint64_t* keys, float* thr, int32_t* umark, int n, int Kg, hipStream_t s);
 hipError_t nnd_launch_join(int kind, const JoinArgs& a, hipStream_t s);
-hipError_t nnd_launch_merge(uint64_t* keys, float* thr, int32_t* ucnt, const uint64_t* ubuf, int ucap,
+hipError_t nnd_launch_merge(uint64_t* keys, float* thr, int32_t* umark, uint64_t* ubuf, int ucap,
                             int n, int Kg, int v0, int nv, uint32_t* changed, hipStream_t s);
 hipError_t nnd_launch_candidates(uint64_t* keys, int n, int Kg, int32_t* revcnt, int32_t* revoff,
                                  int32_t* revcur, int32_t* rev, int32_t* scan_sums, int mc,

@@ -49,14 +49,14 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 // float.rs:210-214), the rest empty.
 // ---------------------------------------------------------------------------
 __global__ void nnd_init_lists_kernel(uint64_t* __restrict__ keys, float* __restrict__ thr,
-                                      int32_t* __restrict__ ucnt, int n, int Kg) {
+                                      int32_t* __restrict__ umark, int n, int Kg) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)n * Kg) return;
   const int v = (int)(i / Kg), s = (int)(i % Kg);
   keys[i] = s == 0 ? gkey(1.0f, v, 0, 1) : EVOC_KEY_NONE;
   if (s == 0) {
     thr[v] = Kg == 1 ? 1.0f : -INFINITY;
-    ucnt[v] = 0;
+    umark[v] = 0;
   }
 }
 
@@ -72,12 +72,21 @@ struct JoinShared {
   float cthr[128];
 };
 
+// An offer of cand to target: the 64-bit maximum into two hashed slots of the
+// target's buffer.  A slot's final content is the best key hashed to it, so the
+// merged lists do not depend on the order in which waves arrive; an offer is
+// lost only when a better one holds both of its slots.
 __device__ __forceinline__ void offer(const JoinArgs& a, int32_t target, int32_t cand, float s) {
-  const int slot = atomicAdd(&a.ucnt[target], 1);
-  if (slot < a.ucap)
-    a.ubuf[(int64_t)target * a.ucap + slot] = gkey(s, cand, 0, 1);
-  else
-    atomicAdd(a.dropped, 1u);
+  const unsigned long long key = gkey(s, cand, 0, 1);
+  const uint64_t h = mix64((((uint64_t)(uint32_t)target << 32) | (uint32_t)cand) ^
+                           ((uint64_t)a.salt * 0xd1b54a32d192ed03ull));
+  const uint32_t s1 = (uint32_t)(((h & 0xffffffffull) * (uint64_t)a.ucap) >> 32);
+  const uint32_t s2 = (uint32_t)(((h >> 32) * (uint64_t)a.ucap) >> 32);
+  unsigned long long* b = (unsigned long long*)(a.ubuf + (int64_t)target * a.ucap);
+  const unsigned long long o1 = atomicMax(b + s1, key);
+  const unsigned long long o2 = s2 != s1 ? atomicMax(b + s2, key) : o1;
+  if (o1 > key && o2 > key) atomicAdd(a.dropped, 1u);
+  a.umark[target] = 1;
 }
 
 // gather the item's lists into LDS; returns (nN, nC) through refs
@@ -209,31 +218,37 @@ __global__ __launch_bounds__(256) void nnd_join_jaccard_kernel(JoinArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Merge offers into the lists: one wave per vertex, list (<= 64) + offers
-// (<= 192) sorted best first, duplicates of one neighbour collapse onto the
-// copy already in the list (flagged_heap_push rejects a present index,
-// src/heap.rs:156-160), the first Kg survive.  New entries carry flag 1.
+// Merge offers into the lists: one wave per marked vertex, list (<= 64) +
+// offer slots (NND_OFFER_CAP) sorted best first, duplicates of one neighbour
+// collapse onto the copy already in the list (flagged_heap_push rejects a
+// present index, src/heap.rs:156-160), the first Kg survive.  New entries
+// carry flag 1.  The slots are emptied for the next round of offers.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void nnd_merge_kernel(uint64_t* __restrict__ keys,
                                                         float* __restrict__ thr,
-                                                        int32_t* __restrict__ ucnt,
-                                                        const uint64_t* __restrict__ ubuf, int ucap,
+                                                        int32_t* __restrict__ umark,
+                                                        uint64_t* __restrict__ ubuf, int ucap,
                                                         int n, int Kg, int v0, int nv,
                                                         uint32_t* __restrict__ changed) {
-  constexpr int E = 4;
+  constexpr int E = 1 + NND_OFFER_CAP / 64;  // list, then the offers
+  static_assert(NND_OFFER_CAP % 64 == 0, "offer buffers fill whole registers of the wave sort");
   const int lane = threadIdx.x & 63;
   const int vi = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (vi >= nv) return;
   const int v = v0 + vi;
-  const int cnt = min(ucnt[v], ucap);
-  if (cnt == 0) return;
+  if (umark[v] == 0) return;
   uint64_t x[E];
   x[0] = lane < Kg ? (keys[(int64_t)v * Kg + lane] | (keys[(int64_t)v * Kg + lane] ? 2ull : 0ull))
                    : EVOC_KEY_NONE;
 #pragma unroll
   for (int e = 1; e < E; e++) {
     const int u = (e - 1) * 64 + lane;
-    x[e] = u < cnt ? ubuf[(int64_t)v * ucap + u] : EVOC_KEY_NONE;
+    x[e] = u < ucap ? ubuf[(int64_t)v * ucap + u] : EVOC_KEY_NONE;
+  }
+#pragma unroll
+  for (int e = 1; e < E; e++) {
+    const int u = (e - 1) * 64 + lane;
+    if (u < ucap) ubuf[(int64_t)v * ucap + u] = EVOC_KEY_NONE;
   }
   wave_sort_desc<E>(x);
   // keep[i]: non-empty and not a repeat of element i - 1 (same score and index)
@@ -266,7 +281,7 @@ __global__ __launch_bounds__(256) void nnd_merge_kernel(uint64_t* __restrict__ k
   inserted = wave_sum_i32(inserted);
   if (lane == 0) {
     atomicAdd(changed, (uint32_t)inserted);
-    ucnt[v] = 0;
+    umark[v] = 0;
   }
 }
 
@@ -678,9 +693,9 @@ __global__ void fill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v) {
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
-hipError_t nnd_launch_init(uint64_t* keys, float* thr, int32_t* ucnt, int n, int Kg, hipStream_t s) {
+hipError_t nnd_launch_init(uint64_t* keys, float* thr, int32_t* umark, int n, int Kg, hipStream_t s) {
   hipLaunchKernelGGL(nnd_init_lists_kernel, dim3(blocks_for((int64_t)n * Kg, 256)), dim3(256), 0, s, keys,
-                     thr, ucnt, n, Kg);
+                     thr, umark, n, Kg);
   return hipGetLastError();
 }
 
@@ -703,9 +718,10 @@ hipError_t nnd_launch_random_lists(const uint64_t* keys, int n, int Kg, int mc, 
   return hipGetLastError();
 }
 
-hipError_t nnd_launch_merge(uint64_t* keys, float* thr, int32_t* ucnt, const uint64_t* ubuf, int ucap, int n,
+hipError_t nnd_launch_merge(uint64_t* keys, float* thr, int32_t* umark, uint64_t* ubuf, int ucap, int n,
                             int Kg, int v0, int nv, uint32_t* changed, hipStream_t s) {
-  hipLaunchKernelGGL(nnd_merge_kernel, dim3(blocks_for(nv, 4)), dim3(256), 0, s, keys, thr, ucnt, ubuf, ucap,
+  if (ucap < 1 || ucap > NND_OFFER_CAP || Kg < 1 || Kg > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(nnd_merge_kernel, dim3(blocks_for(nv, 4)), dim3(256), 0, s, keys, thr, umark, ubuf, ucap,
                      n, Kg, v0, nv, changed);
   return hipGetLastError();
 }

@@ -73,7 +73,8 @@ typedef struct evoc_knn_options {
   /* mi355x extensions */
   int32_t mode;              /* enum evoc_mode, default EXACT */
   int32_t precision;         /* enum evoc_precision, default BF16_RESCORE */
-  int32_t margin;            /* BF16 candidates kept beyond k for the FP32 re-score, default 16 */
+  int32_t margin;            /* candidates kept beyond k, default 16: the BF16 form re-scores them
+                                in FP32; NN-descent keeps k + margin list entries per vertex */
   int32_t strict_precision;  /* RlxBackend strict_precision; nonzero -> EVOC_ERR_NOT_BITWISE */
   int32_t rayon_threads;     /* caller's rayon::current_num_threads() for the n_trees default (0 = 1) */
   int32_t reserved0;
@@ -101,7 +102,9 @@ typedef struct evoc_knn_output {
   double score_seconds;    /* device time of the all-pairs scoring kernel (HIP events) */
   int64_t iterations;      /* NN-descent iterations run (0 for exact) */
   int64_t rescore_rounds;  /* exact: extra scoring rounds for overflowing buffers */
-  int64_t offers_dropped;  /* NN-descent: candidate offers lost to full per-vertex buffers */
+  int64_t offers_dropped;  /* NN-descent: offers that found a better offer in both of their
+                              hashed buffer slots on arrival (a diagnostic that may vary between
+                              calls; the lists do not depend on arrival order) */
   double queue_seconds;    /* serving: time from submit until a slot started the request */
   int64_t score_pairs;     /* exact: row pairs the scoring kernel scored (one per unordered pair
                               in the symmetric pass, one per query and candidate otherwise) */

@@ -111,6 +111,8 @@ def test_nndescent_advances_rng_like_reference(ctx):
 
 
 def test_nndescent_deterministic_across_calls(ctx):
+    """Same inputs, same graph: each offer slot keeps the best offer hashed to it,
+    whatever order the waves arrive in, so the lists are reproducible."""
     x = synth.blobs_f32(3000, 48, 10, 0.3, seed=9)
     a = nnd(ctx, x, 10, 4)
     b = nnd(ctx, x, 10, 4)
