@@ -569,108 +569,140 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
     bool c_fast = !diag;
 #pragma unroll
     for (int g = 0; g < T / 64; g++) c_fast = c_fast && cexact[g] != 0;
-    static_for<BM>([&](auto bmc) __attribute__((always_inline)) {
-      constexpr int bm = decltype(bmc)::value;
-      __builtin_amdgcn_sched_barrier(0);
+    // One accumulator block at a time, rolled: the block's 16 values are copied
+    // out under a uniform branch, so the epilogue's code is one block's worth
+    // (unrolled over all BM x BN blocks it outgrows the instruction cache and
+    // is fetched again on every tile).
+#pragma unroll 1
+    for (int sb = 0; sb < BM * BN; sb++) {
+      const int bm = sb / BN, bn = sb % BN;
+      float v[16];  // raw accumulator values of this block
+      static_for<BM * BN>([&](auto sbc) __attribute__((always_inline)) {
+        constexpr int c = decltype(sbc)::value;
+        if (sb == c) {
+#pragma unroll
+          for (int j = 0; j < 16; j++) v[j] = (float)acc[c / BN][c % BN][j];
+        }
+      });
+      auto val = [&](int j) __attribute__((always_inline)) {
+        return KIND == EVOC_ELEM_I8 ? v[j] : f32_score(v[j]);
+      };
+      // value j of the block for a per-lane j: a tree of bitwise selects on
+      // all-ones / zero masks, no branches.  The masks are opaque so that the
+      // tree is not folded back into an indexed load (which would put the
+      // block in scratch memory).
+      auto pick = [&](int j) __attribute__((always_inline)) {
+        uint32_t m0 = 0u - (uint32_t)(j & 1), m1 = 0u - (uint32_t)((j >> 1) & 1);
+        uint32_t m2 = 0u - (uint32_t)((j >> 2) & 1), m3 = 0u - (uint32_t)((j >> 3) & 1);
+        asm volatile("" : "+v"(m0), "+v"(m1), "+v"(m2), "+v"(m3));
+        auto sel = [](uint32_t m, uint32_t hi, uint32_t lo) __attribute__((always_inline)) {
+          return (m & hi) | (~m & lo);
+        };
+        uint32_t t8[8], t4[4], t2[2];
+#pragma unroll
+        for (int i = 0; i < 8; i++) t8[i] = sel(m0, __float_as_uint(v[2 * i + 1]), __float_as_uint(v[2 * i]));
+#pragma unroll
+        for (int i = 0; i < 4; i++) t4[i] = sel(m1, t8[2 * i + 1], t8[2 * i]);
+#pragma unroll
+        for (int i = 0; i < 2; i++) t2[i] = sel(m2, t4[2 * i + 1], t4[2 * i]);
+        const float raw = __uint_as_float(sel(m3, t2[1], t2[0]));
+        return KIND == EVOC_ELEM_I8 ? raw : f32_score(raw);
+      };
       const int cb = c0 + wm_e * 32 * BM + bm * 32;  // first candidate of the block row
       // candidate-side fast bounds of this lane's 16 rows 8*(j/4) + 4*(lane/32) + j%4
       float cf[16];
       const int rl = cb - c0 + 4 * (lane_e >> 5);
 #pragma unroll
       for (int g = 0; g < 4; g++) {
-        const float4 v = diag ? make_float4(INFINITY, INFINITY, INFINITY, INFINITY)
+        const float4 f = diag ? make_float4(INFINITY, INFINITY, INFINITY, INFINITY)
                               : *(const float4*)(cfast_l + rl + 8 * g);
-        cf[4 * g] = v.x;
-        cf[4 * g + 1] = v.y;
-        cf[4 * g + 2] = v.z;
-        cf[4 * g + 3] = v.w;
+        cf[4 * g] = f.x;
+        cf[4 * g + 1] = f.y;
+        cf[4 * g + 2] = f.z;
+        cf[4 * g + 3] = f.w;
       }
-      static_for<BN>([&](auto bnc) __attribute__((always_inline)) {
-        constexpr int bn = decltype(bnc)::value;
-        __builtin_amdgcn_sched_barrier(0);
-        const int ql = wn_e * 32 * BN + bn * 32 + (lane_e & 31);
-        const int q = q0 + ql;
-        const float tfast = tfast_l[ql];
-        const bool dblk = diag && cb <= q + 31 - (lane_e & 31) && q - (lane_e & 31) <= cb + 31;
-        uint32_t m = 0, mc = 0;
+      const int ql = wn_e * 32 * BN + bn * 32 + (lane_e & 31);
+      const int q = q0 + ql;
+      const float tfast = tfast_l[ql];
+      const bool dblk = diag && cb <= q + 31 - (lane_e & 31) && q - (lane_e & 31) <= cb + 31;
+      uint32_t m = 0, mc = 0;
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-          const float raw = (float)acc[bm][bn][j];
-          m |= (raw >= tfast ? 1u : 0u) << j;
-          mc |= (raw >= cf[j] ? 1u : 0u) << j;
-        }
-        if (dblk) m = 0xffffu;  // the self pair takes the exact path
-        if (q >= a.n_q) mc = 0;  // a query row past the end is nobody's neighbour
-        if (__ballot((m | mc) != 0) == 0) return;
-        if (m) {  // query side, as pass 2
-          const float tau = tau_l[ql];
-          uint32_t take;
-          if (!dblk && bound_is_exact<KIND>(tau, key_bound)) {
-            take = m & valid_rows(cb, lane_e, a.n_c);
-          } else {
-            const uint64_t tkey = tkey_l[ql];
-            take = 0;
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-              if (m & (1u << j)) {
-                const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
-                const bool self = (c == q);
-                const float sc = self ? 1.0f : M::value(acc[bm][bn], j);
-                bool tk = (c < a.n_c) && (q < a.n_q) && (sc >= tau) && !(self && a.skip_self);
-                if (tk && key_bound) tk = make_key(sc, c, self) >= tkey;
-                take |= (tk ? 1u : 0u) << j;
-              }
-            }
-          }
-          if (take) {
-            int pos = atomicAdd(&cnt[ql], __popc(take));
-            uint64_t* dst = a.bufs + ((int64_t)split * a.n_q + q) * a.caps;
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-              if (take & (1u << j)) {
-                const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
-                const bool self = (c == q);
-                if (pos < a.caps) dst[pos] = make_key(self ? 1.0f : M::value(acc[bm][bn], j), c, self);
-                pos++;
-              }
-            }
-          }
-        }
-        if (mc) {  // candidate side: the pair seen from row c (never self here)
-          uint32_t takec;
-          if (c_fast) {
-            takec = mc;  // rows past the end have infinite bounds
-          } else {
-            takec = 0;
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-              if (mc & (1u << j)) {
-                const int cr = cb - c0 + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
-                const float sc = M::value(acc[bm][bn], j);
-                bool tk = (c0 + cr < a.n_c) && (sc >= ctau_l[cr]);
-                if (tk && key_bound) tk = make_key(sc, q, false) >= ckey_l[cr];
-                takec |= (tk ? 1u : 0u) << j;
-              }
-            }
-          }
+      for (int j = 0; j < 16; j++) {
+        m |= (v[j] >= tfast ? 1u : 0u) << j;
+        mc |= (v[j] >= cf[j] ? 1u : 0u) << j;
+      }
+      if (dblk) m = 0xffffu;  // the self pair takes the exact path
+      if (q >= a.n_q) mc = 0;  // a query row past the end is nobody's neighbour
+      if (__ballot((m | mc) != 0) == 0) continue;
+      if (m) {  // query side, as pass 2
+        const float tau = tau_l[ql];
+        uint32_t take;
+        if (!dblk && bound_is_exact<KIND>(tau, key_bound)) {
+          take = m & valid_rows(cb, lane_e, a.n_c);
+        } else {
+          const uint64_t tkey = tkey_l[ql];
+          take = 0;
 #pragma unroll
           for (int j = 0; j < 16; j++) {
-            if (takec & (1u << j)) {
-              const int cr = cb - c0 + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
-              const uint64_t key = make_key(M::value(acc[bm][bn], j), q, false);
-              const int slot = atomicAdd(&scnt[cr], 1);
-              if (slot < SYM_SLOTS) {
-                stage[cr * SYM_SLOTS + slot] = key;
-              } else {
-                const int c = c0 + cr;
-                const int pos = atomicAdd(&a.ccounts[c], 1);
-                if (pos < a.ccap) a.cbufs[(int64_t)c * a.ccap + pos] = key;
-              }
+            if (m & (1u << j)) {
+              const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+              const bool self = (c == q);
+              const float sc = self ? 1.0f : val(j);
+              bool tk = (c < a.n_c) && (q < a.n_q) && (sc >= tau) && !(self && a.skip_self);
+              if (tk && key_bound) tk = make_key(sc, c, self) >= tkey;
+              take |= (tk ? 1u : 0u) << j;
             }
           }
         }
-      });
-    });
+        if (take) {
+          // one append per set bit, lowest first: the wave loops as often as
+          // its fullest lane has bits, not once per value
+          int pos = atomicAdd(&cnt[ql], __popc(take));
+          uint64_t* dst = a.bufs + ((int64_t)split * a.n_q + q) * a.caps;
+          while (take) {
+            const int j = __builtin_ctz(take);
+            take &= take - 1;
+            const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+            const bool self = (c == q);
+            const float sc = pick(j);
+            if (pos < a.caps) dst[pos] = make_key(self ? 1.0f : sc, c, self);
+            pos++;
+          }
+        }
+      }
+      if (mc) {  // candidate side: the pair seen from row c (never self here)
+        uint32_t takec;
+        if (c_fast) {
+          takec = mc;  // rows past the end have infinite bounds
+        } else {
+          takec = 0;
+#pragma unroll
+          for (int j = 0; j < 16; j++) {
+            if (mc & (1u << j)) {
+              const int cr = cb - c0 + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+              const float sc = val(j);
+              bool tk = (c0 + cr < a.n_c) && (sc >= ctau_l[cr]);
+              if (tk && key_bound) tk = make_key(sc, q, false) >= ckey_l[cr];
+              takec |= (tk ? 1u : 0u) << j;
+            }
+          }
+        }
+        while (takec) {
+          const int j = __builtin_ctz(takec);
+          takec &= takec - 1;
+          const int cr = cb - c0 + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+          const uint64_t key = make_key(pick(j), q, false);
+          const int slot = atomicAdd(&scnt[cr], 1);
+          if (slot < SYM_SLOTS) {
+            stage[cr * SYM_SLOTS + slot] = key;
+          } else {
+            const int c = c0 + cr;
+            const int pos = atomicAdd(&a.ccounts[c], 1);
+            if (pos < a.ccap) a.cbufs[(int64_t)c * a.ccap + pos] = key;
+          }
+        }
+      }
+    }
     if (!diag) {
       __syncthreads();  // every wave's staged keys are in
       for (int i = threadIdx.x; i < T; i += En::THREADS) {
