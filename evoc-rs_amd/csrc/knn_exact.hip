@@ -60,6 +60,9 @@ template <int ROWS, int WAVES>
 __device__ __forceinline__ void dma_stage(uint8_t* lds_operand, const uint8_t* __restrict__ base,
                                           int64_t row_bytes, int row0, int nrows, int rstride, int kbyte) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the same wave index as a scalar, for the LDS destination (M0) of each
+  // load: scalar arithmetic rather than a vector value read back per load
+  const int wave_s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int rl = lane >> 2;                 // row within the 16-row group
   static_assert(ROWS % (16 * WAVES) == 0, "stage rows must split evenly over the waves");
   // uniform 64-bit base of the tile's first row; per-lane 32-bit offsets
@@ -75,7 +78,7 @@ __device__ __forceinline__ void dma_stage(uint8_t* lds_operand, const uint8_t* _
     const int r = min(grp * 16 + rl, last);
     const uint32_t off = (uint32_t)r * pitch + (uint32_t)(kbyte + chunk * 16);
     __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(tile + off),
-                                     (__attribute__((address_space(3))) void*)(lds_operand + grp * 1024), 16, 0,
+                                     (__attribute__((address_space(3))) void*)(lds_operand + (wave_s + WAVES * i) * 1024), 16, 0,
                                      0);
   }
 }
@@ -207,20 +210,18 @@ struct Engine {
     wait_stage(min(MFMA_NS - 1, nst - 1));
     ring_barrier();
     read_frags(ring(0), 0, 0);
-    // MFMAs of one K step from fragment buffer `par`, each followed by one of
-    // the fragment reads issued just before (an MFMA first: its operand wait
-    // then covers the previous step's reads only), then the remaining MFMAs
+    // MFMAs of one K step from fragment buffer `par`: one MFMA, then all the
+    // fragment reads of the next step issued just before (the first MFMA's
+    // operand wait covers the previous step's reads only), then the remaining
+    // MFMAs, which give the reads the whole step to land
     auto mma_step = [&](int par) __attribute__((always_inline)) {
 #pragma unroll
       for (int bm = 0; bm < BM; bm++)
 #pragma unroll
         for (int bn = 0; bn < BN; bn++) M::mma(acc[bm][bn], fa[par][bm], fb[par][bn]);
-#pragma unroll
-      for (int i = 0; i < BM + BN; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
-      }
-      __builtin_amdgcn_sched_group_barrier(0x008, BM * BN, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);        // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x100, BM + BN, 0);  // DS reads
+      __builtin_amdgcn_sched_group_barrier(0x008, BM * BN - 1, 0);
       __builtin_amdgcn_sched_barrier(0);
     };
     for (int s = 0; s < nst; s++) {
