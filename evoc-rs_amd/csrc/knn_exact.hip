@@ -592,7 +592,7 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
       // all-ones / zero masks, no branches.  The masks are opaque so that the
       // tree is not folded back into an indexed load (which would put the
       // block in scratch memory).
-      auto pick = [&](int j) __attribute__((always_inline)) {
+      auto pick_bits = [&](int j) __attribute__((always_inline)) {
         uint32_t m0 = 0u - (uint32_t)(j & 1), m1 = 0u - (uint32_t)((j >> 1) & 1);
         uint32_t m2 = 0u - (uint32_t)((j >> 2) & 1), m3 = 0u - (uint32_t)((j >> 3) & 1);
         asm volatile("" : "+v"(m0), "+v"(m1), "+v"(m2), "+v"(m3));
@@ -606,7 +606,10 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
         for (int i = 0; i < 4; i++) t4[i] = sel(m1, t8[2 * i + 1], t8[2 * i]);
 #pragma unroll
         for (int i = 0; i < 2; i++) t2[i] = sel(m2, t4[2 * i + 1], t4[2 * i]);
-        const float raw = __uint_as_float(sel(m3, t2[1], t2[0]));
+        return sel(m3, t2[1], t2[0]);
+      };
+      auto pick = [&](int j) __attribute__((always_inline)) {
+        const float raw = __uint_as_float(pick_bits(j));
         return KIND == EVOC_ELEM_I8 ? raw : f32_score(raw);
       };
       const int cb = c0 + wm_e * 32 * BM + bm * 32;  // first candidate of the block row
@@ -635,8 +638,44 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
       if (dblk) m = 0xffffu;  // the self pair takes the exact path
       if (q >= a.n_q) mc = 0;  // a query row past the end is nobody's neighbour
       if (__ballot((m | mc) != 0) == 0) continue;
+      const float tau = tau_l[ql];
+      // Common case, decided per wave: off the diagonal, every row's bound
+      // exact (first round, positive for f32 scores) on both sides.  Then a
+      // value that passes is the score itself (positive for f32: its order
+      // key is its bits with the sign bit set) and never a self pair, and one
+      // loop over the union of both sides' bits appends each value once.
+      if (c_fast && __ballot(!bound_is_exact<KIND>(tau, key_bound)) == 0) {
+        const uint32_t take = m & valid_rows(cb, lane_e, a.n_c);
+        int pos = 0;
+        if (take) pos = atomicAdd(&cnt[ql], __popc(take));
+        uint64_t* dst = a.bufs + ((int64_t)split * a.n_q + q) * a.caps;
+        uint32_t all = take | mc;
+        while (all) {
+          const int j = __builtin_ctz(all);
+          all &= all - 1;
+          const uint32_t bits = pick_bits(j);
+          const uint32_t hi = KIND == EVOC_ELEM_I8 ? ord_f32(__uint_as_float(bits)) : (bits | 0x80000000u);
+          const int roff = 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+          if ((take >> j) & 1u) {  // make_key(score, cb + roff, false)
+            if (pos < a.caps) dst[pos] = ((uint64_t)hi << 32) | (uint64_t)(0xfffffffeu - (uint32_t)(cb + roff));
+            pos++;
+          }
+          if ((mc >> j) & 1u) {  // make_key(score, q, false) for row cb + roff
+            const int cr = cb - c0 + roff;
+            const uint64_t key = ((uint64_t)hi << 32) | (uint64_t)(0xfffffffeu - (uint32_t)q);
+            const int slot = atomicAdd(&scnt[cr], 1);
+            if (slot < SYM_SLOTS) {
+              stage[cr * SYM_SLOTS + slot] = key;
+            } else {
+              const int c = c0 + cr;
+              const int cp = atomicAdd(&a.ccounts[c], 1);
+              if (cp < a.ccap) a.cbufs[(int64_t)c * a.ccap + cp] = key;
+            }
+          }
+        }
+        continue;
+      }
       if (m) {  // query side, as pass 2
-        const float tau = tau_l[ql];
         uint32_t take;
         if (!dblk && bound_is_exact<KIND>(tau, key_bound)) {
           take = m & valid_rows(cb, lane_e, a.n_c);
