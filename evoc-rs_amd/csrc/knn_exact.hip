@@ -342,14 +342,29 @@ __global__ __launch_bounds__(64 * WM * WN) void pass1_groupmax_kernel(TileArgs a
         constexpr int bn = decltype(bnc)::value;
         __builtin_amdgcn_sched_barrier(0);  // keep this block's accumulator reads here
         const int q = q0 + wn_e * 32 * BN + bn * 32 + (lane_e & 31);
+        const int cb = c0 + wm_e * 32 * BM + bm * 32;  // first candidate of the block
+        // the block's 32 candidates (data rows cb * c_stride ..) and the wave's 32
+        // queries (data rows q - lane % 32 + q_offset ..) may share a row only where
+        // the two ranges meet; elsewhere, with every candidate in range, the group
+        // maximum is the score of the largest raw value (scores are monotone in it)
+        const int64_t qd0 = (int64_t)(q - (lane_e & 31)) + a.q_offset;
+        const bool plain = cb + 32 <= a.n_c && ((int64_t)(cb + 31) * a.c_stride < qd0 ||
+                                                (int64_t)cb * a.c_stride > qd0 + 31);
         float m = -INFINITY;
+        if (plain) {
+          float r = (float)acc[bm][bn][0];
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-          const int c = c0 + wm_e * 32 * BM + bm * 32 + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
-          const bool self = (int64_t)c * a.c_stride == (int64_t)q + a.q_offset;
-          if (c < a.n_c && !(self && a.skip_self)) {
-            const float s = self ? 1.0f : M::value(acc[bm][bn], j);
-            m = fmaxf(m, s);
+          for (int j = 1; j < 16; j++) r = fmaxf(r, (float)acc[bm][bn][j]);
+          m = KIND == EVOC_ELEM_I8 ? r : f32_score(r);
+        } else {
+#pragma unroll
+          for (int j = 0; j < 16; j++) {
+            const int c = cb + 8 * (j >> 2) + 4 * (lane_e >> 5) + (j & 3);
+            const bool self = (int64_t)c * a.c_stride == (int64_t)q + a.q_offset;
+            if (c < a.n_c && !(self && a.skip_self)) {
+              const float s = self ? 1.0f : M::value(acc[bm][bn], j);
+              m = fmaxf(m, s);
+            }
           }
         }
         const int g = ((t * WM + wm_e) * BM + bm) * 2 + (lane_e >> 5);
