@@ -138,6 +138,10 @@ SIGNATURES = {
     "evoc_mi355x_fuzzy_graph": ([_P, C.POINTER(_FuzzyInput), C.POINTER(_FuzzyOutput)], C.c_int),
     "evoc_mi355x_find_duplicates": ([_P, C.c_int32, _P, _P, C.c_int64, C.c_int64, C.c_int32, _P, C.c_int64,
                                      C.POINTER(C.c_int64)], C.c_int),
+    "evoc_npy_read": ([C.c_char_p, _P, C.c_char_p, C.c_int64], C.c_int),
+    "evoc_npz_read": ([C.c_char_p, C.c_char_p, _P, C.c_char_p, C.c_int64], C.c_int),
+    "evoc_npy_write": ([C.c_char_p, _P, C.c_char_p, C.c_int64], C.c_int),
+    "evoc_npy_free": ([_P], None),
     "evoc_numpy_rng_new": ([C.c_uint32], _P),
     "evoc_numpy_rng_free": ([_P], None),
     "evoc_numpy_rng_randint": ([_P, C.c_int64, C.c_int64], C.c_int64),
@@ -727,3 +731,66 @@ def find_duplicates(knn_inds, knn_dists, ctx: Optional[Context] = None):
         _raise(rc, ctx.handle)
     p = pairs[: cnt.value]
     return {(int(a), int(b)) for a, b in zip(p >> np.uint64(32), p & np.uint64(0xFFFFFFFF))}
+
+
+# ---------------------------------------------------------------------------
+# .npy / .npz arrays through the library's own reader and writer
+# (include/evoc_mi355x.h: evoc_npy_read / evoc_npz_read / evoc_npy_write)
+# ---------------------------------------------------------------------------
+class _NpyArray(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("ndim", C.c_int32), ("shape", C.c_int64 * 8),
+                ("data", C.c_void_p), ("nbytes", C.c_uint64)]
+
+
+_NPY_TYPES = [np.bool_, np.int8, np.uint8, np.int32, np.uint32, np.int64, np.uint64, np.float32,
+              np.float64]
+
+
+def _npy_take(a: "_NpyArray") -> np.ndarray:
+    try:
+        shape = tuple(a.shape[i] for i in range(a.ndim))
+        out = np.empty(shape, _NPY_TYPES[a.dtype])
+        if a.nbytes:
+            C.memmove(out.ctypes.data, a.data, a.nbytes)
+        return out
+    finally:
+        lib().evoc_npy_free(C.byref(a))
+
+
+def npy_read(path: str) -> np.ndarray:
+    """ndarray-npy's ReadNpyExt::read_npy (tests/parity.rs:22-30) in the library."""
+    a, err = _NpyArray(), C.create_string_buffer(512)
+    rc = lib().evoc_npy_read(os.fsencode(path), C.byref(a), err, len(err))
+    if rc != OK:
+        raise BackendError(rc, err.value.decode())
+    return _npy_take(a)
+
+
+def npz_read(path: str, member: str) -> np.ndarray:
+    """ndarray-npy's NpzReader::by_name (src/graph_construction.rs:117-140)."""
+    a, err = _NpyArray(), C.create_string_buffer(512)
+    rc = lib().evoc_npz_read(os.fsencode(path), member.encode(), C.byref(a), err, len(err))
+    if rc != OK:
+        raise BackendError(rc, err.value.decode())
+    return _npy_take(a)
+
+
+def npy_write(path: str, arr) -> None:
+    """Writes a C-order little-endian array as .npy (format 1.0)."""
+    x = np.asarray(arr, order="C")  # (ascontiguousarray would make a 0-d array 1-d)
+    codes = {np.dtype(t): i for i, t in enumerate(_NPY_TYPES)}
+    if x.dtype.newbyteorder("=") not in codes or x.dtype.byteorder == ">":
+        raise BackendError(ERR_UNSUPPORTED, f"dtype {x.dtype} is not supported")
+    if x.ndim > 8:
+        raise BackendError(ERR_UNSUPPORTED, "more than 8 dimensions")
+    a = _NpyArray()
+    a.dtype = codes[x.dtype.newbyteorder("=")]
+    a.ndim = x.ndim
+    for i, d in enumerate(x.shape):
+        a.shape[i] = d
+    a.data = x.ctypes.data if x.nbytes else None
+    a.nbytes = x.nbytes
+    err = C.create_string_buffer(512)
+    rc = lib().evoc_npy_write(os.fsencode(path), C.byref(a), err, len(err))
+    if rc != OK:
+        raise BackendError(rc, err.value.decode())

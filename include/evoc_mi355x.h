@@ -249,6 +249,45 @@ int evoc_mi355x_find_duplicates(evoc_mi355x_ctx* ctx, int32_t location, const in
                                 const float* knn_dists, int64_t n, int64_t k, int32_t out_location,
                                 uint64_t* pairs, int64_t capacity, int64_t* n_pairs);
 
+/* ---- .npy / .npz arrays: the on-disk format of the reference's fixtures, goldens
+ * and bench inputs (ndarray-npy ReadNpyExt / NpzReader::by_name, used by
+ * tests/parity.rs:22-30, benches/evoc_bench.rs:4-15, src/knn.rs:460-472 and
+ * src/graph_construction.rs:117-140).  Host-only; C order,
+ * little-endian scalars; .npz members stored uncompressed (numpy.savez), zip64
+ * included.  Readers allocate `data` (release with evoc_npy_free); errors return
+ * EVOC_ERR_INVALID_ARGUMENT (missing file, malformed or truncated data, unknown
+ * member) or EVOC_ERR_UNSUPPORTED (Fortran order, big-endian or other dtypes,
+ * compressed members, format versions beyond 3.0) or EVOC_ERR_OUT_OF_BUDGET
+ * (the host cannot allocate the array), with a message in err (err_len bytes,
+ * may be NULL). ---- */
+#define EVOC_NPY_MAX_DIMS 8
+typedef enum {
+  EVOC_NPY_BOOL = 0,
+  EVOC_NPY_I8 = 1,
+  EVOC_NPY_U8 = 2,
+  EVOC_NPY_I32 = 3,
+  EVOC_NPY_U32 = 4,
+  EVOC_NPY_I64 = 5,
+  EVOC_NPY_U64 = 6,
+  EVOC_NPY_F32 = 7,
+  EVOC_NPY_F64 = 8
+} evoc_npy_dtype;
+
+typedef struct {
+  int32_t dtype; /* evoc_npy_dtype */
+  int32_t ndim;  /* 0 .. EVOC_NPY_MAX_DIMS */
+  int64_t shape[EVOC_NPY_MAX_DIMS];
+  void* data;      /* C order */
+  uint64_t nbytes; /* product(shape) x element size */
+} evoc_npy_array;
+
+int evoc_npy_read(const char* path, evoc_npy_array* out, char* err, int64_t err_len);
+/* member: the array's name in the archive, with or without ".npy" */
+int evoc_npz_read(const char* path, const char* member, evoc_npy_array* out, char* err, int64_t err_len);
+/* writes format 1.0 (2.0 for headers beyond 64 KiB), header padded to 64 bytes */
+int evoc_npy_write(const char* path, const evoc_npy_array* a, char* err, int64_t err_len);
+void evoc_npy_free(evoc_npy_array* a);
+
 /* ---- NumPy RandomState (src/numpy_rng.rs:12-226), for C / Python callers ---- */
 evoc_numpy_rng* evoc_numpy_rng_new(uint32_t seed);        /* check_random_state(Some(seed)) */
 void evoc_numpy_rng_free(evoc_numpy_rng* rng);
