@@ -53,6 +53,21 @@ def test_goldens(ctx, name, seed, prec_name, prec):
     assert pos == int(np.load(os.path.join(GOLDEN, name, "rng_after_knn_meta.npz"))["pos"])
 
 
+def test_golden_through_native_npy_reader(ctx):
+    """The fixture, its goldens and the RNG-state golden read with the library's
+    own .npy / .npz reader (what a C or Rust harness without NumPy would use),
+    then the same parity rule (tests/parity.rs:34-68)."""
+    d = os.path.join(GOLDEN, "medium_800")
+    x = E.npy_read(os.path.join(d, "data.npy"))
+    ei, ed = E.npy_read(os.path.join(d, "nn_inds.npy")), E.npy_read(os.path.join(d, "nn_dists.npy"))
+    rng = E.NumpyRandomState(42)
+    idx, dist = E.knn_graph(x, E.KnnGraphOptions(n_neighbors=15, deterministic=True), rng, ctx=ctx)
+    assert_knn_parity(idx, dist, ei, ed)
+    key, pos, _, _ = rng.get_state()
+    assert np.array_equal(key, E.npy_read(os.path.join(d, "rng_after_knn_key.npy")))
+    assert pos == int(E.npz_read(os.path.join(d, "rng_after_knn_meta.npz"), "pos"))
+
+
 def test_small_200_default_options(ctx):
     """tests/parity.rs:235-260 (default options).  The reference asserts index equality for
     its own NN-descent; with a different f32 summation order the contract is the tie-aware
