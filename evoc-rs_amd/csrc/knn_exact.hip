@@ -1271,7 +1271,11 @@ __global__ __launch_bounds__(256) void rescore_kernel(const float* __restrict__ 
   // the float4 chunks t, t + 16, ... of the row with fmaf (fixed order), then a
   // fixed xor-butterfly over the 16 lanes.  Queries are taken in `order` when
   // given (a permutation of the slab's queries: only the order of the work
-  // changes, each query's result is its own)
+  // changes, each query's result is its own).  A lane issues all its chunk
+  // loads of a candidate (panels of CH chunks, one panel up to 1024 floats)
+  // before the fmaf chain, holds the query's chunks in registers when the row
+  // is one panel, and takes the query's candidate keys with one load per lane.
+  constexpr int CH = 16;
   const int lane = threadIdx.x & 63;
   const int grp = lane >> 4, t = lane & 15;
   const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -1279,24 +1283,54 @@ __global__ __launch_bounds__(256) void rescore_kernel(const float* __restrict__ 
   const int ql = order ? order[qi] : qi;
   const int q = ql + q_offset;
   const float* xq = X + (int64_t)q * ld;
+  const float4* xq4 = (const float4*)xq;
   const int nv4 = d >> 2;  // full float4 chunks; ld is a multiple of 32 and rows are zero padded
   const int tail = d & 3;
+  const bool one_panel = nv4 <= 16 * CH;
+  float4 qa[CH];
+#pragma unroll
+  for (int u = 0; u < CH; u++) {
+    const int j = t + 16 * u;
+    qa[u] = (one_panel && j < nv4) ? xq4[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   uint64_t v[E];
 #pragma unroll
   for (int e = 0; e < E; e++) v[e] = EVOC_KEY_NONE;
+  // candidate keys: key i in register i / 64 of lane i % 64
+  uint64_t ck[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int i = e * 64 + lane;
+    ck[e] = i < Kc ? cand_keys[(int64_t)ql * Kc + i] : EVOC_KEY_NONE;
+  }
   for (int i0 = 0; i0 < Kc; i0 += 4) {
     const int i = i0 + grp;
-    const uint64_t key = i < Kc ? cand_keys[(int64_t)ql * Kc + i] : EVOC_KEY_NONE;
+    // (the broadcast runs in every lane: a shuffle from a lane switched off
+    // by a branch would read nothing defined)
+    const uint64_t kb = wave_elem<E>(ck, min(i, Kc - 1));
+    const uint64_t key = i < Kc ? kb : EVOC_KEY_NONE;
     const int c = key != EVOC_KEY_NONE ? key_index(key, q) : q;
     const float* xc = X + (int64_t)c * ld;
+    const float4* xc4 = (const float4*)xc;
     float part = 0.0f;
-    for (int j = t; j < nv4; j += 16) {
-      const float4 a4 = *(const float4*)(xq + 4 * j);
-      const float4 b4 = *(const float4*)(xc + 4 * j);
-      part = fmaf(a4.x, b4.x, part);
-      part = fmaf(a4.y, b4.y, part);
-      part = fmaf(a4.z, b4.z, part);
-      part = fmaf(a4.w, b4.w, part);
+    for (int p0 = 0; p0 < nv4; p0 += 16 * CH) {
+      float4 b[CH];
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int j = p0 + t + 16 * u;
+        b[u] = j < nv4 ? xc4[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int j = p0 + t + 16 * u;
+        if (j < nv4) {
+          const float4 a4 = one_panel ? qa[u] : xq4[j];
+          part = fmaf(a4.x, b[u].x, part);
+          part = fmaf(a4.y, b[u].y, part);
+          part = fmaf(a4.z, b[u].z, part);
+          part = fmaf(a4.w, b[u].w, part);
+        }
+      }
     }
     if (tail && t == 0)
       for (int r = 4 * nv4; r < d; r++) part = fmaf(xq[r], xc[r], part);
