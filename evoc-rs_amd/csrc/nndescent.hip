@@ -124,7 +124,7 @@ __device__ __forceinline__ void load_item(const JoinArgs& a, int item, JoinShare
 }
 
 template <int KIND>
-__global__ __launch_bounds__(256) void nnd_join_mfma_kernel(JoinArgs a) {
+__global__ __launch_bounds__(256, 2) void nnd_join_mfma_kernel(JoinArgs a) {
   using M = Mfma<KIND>;
   __shared__ JoinShared shm[4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -155,36 +155,60 @@ __global__ __launch_bounds__(256) void nnd_join_mfma_kernel(JoinArgs a) {
   for (int bm = 0; bm < 2; bm++)
 #pragma unroll
     for (int bn = 0; bn < 4; bn++) acc[bm][bn] = M::zero();
+  // Fragments straight from HBM, double-buffered in registers: the next K
+  // step's loads are in flight while this step's MFMAs run.  Every block is
+  // multiplied (blocks past nN / nC read a valid row and are masked below),
+  // and the step count is even (rows are padded to 128 bytes), so the two
+  // buffers alternate with static indices.
   const int ksteps = (int)(a.row_bytes / M::KSTEP_BYTES);
-  for (int ks = 0; ks < ksteps; ks++) {
+  typename M::frag_t fa[2][2], fb[2][4];
+  auto load_step = [&](typename M::frag_t (&ra)[2], typename M::frag_t (&rb)[4], int ks)
+      __attribute__((always_inline)) {
     const int off = ks * M::KSTEP_BYTES;
 #pragma unroll
-    for (int bm = 0; bm < 2; bm++) {
-      if (bm < nbm) {
+    for (int bm = 0; bm < 2; bm++) ra[bm] = M::load(arow[bm] + off);
 #pragma unroll
-        for (int bn = 0; bn < 4; bn++)
-          if (bn < nbn) M::step(acc[bm][bn], arow[bm] + off, brow[bn] + off);
-      }
-    }
+    for (int bn = 0; bn < 4; bn++) rb[bn] = M::load(brow[bn] + off);
+  };
+  auto mma_step = [&](const typename M::frag_t (&ra)[2], const typename M::frag_t (&rb)[4])
+      __attribute__((always_inline)) {
+#pragma unroll
+    for (int bm = 0; bm < 2; bm++)
+#pragma unroll
+      for (int bn = 0; bn < 4; bn++) M::mma(acc[bm][bn], ra[bm], rb[bn]);
+  };
+  load_step(fa[0], fb[0], 0);
+  for (int ks = 0; ks < ksteps; ks += 2) {
+    load_step(fa[1], fb[1], ks + 1);
+    mma_step(fa[0], fb[0]);
+    if (ks + 2 < ksteps) load_step(fa[0], fb[0], ks + 2);
+    mma_step(fa[1], fb[1]);
   }
+  // Offers, one 32 x 32 block at a time (rolled: the block's values are
+  // copied out under a uniform branch, so no accumulator is indexed at run time)
+#pragma unroll 1
+  for (int b = 0; b < 8; b++) {
+    const int bm = b >> 2, bn = b & 3;
+    if (bm >= nbm || bn >= nbn) continue;
+    float v[16];
+    static_for<8>([&](auto bc) __attribute__((always_inline)) {
+      constexpr int cc = decltype(bc)::value;
+      if (b == cc) {
 #pragma unroll
-  for (int bm = 0; bm < 2; bm++) {
-    if (bm >= nbm) continue;
-#pragma unroll
-    for (int bn = 0; bn < 4; bn++) {
-      if (bn >= nbn) continue;
-      const int c = bn * 32 + (lane & 31);
-#pragma unroll
-      for (int j = 0; j < 16; j++) {
-        const int r = bm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
-        if (r >= nN || c >= nC) continue;
-        if (c < nN && r >= c) continue;  // each new-new pair once
-        const int32_t p = sh.nidx[r], q = sh.cidx[c];
-        if (p == q) continue;
-        const float s = M::value(acc[bm][bn], j);
-        if (s > sh.cthr[r]) offer(a, p, q, s);
-        if (s > sh.cthr[c]) offer(a, q, p, s);
+        for (int j = 0; j < 16; j++) v[j] = M::value(acc[cc >> 2][cc & 3], j);
       }
+    });
+    const int c = bn * 32 + (lane & 31);
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const int r = bm * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+      if (r >= nN || c >= nC) continue;
+      if (c < nN && r >= c) continue;  // each new-new pair once
+      const int32_t p = sh.nidx[r], q = sh.cidx[c];
+      if (p == q) continue;
+      const float s = v[j];
+      if (s > sh.cthr[r]) offer(a, p, q, s);
+      if (s > sh.cthr[c]) offer(a, q, p, s);
     }
   }
 }
