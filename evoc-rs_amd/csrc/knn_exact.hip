@@ -1114,9 +1114,11 @@ template <int E, int EK>
 __device__ __forceinline__ void select_bisect(const SelectArgs& a, int q, bool overflow, int total, int nseg,
                                               int kept, int base, const uint64_t* src, uint64_t* sel) {
   const int lane = threadIdx.x & 63;
-  uint64_t v[E];
+  // the segments pick each kept key's address; the loads then go out together
+  // (a load per segment into the same register would wait for the previous one)
+  uint64_t addr[E];
 #pragma unroll
-  for (int e = 0; e < E; e++) v[e] = EVOC_KEY_NONE;
+  for (int e = 0; e < E; e++) addr[e] = 0;
   for (int sg = 0; sg < nseg; sg++) {
     const int c = __builtin_amdgcn_readlane(kept, sg);
     if (c == 0) continue;
@@ -1124,13 +1126,15 @@ __device__ __forceinline__ void select_bisect(const SelectArgs& a, int q, bool o
     // (both halves zero-extended: readlane returns int)
     const uint64_t p = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)src, sg) |
                        ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)src >> 32), sg) << 32);
-    const uint64_t* sp = (const uint64_t*)p;
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int i = e * 64 + lane;
-      if (e * 64 < b0 + c && e * 64 + 64 > b0 && i >= b0 && i < b0 + c) v[e] = sp[i - b0];
+      if (i >= b0 && i < b0 + c) addr[e] = p + (uint64_t)(i - b0) * sizeof(uint64_t);
     }
   }
+  uint64_t v[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) v[e] = addr[e] ? *(const uint64_t*)addr[e] : EVOC_KEY_NONE;
   const uint64_t kth = wave_kth_largest<E>(v, a.K, total);
   // keys above the K-th, in lane order, then the K-th up to K
   int above = 0;
