@@ -73,7 +73,10 @@ int64_t sample_rows() {
   static const int64_t v = [] {
     const char* e = std::getenv("EVOC_MI355X_SAMPLE_ROWS");
     const long long r = e ? std::atoll(e) : 0;
-    return r > 0 ? (int64_t)r : (int64_t)16384;
+    // 8192: the threshold pass costs n x sample, the appends it saves about
+    // n^2 K / sample; on 768-d rows 6k-8k rows balance them at 20k-200k rows
+    // (the stride cap below decides the sample beyond ~130k rows)
+    return r > 0 ? (int64_t)r : (int64_t)8192;
   }();
   return v;
 }
