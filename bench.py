@@ -29,7 +29,7 @@ PEAK = {"bf16": 2500.0, "fp32": 157.3, "int8": 5000.0}  # dense TFLOP/s (TOP/s f
 MEASURED_TRAFFIC = {
     (100000, 768, "pass2_append_kernel<bf16>"): (37.3e9, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, "
                                                          "copy-calibrated (profiles/pass2_bf16_100k_768_pmc.md)"),
-    (100000, 768, "pass2_sym_kernel<bf16>"): (15.5e9, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated "
+    (100000, 768, "pass2_sym_kernel<bf16>"): (17.4e9, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated "
                                                       "(profiles/pass2_sym_bf16_100k_768_traffic.md)"),
 }
 
