@@ -354,7 +354,7 @@ def run_million(args, torch, E, kind):
     return res
 
 
-def run_north_star(args, torch, E):
+def run_million_f32(args, torch, E):
     """1,000,000 x 768 f32 (50 Gaussian blobs, std 0.25, Zipf 1.1, L2-normalised),
     cosine k = 30, exact: BF16 MFMA + FP32 re-score of k + 16 against plain FP32
     MFMA; recall@30 against FP64 brute force on 1,000 sampled rows."""
@@ -502,7 +502,7 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counter readings)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus "
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for --gpus "
               f"{args.gpus}). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
@@ -518,7 +518,7 @@ def main():
         print(json.dumps(run_nndescent(args, torch, E)), flush=True)
         return
     if args.workload == "1m":
-        print(json.dumps(run_north_star(args, torch, E)), flush=True)
+        print(json.dumps(run_million_f32(args, torch, E)), flush=True)
         return
 
     n, d, k = args.n, args.d, args.k
