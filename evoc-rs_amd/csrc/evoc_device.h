@@ -120,11 +120,50 @@ __device__ __forceinline__ void wave_sort_desc(T (&v)[E]) {
 // from the top.  Each count is one ballot per register (no data movement).
 // Registers e with 64 e >= n_used are taken to hold only values below every
 // candidate t (0), and are skipped.  Returns 0 when fewer than K values are >= 1.
+// The values at positions below n_used (element i in register i / 64 of lane
+// i % 64) sort above the padding after them.  When at least K of them exist
+// their K-th largest is the result and lies between their smallest and the
+// largest value, so it has the leading bits those two share: the bisection
+// starts below them.  All lanes of the wave must be active.
+template <typename T>
+__device__ __forceinline__ T wave_shfl_xor_bits(T x, int m) {
+  if constexpr (sizeof(T) == 8) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, m, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), m, 64);
+    return ((T)hi << 32) | (T)lo;
+  } else {
+    return (T)(uint32_t)__shfl_xor((int)(uint32_t)x, m, 64);
+  }
+}
+
 template <int E, typename T>
 __device__ __forceinline__ T wave_kth_largest(const T (&v)[E], int K, int n_used) {
+  constexpr int W = (int)sizeof(T) * 8;
+  const int lane = threadIdx.x & 63;
+  T hi = 0, lo = ~(T)0;
+#pragma unroll
+  for (int e = 0; e < E; e++)
+    if (e * 64 + lane < n_used) {
+      hi = v[e] > hi ? v[e] : hi;
+      lo = v[e] < lo ? v[e] : lo;
+    }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const T h = wave_shfl_xor_bits(hi, m), l = wave_shfl_xor_bits(lo, m);
+    hi = h > hi ? h : hi;
+    lo = l < lo ? l : lo;
+  }
   T t = 0;
+  int top = W - 1;
+  if (n_used >= K) {
+    const T diff = hi ^ lo;
+    if (diff == 0) return hi;
+    if constexpr (sizeof(T) == 8) top = 63 - __builtin_clzll((unsigned long long)diff);
+    else top = 31 - __builtin_clz((uint32_t)diff);
+    t = top + 1 >= W ? (T)0 : (T)(hi & (~(T)0 << (top + 1)));
+  }
 #pragma unroll 1
-  for (int b = (int)sizeof(T) * 8 - 1; b >= 0; b--) {
+  for (int b = top; b >= 0; b--) {
     const T cand = t | ((T)1 << b);
     int cnt = 0;
 #pragma unroll
