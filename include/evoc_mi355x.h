@@ -20,7 +20,8 @@
 extern "C" {
 #endif
 
-#define EVOC_MI355X_ABI_VERSION 4
+/* 5: adds evoc_npy_read / evoc_npz_read / evoc_npy_write / evoc_npy_free (struct layouts of 4 unchanged) */
+#define EVOC_MI355X_ABI_VERSION 5
 
 /* EmbeddingData::{Float32, Int8, UInt8}, src/knn.rs:16-20.  UInt8 rows are
  * packed bits compared with bitwise Jaccard; d counts bytes. */
