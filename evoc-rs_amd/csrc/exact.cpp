@@ -67,21 +67,23 @@ struct Plan {
 
 // Candidate rows the threshold pass samples (every stride-th row): a larger
 // sample costs pass 1 more and tightens the bound, so fewer keys are appended
-// and selected.  EVOC_MI355X_SAMPLE_ROWS overrides the default (a tuning knob;
-// results do not depend on it).
-int64_t sample_rows() {
-  static const int64_t v = [] {
+// and selected.  The symmetric pass takes 8192 (the threshold pass costs
+// n x sample, the appends it saves about n^2 K / sample; on 768-d rows 6k-8k
+// rows balance them at 20k-200k rows), the per-split pass of query slabs and
+// serving requests 16384 (with 8192 a serving stream at 60 requests/s fell
+// behind: p50 8.7 ms -> 2.8 s).  The stride cap in make_plan decides the
+// sample beyond ~130k / ~260k rows.  EVOC_MI355X_SAMPLE_ROWS overrides both
+// (a tuning knob; results do not depend on it).
+int64_t sample_rows(bool symmetric) {
+  static const int64_t env = [] {
     const char* e = std::getenv("EVOC_MI355X_SAMPLE_ROWS");
     const long long r = e ? std::atoll(e) : 0;
-    // 8192: the threshold pass costs n x sample, the appends it saves about
-    // n^2 K / sample; on 768-d rows 6k-8k rows balance them at 20k-200k rows
-    // (the stride cap below decides the sample beyond ~130k rows)
-    return r > 0 ? (int64_t)r : (int64_t)8192;
+    return r > 0 ? (int64_t)r : (int64_t)0;
   }();
-  return v;
+  return env ? env : symmetric ? 8192 : 16384;
 }
 
-Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
+Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu, bool symmetric = false) {
   Plan p;
   const int TC = tile_rows_c(kind), TQ = tile_rows_q(kind);
   const int64_t n_ctiles = cdiv(n, TC);
@@ -101,7 +103,7 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu) {
   }
   p.pass1 = true;
   const int gpt = groups_per_tile(kind);
-  int64_t stride = std::min<int64_t>(std::max<int64_t>(cdiv(n, sample_rows()), 1), 16);
+  int64_t stride = std::min<int64_t>(std::max<int64_t>(cdiv(n, sample_rows(symmetric)), 1), 16);
   for (;;) {
     const int64_t nsamp = cdiv(n, stride);
     const int64_t ng = cdiv(nsamp, TC) * gpt;
@@ -153,7 +155,7 @@ SymPlan make_sym_plan(int kind, int64_t n, int K, int n_cu) {
   const int T = tile_rows_c(kind);
   const int64_t nt = cdiv(n, T);
   sp.nqb = nt;
-  sp.p = make_plan(kind, n, n, K, n_cu);
+  sp.p = make_plan(kind, n, n, K, n_cu, true);
   Plan& p = sp.p;
   if (!p.pass1 || tile_rows_q(kind) != T) return sp;
   const int64_t per_cu = tile_lds_bytes(kind, 3) <= 80 * 1024 ? 2 : 1;
