@@ -122,6 +122,31 @@ struct NndStats {
   std::vector<uint32_t> changed;
 };
 
+// Device random-projection forest (nnd_driver.cpp over nndescent.hip's level
+// kernels).  ForestScratch holds one tree's working set; forest_tree builds a
+// tree from a seed and leaves its row permutation and (start, length) leaves in
+// that scratch until the next tree.
+struct ForestScratch {
+  DevBuf perm, perm2, seg_of, side, segs, nsegs, leaves, fcnt, hyper;
+  int max_seg = 0;
+  int64_t hyper_ld = 0;
+  std::vector<int32_t> iota;
+};
+struct ForestTree {
+  const int32_t* perm = nullptr;  // device, n rows in leaf order
+  const int2* leaves = nullptr;   // device, n_leaves (start, length) into perm, in no particular order
+  int n_leaves = 0;
+  int depth = 0;                  // levels split (the root is level 0)
+};
+namespace evoc {
+struct ForestArgs;
+}
+int forest_alloc(Ctx* ctx, ForestScratch& fs, int kind, int64_t n, int dim, int64_t row_bytes, int leaf_size);
+evoc::ForestArgs forest_args(const ForestScratch& fs, int kind, int64_t n, int dim, int leaf_size,
+                             const uint8_t* rows, int64_t row_bytes, const float* frows, int64_t frow_ld);
+int forest_tree(Ctx* ctx, ForestScratch& fs, evoc::ForestArgs fa, int64_t max_depth, uint64_t seed,
+                ForestTree* out);
+
 // NN-descent over all rows (nndescent.hip); out_keys: n x Kg exact-format keys.
 int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, const uint32_t* pop,
                      const float* frows, int64_t frow_ld, int dim, int64_t n, int Kg,
@@ -142,6 +167,15 @@ int64_t exact_slab(int kind, int64_t n, int K, int n_cu, uint64_t avail);
 int64_t time_slab(int kind, int64_t n, int64_t row_bytes, double ms);
 uint64_t exact_scratch_bytes(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, bool minimum,
                              bool serving);
+
+// The plan exact_search would use for these arguments with enough budget for
+// the symmetric pass (a host-only diagnostic; evoc_mi355x_exact_plan).
+struct ExactPlanInfo {
+  int symmetric = 0, n_splits = 0, caps = 0, ccap = 0, stride = 0, n_sample = 0, segments = 0, tile_shape = -1;
+  int64_t work_items = 0;
+  uint64_t scratch_bytes = 0;
+};
+int exact_plan_info(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, bool serving, ExactPlanInfo* out);
 
 // Exact K best candidates per query over all rows (see knn_exact.hip).
 // rows: n x row_bytes scored rows; pop: per-row popcounts (binary only).

@@ -17,7 +17,10 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
+
+#include <sched.h>
 
 #include "evoc_host.h"
 #include "evoc_kernels.h"
@@ -400,9 +403,26 @@ int64_t evoc_knn_effective_k(int64_t n_neighbors, int64_t n) {  // src/knn.rs:40
   return k < 1 ? 1 : k;
 }
 
+// rayon::current_num_threads() of a caller that runs in rayon's global pool:
+// RAYON_NUM_THREADS when set to a positive count, else the CPUs this process
+// may run on (std::thread::available_parallelism, which rayon uses).
+static int64_t host_rayon_threads() {
+  if (const char* e = std::getenv("RAYON_NUM_THREADS")) {
+    const long long v = std::atoll(e);
+    if (v > 0) return (int64_t)v;
+  }
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) {
+    const int c = CPU_COUNT(&set);
+    if (c > 0) return c;
+  }
+  const unsigned h = std::thread::hardware_concurrency();
+  return h > 0 ? (int64_t)h : 1;
+}
+
 static int64_t default_n_trees(const evoc_knn_options* o) {  // src/knn.rs:404-407
   if (o->n_trees > 0) return o->n_trees;
-  int64_t t = o->deterministic ? 1 : (o->rayon_threads > 0 ? o->rayon_threads : 1);
+  const int64_t t = o->deterministic ? 1 : (o->rayon_threads > 0 ? o->rayon_threads : host_rayon_threads());
   return std::min<int64_t>(std::max<int64_t>(t, 4), 8);
 }
 
@@ -539,6 +559,30 @@ extern "C" int evoc_mi355x_request_bytes(const evoc_mi355x_ctx* ctx, const evoc_
   }
   if (full) *full = hi;
   if (minimum) *minimum = lo;
+  return EVOC_OK;
+}
+
+extern "C" int evoc_mi355x_exact_plan(const evoc_knn_input* in, const evoc_knn_options* o, int32_t n_cu,
+                                      int32_t serving, evoc_exact_plan* out) {
+  if (!in || !o || !out || in->n <= 0 || in->d <= 0 || o->n_neighbors <= 0 || n_cu <= 0 ||
+      in->dtype < EVOC_DTYPE_F32 || in->dtype > EVOC_DTYPE_U8)
+    return EVOC_ERR_INVALID_ARGUMENT;
+  const CallShape c = call_shape(in, o, EVOC_MEM_DEVICE, evoc_knn_effective_k(o->n_neighbors, in->n));
+  if (c.K > 1024) return EVOC_ERR_UNSUPPORTED;
+  ExactPlanInfo p;
+  exact_plan_info(c.kind, c.n, c.row_bytes, (int)c.K, n_cu, serving != 0, &p);
+  std::memset(out, 0, sizeof(*out));
+  out->symmetric = p.symmetric;
+  out->n_splits = p.n_splits;
+  out->caps = p.caps;
+  out->ccap = p.ccap;
+  out->stride = p.stride;
+  out->n_sample = p.n_sample;
+  out->segments = p.segments;
+  out->tile_shape = p.tile_shape;
+  out->candidates = (int32_t)c.K;
+  out->work_items = p.work_items;
+  out->scratch_bytes = p.scratch_bytes;
   return EVOC_OK;
 }
 

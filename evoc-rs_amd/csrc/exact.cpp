@@ -19,6 +19,7 @@ namespace {
 
 constexpr int MAX_SORT = 2048;  // largest per-query candidate list one wave sorts
 constexpr int MAX_ROUNDS = 8;
+constexpr int SELECT_MAX_SEGMENTS = 64;  // segments select_topk_kernel merges per query (one per lane)
 
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 int64_t pow2ceil(int64_t v) {
@@ -163,6 +164,10 @@ SymPlan make_sym_plan(int kind, int64_t n, int K, int n_cu) {
   const int64_t pairs = nt * (nt + 1) / 2;
   int64_t tpi = std::min<int64_t>(nt, std::max<int64_t>(1, pairs / (16 * slots)));
   if (nt >= 64 && cdiv(nt, tpi) < 4) tpi = cdiv(nt, 4);
+  // selection merges a row's split buffers and its candidate-side list in
+  // registers only for at most SELECT_MAX_SEGMENTS segments (more go through
+  // the slower streaming kernel), so at most that many minus one splits
+  tpi = std::max<int64_t>(tpi, cdiv(nt, SELECT_MAX_SEGMENTS - 1));
   p.tiles_per_split = (int)tpi;
   p.n_splits = (int)cdiv(nt, tpi);
   // the items resident on one XCD at a time form a block of about group_q query
@@ -440,4 +445,36 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
+}
+
+int exact_plan_info(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, bool serving, ExactPlanInfo* out) {
+  ShapeScope shape_scope(choose_tile_shape(kind, n, row_bytes));
+  const int TQ = tile_rows_q(kind);
+  *out = ExactPlanInfo{};
+  out->tile_shape = tile_shape(kind);
+  if (!serving && symmetric_enabled()) {
+    const SymPlan sp = make_sym_plan(kind, n, K, n_cu);
+    if (!sp.work.empty()) {
+      out->symmetric = 1;
+      out->n_splits = sp.p.n_splits;
+      out->caps = sp.p.caps;
+      out->ccap = sp.ccap;
+      out->stride = sp.p.pass1 ? sp.p.stride : 0;
+      out->n_sample = sp.p.pass1 ? sp.p.n_sample : 0;
+      out->segments = sp.p.n_splits + 1;
+      out->work_items = (int64_t)sp.work.size();
+      out->scratch_bytes = sp.scratch_bytes(n);
+      return 0;
+    }
+  }
+  const int64_t nq = serving ? std::min<int64_t>(n, time_slab(kind, n, row_bytes, 8.0)) : n;
+  const Plan p = make_plan(kind, n, nq, K, n_cu);
+  out->n_splits = p.n_splits;
+  out->caps = p.caps;
+  out->stride = p.pass1 ? p.stride : 0;
+  out->n_sample = p.pass1 ? p.n_sample : 0;
+  out->segments = p.n_splits;
+  out->work_items = cdiv(nq, TQ) * p.n_splits;
+  out->scratch_bytes = p.scratch_bytes(nq, cdiv(nq, TQ));
+  return 0;
 }

@@ -42,6 +42,93 @@ struct Check {
 
 }  // namespace
 
+int forest_alloc(Ctx* ctx, ForestScratch& fs, int kind, int64_t n, int dim, int64_t row_bytes, int leaf_size) {
+  fs.max_seg = (int)(n / (leaf_size + 1) + 2);
+  fs.hyper_ld = kind == EVOC_ELEM_U8BITS ? (row_bytes * 2 + 3) / 4 : dim;
+  int rc;
+  if ((rc = fs.perm.alloc(ctx, (size_t)n * 4, "forest perm")) || (rc = fs.perm2.alloc(ctx, (size_t)n * 4, "forest perm")) ||
+      (rc = fs.seg_of.alloc(ctx, (size_t)n * 4, "forest segments")) ||
+      (rc = fs.side.alloc(ctx, (size_t)n, "forest sides")) ||
+      (rc = fs.segs.alloc(ctx, (size_t)fs.max_seg * 16, "forest segments")) ||
+      (rc = fs.nsegs.alloc(ctx, (size_t)fs.max_seg * 16, "forest segments")) ||
+      (rc = fs.leaves.alloc(ctx, (size_t)(2 * n + 2) * 8, "forest leaves")) ||
+      (rc = fs.fcnt.alloc(ctx, 16, "forest counters")) ||
+      (rc = fs.hyper.alloc(ctx, (size_t)fs.max_seg * fs.hyper_ld * 4, "hyperplanes")))
+    return rc;
+  fs.iota.resize((size_t)n);
+  for (int64_t i = 0; i < n; i++) fs.iota[(size_t)i] = (int32_t)i;
+  return EVOC_OK;
+}
+
+ForestArgs forest_args(const ForestScratch& fs, int kind, int64_t n, int dim, int leaf_size, const uint8_t* rows,
+                       int64_t row_bytes, const float* frows, int64_t frow_ld) {
+  ForestArgs fa = {};
+  fa.kind = kind;
+  fa.n = (int32_t)n;
+  fa.dim = dim;
+  fa.leaf_size = leaf_size;
+  fa.rows = rows;
+  fa.row_bytes = row_bytes;
+  fa.frows = frows;
+  fa.frow_ld = frow_ld;
+  fa.hyper = (float*)fs.hyper.ptr;
+  fa.hyper_ld = fs.hyper_ld;
+  return fa;
+}
+
+// One tree, level by level (float.rs:97-126): every segment longer than
+// leaf_size is split until max_depth levels have run; the leaves are
+// (start, length) ranges of the tree's row permutation.
+int forest_tree(Ctx* ctx, ForestScratch& fs, ForestArgs fa, int64_t max_depth, uint64_t seed, ForestTree* out) {
+  hipStream_t s = ctx->stream;
+  Check ck{ctx};
+  const int n = fa.n;
+  fa.seed = seed;
+  int32_t* d_perm = (int32_t*)fs.perm.ptr;
+  int32_t* d_perm2 = (int32_t*)fs.perm2.ptr;
+  int4* d_segs = (int4*)fs.segs.ptr;
+  int4* d_next = (int4*)fs.nsegs.ptr;
+  int32_t* d_fc = (int32_t*)fs.fcnt.ptr;  // [0] next segments, [1] leaves
+  ck(upload_sync(d_perm, fs.iota.data(), (size_t)n * 4, s), "forest perm");
+  ck(launch_fill_i32(d_fc, 4, 0, s), "forest counters");
+  int nseg;
+  if (n > fa.leaf_size && max_depth > 0) {
+    const int4 root = make_int4(0, n, 0, 0);
+    ck(upload_sync(d_segs, &root, sizeof(root), s), "forest root");
+    nseg = 1;
+  } else {
+    const int2 leaf = make_int2(0, n);
+    const int32_t one = 1;
+    ck(upload_sync(fs.leaves.ptr, &leaf, sizeof(leaf), s), "forest leaf");
+    ck(upload_sync(d_fc + 1, &one, 4, s), "forest leaf");
+    nseg = 0;
+  }
+  int depth = 0;
+  for (int level = 0; nseg > 0 && ck.e == hipSuccess; level++) {
+    const int last = level + 1 >= max_depth ? 1 : 0;
+    ck(launch_fill_i32(d_fc, 1, 0, s), "forest counters");
+    ck(rp_launch_level(d_segs, nseg, d_perm, d_perm2, (int32_t*)fs.seg_of.ptr, (int8_t*)fs.side.ptr, fa, level,
+                       d_next, d_fc, (int2*)fs.leaves.ptr, d_fc + 1, last, s),
+       "forest level");
+    std::swap(d_perm, d_perm2);
+    std::swap(d_segs, d_next);
+    int32_t nn = 0;
+    ck(hipMemcpyAsync(&nn, d_fc, 4, hipMemcpyDeviceToHost, s), "forest counters");
+    ck(hipStreamSynchronize(s), "forest level");
+    nseg = nn;
+    depth = level + 1;
+  }
+  int32_t nl = 0;
+  ck(hipMemcpyAsync(&nl, d_fc + 1, 4, hipMemcpyDeviceToHost, s), "forest leaves");
+  ck(hipStreamSynchronize(s), "forest");
+  if (ck.e != hipSuccess) return ck.fail();
+  out->perm = d_perm;
+  out->leaves = (const int2*)fs.leaves.ptr;
+  out->n_leaves = nl;
+  out->depth = depth;
+  return EVOC_OK;
+}
+
 uint64_t nndescent_bytes(int kind, int64_t n, int Kg, int max_candidates, int64_t leaf_size, int64_t row_bytes,
                          int dim) {
   auto r = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
@@ -110,77 +197,21 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
 
   // ---- random-projection forest: leaves seed the lists (float.rs:243-348) ----
   if (P.n_trees > 0 && n > 1 && ck.e == hipSuccess) {
-    const int max_seg = n / (leaf_size + 1) + 2;
-    DevBuf perm, perm2, seg_of, side, segs, nsegs, leaves, fcnt, hyper;
-    const int64_t hyper_ld = kind == EVOC_ELEM_U8BITS ? (row_bytes * 2 + 3) / 4 : dim;
-    if ((rc = perm.alloc(ctx, (size_t)n * 4, "forest perm")) || (rc = perm2.alloc(ctx, (size_t)n * 4, "forest perm")) ||
-        (rc = seg_of.alloc(ctx, (size_t)n * 4, "forest segments")) ||
-        (rc = side.alloc(ctx, (size_t)n, "forest sides")) ||
-        (rc = segs.alloc(ctx, (size_t)max_seg * 16, "forest segments")) ||
-        (rc = nsegs.alloc(ctx, (size_t)max_seg * 16, "forest segments")) ||
-        (rc = leaves.alloc(ctx, (size_t)(2 * (int64_t)n + 2) * 8, "forest leaves")) ||
-        (rc = fcnt.alloc(ctx, 16, "forest counters")) ||
-        (rc = hyper.alloc(ctx, (size_t)max_seg * hyper_ld * 4, "hyperplanes")))
-      return rc;
-    ForestArgs fa = {};
-    fa.kind = kind;
-    fa.n = n;
-    fa.dim = dim;
-    fa.leaf_size = leaf_size;
-    fa.rows = rows;
-    fa.row_bytes = row_bytes;
-    fa.frows = frows;
-    fa.frow_ld = frow_ld;
-    fa.hyper = (float*)hyper.ptr;
-    fa.hyper_ld = hyper_ld;
-    std::vector<int32_t> iota((size_t)n);
-    for (int i = 0; i < n; i++) iota[(size_t)i] = i;
+    ForestScratch fs;
+    if ((rc = forest_alloc(ctx, fs, kind, n, dim, row_bytes, leaf_size))) return rc;
+    ForestArgs fa = forest_args(fs, kind, n, dim, leaf_size, rows, row_bytes, frows, frow_ld);
     for (int t = 0; t < P.n_trees && ck.e == hipSuccess; t++) {
-      fa.seed = mix(P.tree_seeds[(size_t)t]);
-      int32_t* d_perm = (int32_t*)perm.ptr;
-      int32_t* d_perm2 = (int32_t*)perm2.ptr;
-      int4* d_segs = (int4*)segs.ptr;
-      int4* d_next = (int4*)nsegs.ptr;
-      int32_t* d_fc = (int32_t*)fcnt.ptr;  // [0] next segments, [1] leaves
-      ck(upload_sync(d_perm, iota.data(), (size_t)n * 4, s), "forest perm");
-      ck(launch_fill_i32(d_fc, 4, 0, s), "forest counters");
-      int nseg;
-      if (n > leaf_size) {
-        const int4 root = make_int4(0, n, 0, 0);
-        ck(upload_sync(d_segs, &root, sizeof(root), s), "forest root");
-        nseg = 1;
-      } else {
-        const int2 leaf = make_int2(0, n);
-        const int32_t one = 1;
-        ck(upload_sync(leaves.ptr, &leaf, sizeof(leaf), s), "forest leaf");
-        ck(upload_sync(d_fc + 1, &one, 4, s), "forest leaf");
-        nseg = 0;
-      }
-      for (int level = 0; nseg > 0 && ck.e == hipSuccess; level++) {
-        const int last = level + 1 >= P.max_depth ? 1 : 0;
-        ck(launch_fill_i32(d_fc, 1, 0, s), "forest counters");
-        ck(rp_launch_level(d_segs, nseg, d_perm, d_perm2, (int32_t*)seg_of.ptr, (int8_t*)side.ptr, fa, level, d_next,
-                           d_fc, (int2*)leaves.ptr, d_fc + 1, last, s),
-           "forest level");
-        std::swap(d_perm, d_perm2);
-        std::swap(d_segs, d_next);
-        int32_t nn = 0;
-        ck(hipMemcpyAsync(&nn, d_fc, 4, hipMemcpyDeviceToHost, s), "forest counters");
-        ck(hipStreamSynchronize(s), "forest level");
-        nseg = nn;
-      }
-      int32_t nl = 0;
-      ck(hipMemcpyAsync(&nl, d_fc + 1, 4, hipMemcpyDeviceToHost, s), "forest leaves");
-      ck(hipStreamSynchronize(s), "forest");
+      ForestTree tr;
+      if ((rc = forest_tree(ctx, fs, fa, P.max_depth, mix(P.tree_seeds[(size_t)t]), &tr))) return rc;
       JoinArgs jl = ja;
       jl.mode = 1;
-      jl.n_items = nl;
-      jl.leaves = (const int2*)leaves.ptr;
-      jl.perm = d_perm;
+      jl.n_items = tr.n_leaves;
+      jl.leaves = tr.leaves;
+      jl.perm = tr.perm;
       jl.salt = 1;
-      if (nl > 0) ck(nnd_launch_join(kind, jl, s), "leaf join");
+      if (tr.n_leaves > 0) ck(nnd_launch_join(kind, jl, s), "leaf join");
       merge_all();
-      if (st) st->leaves += nl;
+      if (st) st->leaves += tr.n_leaves;
     }
   }
 

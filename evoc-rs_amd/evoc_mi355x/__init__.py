@@ -90,6 +90,13 @@ class _Output(C.Structure):
                 ("score_pairs", C.c_int64)]
 
 
+class _ExactPlan(C.Structure):
+    _fields_ = [("symmetric", C.c_int32), ("n_splits", C.c_int32), ("caps", C.c_int32), ("ccap", C.c_int32),
+                ("stride", C.c_int32), ("n_sample", C.c_int32), ("segments", C.c_int32),
+                ("tile_shape", C.c_int32), ("candidates", C.c_int32), ("reserved0", C.c_int32),
+                ("work_items", C.c_int64), ("scratch_bytes", C.c_uint64)]
+
+
 class _FuzzyInput(C.Structure):
     _fields_ = [("location", C.c_int32), ("symmetrize", C.c_int32), ("knn_indices", C.c_void_p),
                 ("knn_dists", C.c_void_p), ("n", C.c_int64), ("k", C.c_int64), ("n_neighbors", C.c_float),
@@ -123,6 +130,7 @@ SIGNATURES = {
                                       C.POINTER(_Output)], C.c_int),
     "evoc_mi355x_request_bytes": ([_P, C.POINTER(_Input), C.POINTER(_Options), C.c_int32,
                                    C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+    "evoc_mi355x_exact_plan": ([C.POINTER(_Input), C.POINTER(_Options), C.c_int32, C.c_int32, _P], C.c_int),
     "evoc_mi355x_autotune": ([_P, C.POINTER(_Input), C.POINTER(_Options), C.c_int64, C.POINTER(C.c_int32),
                               C.POINTER(C.c_double)], C.c_int),
     "evoc_mi355x_server_create": ([C.c_int32, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
@@ -251,6 +259,9 @@ class KnnGraphOptions:
     precision: int = PREC_BF16_RESCORE
     margin: int = 16
     strict_precision: bool = False
+    # rayon::current_num_threads() of the caller, for the n_trees default (src/knn.rs:404-407);
+    # 0: the count rayon's global pool has in this process (RAYON_NUM_THREADS, else the CPUs
+    # this process may run on), as for a caller that runs in that pool
     rayon_threads: int = 0
 
     def _c(self) -> _Options:
@@ -282,6 +293,19 @@ def effective_k(n_neighbors: int, n: int) -> int:
 def rng_draws(dtype_code: int, n: int, options: KnnGraphOptions) -> int:
     o = options._c()
     return int(lib().evoc_knn_rng_draws(dtype_code, n, C.byref(o)))
+
+
+def exact_plan(dtype_code: int, n: int, d: int, options: KnnGraphOptions, n_cu: int = 256,
+               serving: bool = False) -> dict:
+    """The plan an exact search of an n x d matrix follows on a device with n_cu compute units
+    (evoc_mi355x_exact_plan; host only, no device touched)."""
+    i = _Input(dtype_code, MEM_DEVICE, None, n, d, d)
+    o = options._c()
+    p = _ExactPlan()
+    rc = lib().evoc_mi355x_exact_plan(C.byref(i), C.byref(o), n_cu, int(serving), C.byref(p))
+    if rc != OK:
+        raise BackendError(rc, "evoc_mi355x_exact_plan")
+    return {name: getattr(p, name) for name, _ in _ExactPlan._fields_ if name != "reserved0"}
 
 
 def _raise(code: int, ctx) -> None:

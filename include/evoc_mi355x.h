@@ -20,8 +20,9 @@
 extern "C" {
 #endif
 
-/* 5: adds evoc_npy_read / evoc_npz_read / evoc_npy_write / evoc_npy_free (struct layouts of 4 unchanged) */
-#define EVOC_MI355X_ABI_VERSION 5
+/* 6: adds evoc_mi355x_exact_plan; rayon_threads = 0 follows the host's thread count
+ * 5: adds evoc_npy_read / evoc_npz_read / evoc_npy_write / evoc_npy_free (struct layouts of 4 unchanged) */
+#define EVOC_MI355X_ABI_VERSION 6
 
 /* EmbeddingData::{Float32, Int8, UInt8}, src/knn.rs:16-20.  UInt8 rows are
  * packed bits compared with bitwise Jaccard; d counts bytes. */
@@ -77,7 +78,9 @@ typedef struct evoc_knn_options {
   int32_t margin;            /* candidates kept beyond k, default 16: the BF16 form re-scores them
                                 in FP32; NN-descent keeps k + margin list entries per vertex */
   int32_t strict_precision;  /* RlxBackend strict_precision; nonzero -> EVOC_ERR_NOT_BITWISE */
-  int32_t rayon_threads;     /* caller's rayon::current_num_threads() for the n_trees default (0 = 1) */
+  int32_t rayon_threads;     /* caller's rayon::current_num_threads() for the n_trees default; 0: the
+                                count rayon's global pool has in this process (RAYON_NUM_THREADS when
+                                set, else the CPUs the process may run on) */
   int32_t reserved0;
 } evoc_knn_options;
 
@@ -155,6 +158,28 @@ int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
 int evoc_mi355x_request_bytes(const evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
                               const evoc_knn_options* opts, int32_t out_location,
                               uint64_t* full, uint64_t* minimum);
+
+/* The plan an exact search (mode EXACT) of `in` and `opts` follows on a device
+ * with n_cu compute units, given budget for the whole matrix in one slab.  Host
+ * only: no device is touched and `in->data` is not read.  No counterpart in the
+ * reference; a diagnostic of the scoring and selection passes. */
+typedef struct evoc_exact_plan {
+  int32_t symmetric;   /* 1: each unordered pair of row tiles is scored once */
+  int32_t n_splits;    /* candidate splits of the scoring pass (one buffer each per query) */
+  int32_t caps;        /* key slots per split and query */
+  int32_t ccap;        /* symmetric pass: candidate-side key slots per row */
+  int32_t stride;      /* threshold pass: every stride-th row is sampled (0: no threshold pass) */
+  int32_t n_sample;    /* threshold pass: rows sampled */
+  int32_t segments;    /* key lists the selection merges per query */
+  int32_t tile_shape;  /* MFMA tile shape (-1: binary rows) */
+  int32_t candidates;  /* K: keys kept per query (k, or k + margin for the BF16 form) */
+  int32_t reserved0;
+  int64_t work_items;  /* workgroup work items of the scoring pass */
+  uint64_t scratch_bytes; /* device scratch of the search (outside the rows and results) */
+} evoc_exact_plan;
+/* serving != 0: the plan of a serving slot (per-split pass over time-bounded query slabs) */
+int evoc_mi355x_exact_plan(const evoc_knn_input* in, const evoc_knn_options* opts, int32_t n_cu,
+                           int32_t serving, evoc_exact_plan* out);
 
 /* Tile-shape autotuner.  No counterpart in the reference.  Times the exact
  * search on the first min(n, max_rows) rows of `in` (max_rows <= 0: all) once

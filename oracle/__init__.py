@@ -88,6 +88,8 @@ def _declare(L):
                                   _I64, C.POINTER(_I64), C.POINTER(_I64), C.c_int]
     L.orc_make_forest.restype = C.POINTER(C.c_int32)
     L.orc_free.argtypes = [_P]
+    L.orc_rp_split.argtypes = [C.c_int, _P, _I64, _I64, _P, _I64, _P, _P, _P]
+    L.orc_rp_split.restype = _I64
     L.orc_nn_descent.argtypes = [C.c_int, _P, _I64, _I64, C.c_int, _I64, _P, _I64, _I64,
                                  C.c_float, C.c_float, C.c_int, _P, _I64, _I64, C.c_int, _I64,
                                  C.c_int, _P, _P, C.POINTER(_I64)]
@@ -192,6 +194,22 @@ def make_forest(data: np.ndarray, n_neighbors: int, n_trees: int, leaf_size: int
     out = np.ctypeslib.as_array(p, shape=(rows.value * cols.value,)).copy().reshape(rows.value, cols.value)
     lib().orc_free(C.cast(p, C.c_void_p))
     return out
+
+
+def rp_split(data: np.ndarray, ind: np.ndarray, state: np.ndarray):
+    """One random-projection split of rows `ind` (float.rs:13-95 / int8.rs:24-123 /
+    uint8.rs:38-111) with tau state `state` (3 int64, advanced in place).
+    Returns (left rows, right rows, (left pivot, right pivot))."""
+    data = np.ascontiguousarray(data)
+    ind = np.ascontiguousarray(ind, np.int32)
+    assert state.dtype == np.int64 and state.shape == (3,) and state.flags.c_contiguous
+    out = np.empty(ind.shape[0], np.int32)
+    piv = np.zeros(2, np.int64)
+    nl = lib().orc_rp_split(_DTYPES[data.dtype], _ptr(data), data.shape[0], data.shape[1], _ptr(ind),
+                            ind.shape[0], _ptr(state), _ptr(out), _ptr(piv))
+    if nl < 0:
+        raise MemoryError("orc_rp_split")
+    return out[:nl].copy(), out[nl:].copy(), (int(piv[0]), int(piv[1]))
 
 
 def nn_descent(data: np.ndarray, k: int, rng_state: np.ndarray, max_candidates: int, n_iters: int,

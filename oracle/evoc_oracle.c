@@ -560,9 +560,10 @@ typedef struct {
     uint8_t *pos, *neg;  /* u8 masks, d bytes */
 } orc_plane;
 
-/* float.rs:13-95 / int8.rs:24-123 / uint8.rs:38-111: returns n_left, fills out (left then right) */
+/* float.rs:13-95 / int8.rs:24-123 / uint8.rs:38-111: returns n_left, fills tl / tr, and the
+ * two pivot rows in piv[0] (left) and piv[1] (right) when piv is not NULL */
 static int64_t rp_split(const orc_data *D, const int32_t *ind, int64_t m, int64_t *st,
-                        orc_plane *pl, int32_t *tl, int32_t *tr, int64_t *n_right)
+                        orc_plane *pl, int32_t *tl, int32_t *tr, int64_t *n_right, int64_t *piv)
 {
     const int64_t dim = D->d;
     uint32_t isz = (uint32_t)m;
@@ -571,6 +572,10 @@ static int64_t rp_split(const orc_data *D, const int32_t *ind, int64_t m, int64_
     ri += (li == ri);
     ri %= isz;
     int64_t left = ind[li], right = ind[ri];
+    if (piv) {
+        piv[0] = left;
+        piv[1] = right;
+    }
     if (D->dtype == ORC_F32) {
         const float *X = (const float *)D->x;
         const float *a = X + left * dim, *b = X + right * dim;
@@ -676,7 +681,7 @@ static int make_tree(const orc_data *D, const int32_t *ind, int64_t m, orc_leave
             return -1;
         }
         int64_t nr = 0;
-        int64_t nl = rp_split(D, ind, m, st, pl, tl, tr, &nr);
+        int64_t nl = rp_split(D, ind, m, st, pl, tl, tr, &nr, NULL);
         int rc = make_tree(D, tl, nl, L, st, leaf_size, max_depth - 1, pl);
         free(tl);
         if (rc == 0)
@@ -781,6 +786,34 @@ int32_t *orc_make_forest(int dtype, const void *x, int64_t n, int64_t d, int64_t
 }
 
 void orc_free(void *p) { free(p); }
+
+/*
+ * One random-projection split as make_tree takes it (float.rs:13-95 / int8.rs:24-123 /
+ * uint8.rs:38-111), for the tests: rows ind[0..m) of the n x d matrix, the tau state st
+ * (advanced in place), left members then right members into out (m entries), the two
+ * pivot rows into piv[2].  Returns the left count, or -1 if scratch cannot be allocated.
+ */
+int64_t orc_rp_split(int dtype, const void *x, int64_t n, int64_t d, const int32_t *ind, int64_t m,
+                     int64_t *st, int32_t *out, int64_t *piv)
+{
+    orc_data D = {dtype, x, n, d, 0};
+    orc_plane pl;
+    pl.hv = (float *)malloc(sizeof(float) * (size_t)(d > 0 ? d : 1));
+    pl.pos = (uint8_t *)malloc((size_t)(d > 0 ? d : 1));
+    pl.neg = (uint8_t *)malloc((size_t)(d > 0 ? d : 1));
+    int32_t *tr = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
+    int64_t nl = -1;
+    if (pl.hv && pl.pos && pl.neg && tr && m >= 2) {
+        int64_t nr = 0;
+        nl = rp_split(&D, ind, m, st, &pl, out, tr, &nr, piv);
+        memcpy(out + nl, tr, sizeof(int32_t) * (size_t)nr);
+    }
+    free(pl.hv);
+    free(pl.pos);
+    free(pl.neg);
+    free(tr);
+    return nl;
+}
 
 /* ------------------------------------------------------------------------- */
 /* NN-descent initialisation                                                  */

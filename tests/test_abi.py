@@ -84,7 +84,7 @@ def test_rng_draws_contract():
     assert E.rng_draws(E.DTYPE_F32, 1000, E.KnnGraphOptions(rayon_threads=6)) == 21
     assert E.rng_draws(E.DTYPE_F32, 1000, E.KnnGraphOptions(rayon_threads=32)) == 27
     assert E.rng_draws(E.DTYPE_F32, 1000, E.KnnGraphOptions(n_trees=2)) == 9
-    bf = E.KnnGraphOptions(brute_force_max_n=5000)
+    bf = E.KnnGraphOptions(brute_force_max_n=5000, rayon_threads=1)
     assert E.rng_draws(E.DTYPE_I8, 1000, bf) == 3
     assert E.rng_draws(E.DTYPE_U8, 1000, bf) == 3
     assert E.rng_draws(E.DTYPE_F32, 1000, bf) == 15  # f32 never takes the brute-force branch
@@ -97,3 +97,35 @@ def test_errors_before_device_use():
     with pytest.raises(E.BackendError) as ei:
         E.Context(0)
     assert ei.value.code == E.ERR_NO_DEVICE
+
+
+def test_rng_draws_default_threads_follow_the_host():
+    """rayon_threads = 0 takes rayon's global-pool size: RAYON_NUM_THREADS, else the CPUs this
+    process may run on (src/knn.rs:404-407 calls rayon::current_num_threads())."""
+    if "RAYON_NUM_THREADS" in os.environ and int(os.environ["RAYON_NUM_THREADS"] or 0) > 0:
+        t = int(os.environ["RAYON_NUM_THREADS"])
+    else:
+        t = len(os.sched_getaffinity(0))
+    assert E.rng_draws(E.DTYPE_F32, 1000, E.KnnGraphOptions()) == 3 + 3 * min(max(t, 4), 8)
+    # an explicit thread count wins
+    assert E.rng_draws(E.DTYPE_F32, 1000, E.KnnGraphOptions(rayon_threads=5)) == 18
+
+
+@pytest.mark.parametrize("dtype,d,prec", [(E.DTYPE_F32, 768, E.PREC_BF16_RESCORE), (E.DTYPE_F32, 768, E.PREC_FP32),
+                                          (E.DTYPE_I8, 512, E.PREC_FP32)])
+def test_symmetric_plan_keeps_selection_in_registers(dtype, d, prec):
+    """The selection merges at most 64 key lists per query in registers; the symmetric plan
+    never asks for more (its splits plus the candidate-side list), at mid sizes where it
+    used to cut the matrix into one split per tile, and at the million-row workloads."""
+    for n in (3000, 9000, 12000, 16383, 16384, 30000, 65536, 100000, 1000000):
+        p = E.exact_plan(dtype, n, d, E.KnnGraphOptions(n_neighbors=30, precision=prec))
+        assert p["segments"] <= 64, (n, p)
+        if p["symmetric"]:
+            assert p["segments"] == p["n_splits"] + 1
+            assert p["work_items"] > 0 and p["ccap"] >= 256
+        if n > 2048:
+            assert p["stride"] >= 1 and p["n_sample"] == -(-n // p["stride"])
+        assert p["candidates"] == (46 if prec == E.PREC_BF16_RESCORE and dtype == E.DTYPE_F32 else 30)
+    # serving slots never take the symmetric pass
+    s = E.exact_plan(E.DTYPE_F32, 100000, 768, E.KnnGraphOptions(n_neighbors=30), serving=True)
+    assert s["symmetric"] == 0 and s["segments"] == s["n_splits"] <= 16

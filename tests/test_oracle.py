@@ -211,11 +211,16 @@ def test_transform_closed_forms():
 
 def test_int8_forest_tracks_float_forest():
     """The i8 split normalises both pivot rows (int8.rs:43-76); on an int8 matrix and its
-    L2-normalised f32 copy the hyperplanes agree, so the forests agree up to rounding."""
+    L2-normalised f32 copy the hyperplanes agree (a margin's sign does not change when the
+    tested row is scaled), so the forests agree up to rounding.  A third of the rows are
+    scaled by 0.3 and a third by 0.08 before quantising: with such different pivot norms a
+    split without the per-pivot normalisation picks other hyperplanes."""
     g = np.random.default_rng(8)
     c = g.standard_normal((8, 64))
     x = c[g.integers(0, 8, 600)] + 0.3 * g.standard_normal((600, 64))
     x /= np.linalg.norm(x, axis=1, keepdims=True)
+    x[200:400] *= 0.3
+    x[400:] *= 0.08
     xi = np.clip(np.round(x * 127), -127, 127).astype(np.int8)
     xf = xi.astype(np.float32)
     xf /= np.linalg.norm(xf, axis=1, keepdims=True)
@@ -327,3 +332,108 @@ def test_error_cases():
     with pytest.raises(ValueError):
         O.knn_graph(np.ones((4, 4), np.float32), O.KnnGraphOptions(n_neighbors=0),
                     O.NumpyRandomState(0))
+
+
+def _tau_mod(state, modulus):
+    """tau_rand_int(state).rem_euclid(modulus) (src/rng.rs:22-26), state advanced in place."""
+    return O.tau_rand_int(state) % modulus
+
+
+def _split_setup(n, m, seed):
+    g = np.random.default_rng(seed)
+    ind = g.permutation(n)[:m].astype(np.int32)
+    state = O.NumpyRandomState(seed).randint3_for_tau()
+    return ind, state
+
+
+def _check_split(ind, state0, left, right, piv, margin, tol):
+    """The split a margin implies: the pivots the tau draws pick (rem_euclid, right index
+    bumped past the left one, src/nndescent/uint8.rs:45-49), rows with margin > tol left and
+    < -tol right, each side in input order.  Returns the rows the check decided."""
+    st = state0.copy()
+    m = ind.shape[0]
+    li = _tau_mod(st, m)
+    ri = _tau_mod(st, m)
+    ri = (ri + (li == ri)) % m
+    assert piv == (int(ind[li]), int(ind[ri]))
+    pos = {int(p): t for t, p in enumerate(ind)}
+    lset, rset = set(left.tolist()), set(right.tolist())
+    assert len(lset) + len(rset) == m and not (lset & rset)
+    assert sorted(pos[p] for p in left.tolist()) == [pos[p] for p in left.tolist()]
+    assert sorted(pos[p] for p in right.tolist()) == [pos[p] for p in right.tolist()]
+    sure = np.abs(margin) > tol
+    exp_left = set(ind[sure & (margin > 0)].tolist())
+    exp_right = set(ind[sure & (margin < 0)].tolist())
+    assert exp_left and exp_right  # both sides non-empty: no random fallback split
+    assert exp_left <= lset and exp_right <= rset
+    return sure
+
+
+def test_uint8_split_margin_closed_form():
+    """uint8.rs:56-82: with positive = (a ^ b) & a and negative = (a ^ b) & b, the margin of
+    row x is popc(x & a & ~b) - popc(x & b & ~a), an exact integer; positive rows go left,
+    negative right, zero margins by a tau draw.  Counting only the positive term (a plausible
+    slip) classifies some of these rows differently, and the test would catch it."""
+    g = np.random.default_rng(30)
+    n, nbytes = 500, 24
+    x = np.packbits(g.random((n, nbytes * 8)) < g.uniform(0.1, 0.6, (n, 1)), axis=1)
+    wrong_disagrees = 0
+    for seed in range(31, 37):
+        ind, st = _split_setup(n, 300, seed)
+        st0 = st.copy()
+        left, right, piv = O.rp_split(x, ind, st)
+        bits = np.unpackbits(x, axis=1).astype(np.int64)
+        a, b = bits[piv[0]], bits[piv[1]]
+        xs = bits[ind]
+        margin = (xs * (a * (1 - b))).sum(1) - (xs * (b * (1 - a))).sum(1)
+        _check_split(ind, st0, left, right, piv, margin.astype(np.float64), 0.5)
+        wrong = (xs * (a * (1 - b))).sum(1)
+        in_left = np.isin(ind, left)
+        wrong_disagrees += int(((wrong > 0) != in_left)[margin != 0].sum())
+    assert wrong_disagrees > 0
+
+
+def test_int8_split_normalises_each_pivot():
+    """int8.rs:43-76: the hyperplane is a/|a| - b/|b| (then unit length), so the margin of
+    row x has the sign of x . (a/|a| - b/|b|).  Rows of very different norms (some scaled by
+    0.3 or 0.08 before quantising): a split on a - b alone (no per-pivot normalisation)
+    classifies some rows differently, and the test would catch it."""
+    g = np.random.default_rng(40)
+    n, d = 600, 96
+    c = g.standard_normal((6, d))
+    xf = c[g.integers(0, 6, n)] + 0.5 * g.standard_normal((n, d))
+    xf /= np.linalg.norm(xf, axis=1, keepdims=True)
+    xf *= g.choice([1.0, 0.3, 0.08], size=(n, 1))
+    x = np.clip(np.round(xf * 127), -127, 127).astype(np.int8)
+    x[::97] = 0  # zero rows: a zero pivot contributes 0 (int8.rs:59-68)
+    wrong_disagrees = 0
+    for seed in range(41, 49):
+        ind, st = _split_setup(n, 400, seed)
+        st0 = st.copy()
+        left, right, piv = O.rp_split(x, ind, st)
+        X = x.astype(np.float64)
+        a, b = X[piv[0]], X[piv[1]]
+        na, nb = np.linalg.norm(a), np.linalg.norm(b)
+        hv = (a / na if na > 0 else 0 * a) - (b / nb if nb > 0 else 0 * b)
+        xs = X[ind]
+        margin = xs @ hv
+        tol = 1e-4 * (np.linalg.norm(xs, axis=1) * np.linalg.norm(hv) + 1e-30)
+        sure = _check_split(ind, st0, left, right, piv, margin, tol)
+        wrong = xs @ (a - b)
+        in_left = np.isin(ind, left)
+        wrong_disagrees += int(((wrong > 0) != in_left)[sure].sum())
+    assert wrong_disagrees > 0
+
+
+def test_float_split_margin_closed_form():
+    """float.rs:13-95: the margin of row x has the sign of x . (a - b)."""
+    x = blobs(500, 48, 6, 0.3, seed=50)
+    for seed in range(51, 55):
+        ind, st = _split_setup(500, 350, seed)
+        st0 = st.copy()
+        left, right, piv = O.rp_split(x, ind, st)
+        X = x.astype(np.float64)
+        hv = X[piv[0]] - X[piv[1]]
+        xs = X[ind]
+        margin = xs @ hv
+        _check_split(ind, st0, left, right, piv, margin, 1e-4 * np.linalg.norm(hv))
