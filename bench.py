@@ -1,11 +1,13 @@
 """kNN-graph benchmark on one MI355X.
 
-Default workload: 100,000 x 768 f32 rows from 50 Gaussian blobs (centres
-N(0, I), std 0.25, Zipf(1.1) blob sizes, L2-normalised), cosine k = 30.  A step
-is one knn_graph call over the HBM-resident matrix with device-resident
-outputs (exact search: BF16 MFMA candidates + FP32 re-score of k + 16).
-Recall@k is checked against an FP64 brute force before timing; the FP32-MFMA
-form is timed beside it.  Prints one JSON line.
+Default workload (the metric's configuration): 1,000,000 x 768 f32 rows from 50
+Gaussian blobs (centres N(0, I), std 0.25, Zipf(1.1) blob sizes, L2-normalised),
+cosine k = 30.  A step is one knn_graph call over the HBM-resident matrix with
+device-resident outputs, exact search: BF16 MFMA candidates + FP32 re-score of
+k + 16, timed beside the plain FP32-MFMA form.  Recall@30 and the distance
+error are checked against FP64 on sampled rows before timing.  The same line
+carries the 100,000 x 768 run of the same generator, the serving stream and the
+CPU baseline.  Prints one JSON line.
 """
 import argparse
 import json
@@ -69,11 +71,8 @@ def cpu_baseline(x_np, k, target_s, n_iters=8, delta=0.0, keep=None):
     stored in keep["indices"] (for recall against the same truth)."""
     import oracle as O
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(16, cores))
+    host = host_cpus()
+    cores = host["threads_used"]
     opts = O.KnnGraphOptions(n_neighbors=k, n_iters=n_iters, delta=delta, delta_improv=delta)
 
     def run(m):
@@ -90,7 +89,31 @@ def cpu_baseline(x_np, k, target_s, n_iters=8, delta=0.0, keep=None):
         m, t = m2, run(m2)
     return {"value": m / t, "unit": "points/s", "cores": cores, "kind": "port",
             "sample": f"first {m} rows of the workload, NN-descent k={k}, {n_iters} iterations (delta={delta}), "
-                      f"{cores} worker threads, {t:.1f} s"}
+                      f"{cores} worker threads, {t:.1f} s",
+            "host": host}
+
+
+def host_cpus():
+    """The host's CPUs as lscpu reports them, the CPUs this process may run on, and the
+    threads the CPU baseline uses: all of those, at most 16 (the host CPU share of one GPU
+    on a shared box)."""
+    import subprocess
+
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("CPU(s)", "Model name", "Thread(s) per core", "Core(s) per socket", "Socket(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return {"lscpu_cpus": int(info["CPU(s)"]) if info.get("CPU(s)", "").isdigit() else None,
+            "model": info.get("Model name"), "affinity_cpus": aff, "threads_used": max(1, min(16, aff))}
 
 
 def serving_bench(torch, E, n_req, rate, slots, budget_gb, seed=42):
@@ -354,42 +377,64 @@ def run_million(args, torch, E, kind):
     return res
 
 
-def run_million_f32(args, torch, E):
-    """1,000,000 x 768 f32 (50 Gaussian blobs, std 0.25, Zipf 1.1, L2-normalised),
-    cosine k = 30, exact: BF16 MFMA + FP32 re-score of k + 16 against plain FP32
-    MFMA; recall@30 against FP64 brute force on 1,000 sampled rows."""
-    from evoc_mi355x import synth
-
-    n = args.n if args.n != 100000 else 1000000
-    d, k = args.d, args.k
-    log(f"[bench] generating {n}x{d} f32 blobs on the GPU")
-    x = synth.blobs_f32_device(n, d, 50, 0.25, seed=42, zipf=1.1)
-    g = torch.Generator(device="cuda")
-    g.manual_seed(5)
-    sample = torch.randperm(n, generator=g, device="cuda")[:1000]
+def fp64_sampled_truth(torch, x, sample, k, chunk=131072):
+    """Exact top-k of the sampled rows by FP64 cosine over all n rows, self first (the
+    layout the output lists).  Returns indices [m, k] int64."""
+    n = x.shape[0]
     xs = x[sample].double()
     best = None
-    for s0 in range(0, n, 131072):
-        e0 = min(n, s0 + 131072)
+    for s0 in range(0, n, chunk):
+        e0 = min(n, s0 + chunk)
         sim = xs @ x[s0:e0].double().T
-        selfm = sample[:, None] == torch.arange(s0, e0, device="cuda")[None, :]
-        sim = torch.where(selfm, torch.full_like(sim, 2.0), sim)  # self first, as the output lists it
+        selfm = sample[:, None] == torch.arange(s0, e0, device=x.device)[None, :]
+        sim = torch.where(selfm, torch.full_like(sim, 2.0), sim)
         v, i = torch.topk(sim, k, dim=1)
         i = i + s0
         if best is None:
             best = (v, i)
         else:
-            vv = torch.cat([best[0], v], 1)
-            ii = torch.cat([best[1], i], 1)
-            top = torch.topk(vv, k, dim=1)
-            best = (top.values, torch.gather(ii, 1, top.indices))
-    truth = best[1]
+            top = torch.topk(torch.cat([best[0], v], 1), k, dim=1)
+            best = (top.values, torch.gather(torch.cat([best[1], i], 1), 1, top.indices))
+        del sim
+    return best[1]
+
+
+def distance_error(torch, x, sample, idx, dist, chunk=2048):
+    """|reported distance - FP64 distance of the same pair| over the sampled rows' listed
+    neighbours: the FP64 distance is max(-log2(dot), 0) of the L2-normalised rows in FP64
+    (src/knn.rs:156-171), 0 for the query itself."""
+    mx, tot, cnt = 0.0, 0.0, 0
+    for s0 in range(0, sample.shape[0], chunk):
+        sm = sample[s0:s0 + chunk]
+        xs = x[sm].double()
+        xs = xs / xs.norm(dim=1, keepdim=True)
+        ids = idx[sm].long()
+        nb = x[ids].double()
+        nb = nb / nb.norm(dim=2, keepdim=True)
+        dots = (nb * xs[:, None, :]).sum(dim=2)
+        ref = torch.where(dots > 0, torch.clamp(-torch.log2(dots.clamp(min=1e-300)), min=0.0),
+                          torch.zeros_like(dots))
+        ref = torch.where(ids == sm[:, None], torch.zeros_like(ref), ref)
+        err = (dist[sm].double() - ref).abs()
+        mx = max(mx, float(err.max().item()))
+        tot += float(err.sum().item())
+        cnt += int(err.numel())
+    return {"max_abs": mx, "mean_abs": tot / max(cnt, 1), "pairs": cnt}
+
+
+def exact_f32_leg(args, torch, E, x, sample, steps, fp32_steps):
+    """Exact search of device rows x, k = args.k: BF16 + FP32 re-score (timed over `steps`
+    calls) and plain FP32 (`fp32_steps` calls, none: skipped); recall@k with and without
+    the self column and the distance error against FP64 on the sampled rows."""
+    n, d = x.shape
+    k = args.k
+    truth = fp64_sampled_truth(torch, x, sample, k)
     ctx = E.Context(0)
     idx = torch.empty((n, k), dtype=torch.int32, device="cuda")
     dist = torch.empty((n, k), dtype=torch.float32, device="cuda")
     out = {}
-    runs = [("bf16", E.PREC_BF16_RESCORE, args.steps)] + ([] if args.no_fp32 else [("fp32", E.PREC_FP32, 1)])
-    for name, prec, steps in runs:
+    runs = [("bf16", E.PREC_BF16_RESCORE, steps)] + ([("fp32", E.PREC_FP32, fp32_steps)] if fp32_steps else [])
+    for name, prec, nsteps in runs:
         opts = E.KnnGraphOptions(n_neighbors=k, precision=prec, margin=args.margin)
 
         def step(st, opts=opts):
@@ -398,19 +443,53 @@ def run_million_f32(args, torch, E):
         step(None)
         torch.cuda.synchronize()
         got = idx[sample].long()
-        rec = float((got.unsqueeze(2) == truth.unsqueeze(1)).any(dim=2).float().mean().item())
-        sec, stats = _timed(torch, step, steps, 0)
+        hit = (got.unsqueeze(2) == truth.unsqueeze(1)).any(dim=2)
+        rec = float(hit.float().mean().item())
+        rec_ns = float(hit[:, 1:].float().mean().item()) if k > 1 else rec
+        derr = distance_error(torch, x, sample, idx, dist)
+        # the checked call above is the first warm-up step
+        sec, stats = _timed(torch, step, nsteps, max(0, args.warmup - 1))
         score_s = float(np.mean([s_.score_seconds for s_ in stats]))
         pairs, kern = _scored(stats, n, name)
         flops = 2.0 * pairs * d
-        out[name] = {"points_per_s": n / sec, "ms_per_step": sec * 1e3, "recall_at_k_sampled": rec,
+        out[name] = {"points_per_s": n / sec, "ms_per_step": sec * 1e3,
+                     "device_ms_per_call": float(np.median([s_.device_seconds for s_ in stats])) * 1e3,
+                     "recall_at_k": rec, "recall_at_k_without_self": rec_ns, "recall_rows": int(sample.shape[0]),
+                     "distance_error_vs_fp64": derr,
                      "kernel": kern, "flop_per_launch": flops,
                      "rescore_rounds": max(s_.rescore_rounds for s_ in stats),
                      "score_tflops": flops / score_s / 1e12,
                      "score_frac": flops / score_s / 1e12 / (PEAK["bf16"] if name == "bf16" else PEAK["fp32"]),
-                     "score_ms": score_s * 1e3, "steps": steps}
-        log(f"[bench] 1M {name}: {out[name]}")
+                     "score_ms": score_s * 1e3, "steps": nsteps}
+        log(f"[bench] {n} rows {name}: {out[name]}")
+    return out, ctx, idx, dist
+
+
+def roofline_of(leg, n, d):
+    traffic, src = MEASURED_TRAFFIC.get((n, d, leg["kernel"]), (None, None))
+    return {"bound": "mfma", "kernel": leg["kernel"], "achieved": leg["score_tflops"], "peak": PEAK["bf16"],
+            "unit": "TFLOP/s", "frac": leg["score_frac"], "traffic": traffic, "traffic_source": src,
+            "algorithmic_flop_per_launch": leg["flop_per_launch"], "avg_launch_ms": leg["score_ms"],
+            "avg_launch_ms_source": "HIP events around the scoring launch on the library's stream"}
+
+
+def run_million_f32(args, torch, E):
+    """1,000,000 x 768 f32 (50 Gaussian blobs, std 0.25, Zipf 1.1, L2-normalised),
+    cosine k = 30, exact: BF16 MFMA + FP32 re-score of k + 16 against plain FP32
+    MFMA; recall@30 and the distance error against FP64 on 1,000 sampled rows."""
+    from evoc_mi355x import synth
+
+    n = args.n
+    d, k = args.d, args.k
+    log(f"[bench] generating {n}x{d} f32 blobs on the GPU")
+    x = synth.blobs_f32_device(n, d, 50, 0.25, seed=42, zipf=1.1)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    sample = torch.randperm(n, generator=g, device="cuda")[:1000]
+    out, ctx, idx, dist = exact_f32_leg(args, torch, E, x, sample, args.steps, 0 if args.no_fp32 else 1)
     ctx.close()
+    del idx, dist, x
+    torch.cuda.empty_cache()
     b = out["bf16"]
     f32 = out.get("fp32")
     return _base_line(args, b["points_per_s"], b["ms_per_step"] / 1e3,
@@ -418,13 +497,34 @@ def run_million_f32(args, torch, E):
                                    f"cosine kNN k={k}, exact BF16-MFMA candidates + FP32 re-score of k+{args.margin}",
                        "n": n, "d": d, "k": k, "method": "exact_bf16_rescore"},
                       "bf16 (fp32 accumulate) + fp32 rescore",
-                      {"recall_at_k": b["recall_at_k_sampled"], "recall_sample_rows": 1000,
-                       "roofline": {"bound": "mfma", "kernel": b["kernel"],
-                                    "achieved": b["score_tflops"], "peak": PEAK["bf16"], "unit": "TFLOP/s",
-                                    "frac": b["score_frac"], "traffic": None,
-                                    "algorithmic_flop_per_launch": b["flop_per_launch"],
-                                    "avg_launch_ms": b["score_ms"]},
+                      {"recall_at_k": b["recall_at_k"], "recall_at_k_without_self": b["recall_at_k_without_self"],
+                       "recall_sample_rows": int(sample.shape[0]),
+                       "distance_error_vs_fp64": b["distance_error_vs_fp64"],
+                       "device_ms_per_call": b["device_ms_per_call"],
+                       "roofline": roofline_of(b, n, d), "bf16": b,
                        "fp32": f32, "bf16_over_fp32": b["points_per_s"] / f32["points_per_s"] if f32 else None})
+
+
+def run_c2_exact(args, torch, E, n=100000):
+    """100,000 x 768 of the same generator through the same exact search (recall over all
+    rows), and the fuzzy graph of its kNN graph against the oracle."""
+    from evoc_mi355x import synth
+
+    d, k = args.d, args.k
+    log(f"[bench] generating {n}x{d} f32 blobs on the GPU")
+    x = synth.blobs_f32_device(n, d, 50, 0.25, seed=42, zipf=1.1)
+    sample = torch.arange(n, device="cuda")
+    out, ctx, idx, dist = exact_f32_leg(args, torch, E, x, sample, max(args.steps, 10), 0 if args.no_fp32 else 3)
+    b = out["bf16"]
+    res = {"workload": f"{n}x{d} f32 (same generator), cosine k={k}, exact BF16 + FP32 re-score of k+{args.margin}",
+           "points_per_s": b["points_per_s"], "ms_per_step": b["ms_per_step"], "recall_at_k": b["recall_at_k"],
+           "roofline": roofline_of(b, n, d), "bf16": b, "fp32": out.get("fp32")}
+    E.knn_graph(x, E.KnnGraphOptions(n_neighbors=k, margin=args.margin), E.NumpyRandomState(42), ctx=ctx,
+                out=(idx, dist))
+    torch.cuda.synchronize()
+    res["fuzzy_graph"] = fuzzy_bench(torch, E, idx, dist, k, ctx, args.steps)
+    ctx.close()
+    return res
 
 
 def run_nndescent(args, torch, E):
@@ -482,17 +582,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=100000)
+    ap.add_argument("--n", type=int, default=None, help="rows (default: the workload's)")
     ap.add_argument("--d", type=int, default=768)
     ap.add_argument("--k", type=int, default=30)
     ap.add_argument("--margin", type=int, default=16)
     ap.add_argument("--no-fp32", action="store_true", help="skip timing the plain FP32 form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="1m: skip the 100k line")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c2-nndescent", "c3", "c4", "1m"],
-                    help="c2: 100k x 768 f32 exact + serving stream (default); c2-nndescent: the same rows "
-                         "through device NN-descent; c3: 1M x 512 int8; c4: 1M x 1024-bit binary; "
-                         "1m: 1M x 768 f32 exact, BF16 + re-score vs FP32")
+    ap.add_argument("--workload", default="1m", choices=["1m", "c2", "c2-nndescent", "c3", "c4"],
+                    help="1m: 1M x 768 f32 exact, BF16 + re-score vs FP32, with the 100k line, serving "
+                         "stream and CPU baseline (default); c2: 100k x 768 f32 exact only; c2-nndescent: "
+                         "100k x 768 through device NN-descent; c3: 1M x 512 int8; c4: 1M x 1024-bit binary")
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="requests in the serving stream (0: skip the serving leg)")
     ap.add_argument("--serving-rate", type=float, default=60.0, help="Poisson arrival rate, requests/s")
@@ -505,11 +606,12 @@ def main():
         print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for --gpus "
               f"{args.gpus}). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.n is None:
+        args.n = 1000000 if args.workload in ("1m", "c3", "c4") else 100000
 
     import torch
 
     import evoc_mi355x as E
-    from evoc_mi355x import synth
 
     if args.workload in ("c3", "c4"):
         print(json.dumps(run_million(args, torch, E, "i8" if args.workload == "c3" else "u8")), flush=True)
@@ -517,94 +619,19 @@ def main():
     if args.workload == "c2-nndescent":
         print(json.dumps(run_nndescent(args, torch, E)), flush=True)
         return
-    if args.workload == "1m":
-        print(json.dumps(run_million_f32(args, torch, E)), flush=True)
+    if args.workload == "c2":
+        r = run_c2_exact(args, torch, E, args.n)
+        b = r["bf16"]
+        res = _base_line(args, b["points_per_s"], b["ms_per_step"] / 1e3,
+                         {"workload": r["workload"], "n": args.n, "d": args.d, "k": args.k,
+                          "method": "exact_bf16_rescore"}, "bf16 (fp32 accumulate) + fp32 rescore", r)
+        print(json.dumps(res), flush=True)
         return
 
-    n, d, k = args.n, args.d, args.k
-    log(f"[bench] generating {n}x{d} blobs")
-    x_np = synth.blobs_f32(n, d, 50, 0.25, seed=42, zipf=1.1)
-    x = torch.from_numpy(x_np).cuda()
-    ctx = E.Context(0)
-    idx = torch.empty((n, k), dtype=torch.int32, device="cuda")
-    dist = torch.empty((n, k), dtype=torch.float32, device="cuda")
-
-    def step(prec, st=None):
-        E.knn_graph(x, E.KnnGraphOptions(n_neighbors=k, precision=prec, margin=args.margin),
-                    E.NumpyRandomState(42), ctx=ctx, stats=st, out=(idx, dist))
-
-    # correctness gate: recall against FP64 before timing
-    log("[bench] fp64 truth")
-    truth = fp64_truth(torch, x, k)
-    step(E.PREC_BF16_RESCORE)
-    torch.cuda.synchronize()
-    rec_bf16 = recall_gpu(torch, idx, truth)
-    log(f"[bench] recall@{k} bf16+rescore = {rec_bf16:.5f}")
-
-    def timed(prec):
-        for _ in range(args.warmup):
-            step(prec)
-        torch.cuda.synchronize()
-        stats = []
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            st = E.CallStats()
-            step(prec, st)
-            stats.append(st)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        return (t1 - t0) / args.steps, stats
-
-    log("[bench] timing bf16+rescore")
-    sec, stats = timed(E.PREC_BF16_RESCORE)
-    lat = sorted(s.seconds for s in stats)
-    score_s = float(np.mean([s.score_seconds for s in stats]))
-    pairs, kern = _scored(stats, n, "bf16")
-    flops = 2.0 * pairs * d
-    achieved = flops / score_s / 1e12
-    result = {
-        "metric": METRIC,
-        "value": n / sec,
-        "unit": "points/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": sec * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "bf16 (fp32 accumulate) + fp32 rescore",
-        "data": "synthetic",
-        "config": {"workload": f"{n}x{d} f32 L2-normalised Gaussian blobs (50 centres, std 0.25, "
-                               f"Zipf 1.1 sizes), cosine kNN k={k}, exact BF16-MFMA candidates + FP32 "
-                               f"re-score of k+{args.margin}",
-                   "n": n, "d": d, "k": k, "method": "exact_bf16_rescore"},
-        "recall_at_k": rec_bf16,
-        "latency_ms": {"p50": lat[len(lat) // 2] * 1e3, "p99": lat[min(len(lat) - 1, int(0.99 * len(lat)))] * 1e3},
-        "roofline": {"bound": "mfma", "kernel": kern, "achieved": achieved,
-                     "peak": PEAK["bf16"], "unit": "TFLOP/s", "frac": achieved / PEAK["bf16"],
-                     "traffic": MEASURED_TRAFFIC.get((n, d, kern), (None, None))[0],
-                     "traffic_source": MEASURED_TRAFFIC.get((n, d, kern), (None, None))[1],
-                     "algorithmic_flop_per_launch": flops, "avg_launch_ms": score_s * 1e3},
-    }
-    # the next step of the pipeline on the same device-resident graph: the fuzzy
-    # union (src/graph_construction.rs:175-226), against the oracle at full size
-    step(E.PREC_BF16_RESCORE)
-    torch.cuda.synchronize()
-    result["fuzzy_graph"] = fuzzy_bench(torch, E, idx, dist, k, ctx, args.steps)
-    if not args.no_fp32:
-        log("[bench] timing fp32")
-        step(E.PREC_FP32)
-        torch.cuda.synchronize()
-        rec_fp32 = recall_gpu(torch, idx, truth)
-        sec32, st32 = timed(E.PREC_FP32)
-        s32 = float(np.mean([s.score_seconds for s in st32]))
-        pairs32, kern32 = _scored(st32, n, "fp32")
-        f32flops = 2.0 * pairs32 * d
-        result["fp32"] = {"points_per_s": n / sec32, "ms_per_step": sec32 * 1e3, "recall_at_k": rec_fp32,
-                          "score_kernel": kern32, "score_kernel_tflops": f32flops / s32 / 1e12,
-                          "score_kernel_frac": f32flops / s32 / 1e12 / PEAK["fp32"]}
-        result["bf16_over_fp32"] = (n / sec) / (n / sec32)
+    result = run_million_f32(args, torch, E)
+    if not args.no_extra:
+        log("[bench] 100k x 768 line")
+        result["c2_100k"] = run_c2_exact(args, torch, E, 100000)
     if args.serving_requests > 0:
         log(f"[bench] serving: {args.serving_requests} requests at {args.serving_rate}/s")
         result["serving"] = serving_bench(torch, E, args.serving_requests, args.serving_rate,
@@ -621,8 +648,13 @@ def main():
         del src, dst
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
-        result["cpu_baseline"] = cpu_baseline(x_np, k, args.cpu_seconds)
-    ctx.close()
+        from evoc_mi355x import synth
+
+        # the first 200k rows of the same workload (blob labels are shuffled, so they
+        # sample every blob)
+        xs = synth.blobs_f32_device(args.n, args.d, 50, 0.25, seed=42, zipf=1.1)[:200000].cpu().numpy()
+        torch.cuda.empty_cache()
+        result["cpu_baseline"] = cpu_baseline(xs, args.k, args.cpu_seconds)
     print(json.dumps(result), flush=True)
 
 

@@ -15,6 +15,7 @@ struct evoc_mi355x_ctx {
   uint64_t in_use = 0;   // bytes currently held
   hipStream_t stream = nullptr;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  hipEvent_t ev_score0 = nullptr, ev_score1 = nullptr;  // around the exact search's scoring launch
   // stream-ordered device memory of this context: frees do not synchronise the
   // device, so contexts on other streams keep running (serving slots)
   hipMemPool_t pool = nullptr;

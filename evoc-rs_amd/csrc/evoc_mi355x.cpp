@@ -148,6 +148,8 @@ evoc_mi355x_ctx::~evoc_mi355x_ctx() {
     (void)hipSetDevice(device);
     if (ev_begin) (void)hipEventDestroy(ev_begin);
     if (ev_end) (void)hipEventDestroy(ev_end);
+    if (ev_score0) (void)hipEventDestroy(ev_score0);
+    if (ev_score1) (void)hipEventDestroy(ev_score1);
     while (!cache.empty()) free_cached(cache.size() - 1);
     if (stream) {
       (void)hipStreamSynchronize(stream);
@@ -347,7 +349,8 @@ int evoc_mi355x_create(int32_t device, uint64_t hbm_budget_bytes, evoc_mi355x_ct
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = total_b = 0;
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)((double)free_b * 0.9);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+      hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
+      hipEventCreate(&c->ev_score0) != hipSuccess || hipEventCreate(&c->ev_score1) != hipSuccess) {
     delete c;
     return EVOC_ERR_HIP;
   }
@@ -586,13 +589,11 @@ extern "C" int evoc_mi355x_exact_plan(const evoc_knn_input* in, const evoc_knn_o
   return EVOC_OK;
 }
 
-extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
-                                            const evoc_knn_options* o, const int64_t tau_state[3],
-                                            const int64_t* tree_states, int64_t n_tree_rows,
-                                            evoc_knn_output* out) {
-  if (!ctx || !in || !o || !out) return EVOC_ERR_INVALID_ARGUMENT;
-  std::lock_guard<std::mutex> lock(ctx->mu);
-  CallScope call(ctx);
+// The whole of knn_graph_seeded once the context is locked: validation, input
+// preparation, search, re-score, transform, output.
+static int knn_graph_run(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const evoc_knn_options* o,
+                         const int64_t tau_state[3], const int64_t* tree_states, int64_t n_tree_rows,
+                         evoc_knn_output* out) {
   const auto t_start = std::chrono::steady_clock::now();
   ctx->last_error.clear();
   if (o->n_neighbors <= 0) return ctx->fail(EVOC_ERR_INVALID_NEIGHBORS, "n_neighbors must be positive");
@@ -614,6 +615,8 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
   if (!out->indices || !out->distances) return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "null output arrays");
   out->k_out = k;
   out->offers_dropped = 0;
+  out->slabs = 0;
+  out->score_pairs = 0;
   out->score_seconds = 0.0;
   out->iterations = 0;
   out->rescore_rounds = 0;
@@ -734,6 +737,7 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
     out->rescore_rounds = xs.extra_rounds;
     out->score_seconds = xs.pass2_ms * 1e-3;
     out->score_pairs = xs.score_pairs;
+    out->slabs = xs.slabs;
   }
 
   const uint64_t* final_keys = (const uint64_t*)keys.ptr;
@@ -792,4 +796,14 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
   out->device_seconds = ms * 1e-3;
   out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   return EVOC_OK;
+}
+
+extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
+                                            const evoc_knn_options* o, const int64_t tau_state[3],
+                                            const int64_t* tree_states, int64_t n_tree_rows,
+                                            evoc_knn_output* out) {
+  if (!ctx || !in || !o || !out || !tau_state) return EVOC_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  CallScope call(ctx);
+  return knn_graph_run(ctx, in, o, tau_state, tree_states, n_tree_rows, out);
 }

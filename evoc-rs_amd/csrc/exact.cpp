@@ -277,9 +277,7 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
     sp = make_sym_plan(kind, n, K, ctx->n_cu);
     sym = !sp.work.empty() && sp.scratch_bytes(n) <= avail;
   }
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
+  hipEvent_t e0 = ctx->ev_score0, e1 = ctx->ev_score1;
   int rc = EVOC_OK;
   int64_t nq = 0;
   for (int64_t q0 = 0; q0 < n && rc == EVOC_OK; q0 += nq) {
@@ -442,8 +440,6 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
     }
     if (stats) stats->slabs++;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return rc;
 }
 

@@ -87,7 +87,7 @@ class _Output(C.Structure):
                 ("seconds", C.c_double), ("device_seconds", C.c_double), ("score_seconds", C.c_double),
                 ("iterations", C.c_int64), ("rescore_rounds", C.c_int64), ("offers_dropped", C.c_int64),
                 ("queue_seconds", C.c_double),
-                ("score_pairs", C.c_int64)]
+                ("score_pairs", C.c_int64), ("slabs", C.c_int64)]
 
 
 class _ExactPlan(C.Structure):
@@ -369,6 +369,7 @@ class CallStats:
     offers_dropped: int = 0
     queue_seconds: float = 0.0
     score_pairs: int = 0
+    slabs: int = 0
     extra: dict = field(default_factory=dict)
 
 
@@ -394,6 +395,7 @@ def _fill_stats(stats, res) -> None:
     stats.offers_dropped = res.offers_dropped
     stats.queue_seconds = res.queue_seconds
     stats.score_pairs = res.score_pairs
+    stats.slabs = res.slabs
 
 
 def _input_from(data, dtype_code=None):

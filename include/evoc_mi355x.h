@@ -20,7 +20,8 @@
 extern "C" {
 #endif
 
-/* 6: adds evoc_mi355x_exact_plan; rayon_threads = 0 follows the host's thread count
+/* 6: adds evoc_mi355x_exact_plan and evoc_knn_output.slabs; rayon_threads = 0 follows the host's
+ *    thread count
  * 5: adds evoc_npy_read / evoc_npz_read / evoc_npy_write / evoc_npy_free (struct layouts of 4 unchanged) */
 #define EVOC_MI355X_ABI_VERSION 6
 
@@ -112,6 +113,7 @@ typedef struct evoc_knn_output {
   double queue_seconds;    /* serving: time from submit until a slot started the request */
   int64_t score_pairs;     /* exact: row pairs the scoring kernel scored (one per unordered pair
                               in the symmetric pass, one per query and candidate otherwise) */
+  int64_t slabs;           /* exact: query slabs the search ran in (1 = the whole matrix at once) */
 } evoc_knn_output;
 
 typedef struct evoc_mi355x_ctx evoc_mi355x_ctx;

@@ -51,17 +51,26 @@ def check_invariants(idx, dist, self_first):
         assert (idx[:, 0] == np.arange(n)).mean() > 0.999
 
 
-@pytest.mark.parametrize("prec_name,prec", [("fp32", E.PREC_FP32), ("bf16", E.PREC_BF16_RESCORE)])
-def test_f32_recall_at_equal_iterations(ctx, prec_name, prec):
+# margin 0: lists exactly k wide, as the reference's heap (make_heap(n, n_neighbors),
+# src/nndescent/float.rs:685); margin 16 (the default): k + 16 wide.  The BF16 form is
+# compared at margin 16 only: its margin is also its FP32 re-score reserve, and with none
+# BF16 rounding decides list membership (measured at margin 0 on these rows: recall 0.971
+# against the reference's 0.998; FP32 at margin 0 matches the reference's precision).
+MARGINS = [16, 0]
+
+
+@pytest.mark.parametrize("prec_name,prec,margin", [("fp32", E.PREC_FP32, 16), ("fp32", E.PREC_FP32, 0),
+                                                   ("bf16", E.PREC_BF16_RESCORE, 16)])
+def test_f32_recall_at_equal_iterations(ctx, prec_name, prec, margin):
     n, d, k, iters = 6000, 64, 15, 6
-    print(f"[gpu] nnd f32 {prec_name} n={n}", flush=True)
+    print(f"[gpu] nnd f32 {prec_name} n={n} margin={margin}", flush=True)
     x = synth.blobs_f32(n, d, 30, 0.3, seed=5)
-    idx, dist, st = nnd(ctx, x, k, iters, prec)
+    idx, dist, st = nnd(ctx, x, k, iters, prec, margin=margin)
     check_invariants(idx, dist, self_first=True)
     truth, _ = O.exact_knn(x, k, acc=O.ACC_F64, workers=8)
     r_gpu = recall(idx, truth)
     r_ref = recall(reference_nnd(x, k, iters), truth)
-    print(f"[gpu] nnd f32 {prec_name}: recall gpu={r_gpu:.5f} reference={r_ref:.5f} "
+    print(f"[gpu] nnd f32 {prec_name} margin={margin}: recall gpu={r_gpu:.5f} reference={r_ref:.5f} "
           f"iters={st.iterations} dropped={st.offers_dropped}", flush=True)
     # the reference also stops early once an iteration changes nothing (float.rs:847)
     assert 1 <= st.iterations <= iters
@@ -75,10 +84,12 @@ def test_f32_recall_at_equal_iterations(ctx, prec_name, prec):
         np.testing.assert_allclose(dist[r], exp, atol=2e-6)
 
 
-def test_int8_recall_at_equal_iterations(ctx):
+@pytest.mark.parametrize("margin", MARGINS)
+def test_int8_recall_at_equal_iterations(ctx, margin):
     n, d, k, iters = 5000, 128, 15, 6
+    print(f"[gpu] nnd int8 margin={margin}", flush=True)
     x = synth.blobs_i8(n, d, 25, 0.25, seed=6)
-    idx, dist, st = nnd(ctx, x, k, iters)
+    idx, dist, st = nnd(ctx, x, k, iters, margin=margin)
     # int8 distances are 1/dot for positive dots and 0 otherwise (src/knn.rs:173-176), so they
     # are not monotone over a whole row; check the indices only
     assert (idx >= 0).all() and all(len(set(r.tolist())) == k for r in idx)
@@ -89,10 +100,12 @@ def test_int8_recall_at_equal_iterations(ctx):
     assert r_gpu >= r_ref
 
 
-def test_binary_recall_at_equal_iterations(ctx):
+@pytest.mark.parametrize("margin", MARGINS)
+def test_binary_recall_at_equal_iterations(ctx, margin):
     n, nbytes, k, iters = 5000, 64, 15, 6
+    print(f"[gpu] nnd binary margin={margin}", flush=True)
     x = synth.bits_u8(n, nbytes * 8, 40, 0.3, 0.08, seed=7)
-    idx, dist, st = nnd(ctx, x, k, iters)
+    idx, dist, st = nnd(ctx, x, k, iters, margin=margin)
     check_invariants(idx, dist, self_first=True)
     truth, _ = O.exact_knn(x, k, workers=8)
     r_gpu = recall(idx, truth)
@@ -138,4 +151,35 @@ def test_nndescent_c2_shape_20k(ctx):
         r = recall(idx, truth)
         print(f"[gpu] nnd c2-20k {name}: recall={r:.5f} reference={r_ref:.5f} "
               f"time={st.seconds*1e3:.1f} ms dropped={st.offers_dropped}", flush=True)
+        assert r >= r_ref
+
+
+def test_nndescent_c2_100k(ctx):
+    """The C2 workload itself: 100,000 x 768 rows (50 blobs, std 0.25, Zipf 1.1), k = 30,
+    8 iterations, FP32 at the reference's list width (margin 0) and FP32 and BF16 + FP32
+    re-score at the default (margin 16): recall@30 against FP64 brute force at least the
+    reference's NN-descent at 8 iterations on the same rows (16 workers, 8 trees)."""
+    torch = pytest.importorskip("torch")
+    n, d, k, iters = 100000, 768, 30, 8
+    print(f"[gpu] nnd c2 100k: data + FP64 truth", flush=True)
+    x = synth.blobs_f32(n, d, 50, 0.25, seed=42, zipf=1.1)
+    xt = torch.from_numpy(x).cuda().double()
+    truth = np.empty((n, k), np.int64)
+    for s in range(0, n, 4096):
+        e = min(n, s + 4096)
+        sim = xt[s:e] @ xt.T
+        sim[torch.arange(e - s), torch.arange(s, e)] = 2.0
+        truth[s:e] = torch.topk(sim, k, dim=1).indices.cpu().numpy()
+        del sim
+    del xt
+    torch.cuda.empty_cache()
+    print(f"[gpu] nnd c2 100k: reference NN-descent (oracle, 16 workers)", flush=True)
+    r_ref = recall(reference_nnd(x, k, iters, threads=16), truth)
+    for name, prec, margin in [("fp32", E.PREC_FP32, 0), ("fp32", E.PREC_FP32, 16),
+                               ("bf16", E.PREC_BF16_RESCORE, 16)]:
+        idx, dist, st = nnd(ctx, x, k, iters, prec, threads=16, margin=margin)
+        r = recall(idx, truth)
+        print(f"[gpu] nnd c2 100k {name} margin={margin}: recall={r:.5f} reference={r_ref:.5f} "
+              f"time={st.seconds*1e3:.1f} ms iters={st.iterations}", flush=True)
+        assert st.iterations == iters
         assert r >= r_ref

@@ -138,3 +138,38 @@ def test_submit_errors():
         t.wait(timeout=30.0)
     assert ei.value.code == E.ERR_NOT_BITWISE
     srv.close()
+
+
+def test_c5_sized_requests_equal_alone_and_in_the_mix():
+    """Requests at the serving workload's sizes (n up to 200,000, d in {384, 768, 1024},
+    k in {10, 15, 30}, BF16 + FP32 re-score) submitted together to four slots: the large
+    ones run in query slabs that shrink while others wait or run, and every result still
+    equals the same request run alone, bit for bit."""
+    torch = pytest.importorskip("torch")
+    specs = [(200000, 1024, 30, 10), (150000, 768, 15, 32), (60000, 1024, 10, 100), (3000, 384, 30, 10),
+             (20000, 1024, 15, 32), (1000, 768, 10, 100), (90000, 384, 30, 32), (7000, 1024, 30, 10)]
+    xs = [synth.blobs_f32_device(n, d, nb, 0.25, seed=300 + i) for i, (n, d, k, nb) in enumerate(specs)]
+    print(f"[gpu] serving c5: {len(specs)} requests alone", flush=True)
+    ctx = E.Context(0)
+    alone = []
+    for (n, d, k, nb), x in zip(specs, xs):
+        out = (torch.empty((n, k), dtype=torch.int32, device="cuda"), torch.empty((n, k), dtype=torch.float32, device="cuda"))
+        E.knn_graph(x, E.KnnGraphOptions(n_neighbors=k), E.NumpyRandomState(5), ctx=ctx, out=out)
+        alone.append(out)
+    ctx.close()
+    print("[gpu] serving c5: same requests in the mix", flush=True)
+    srv = E.Server(0, slots=4)
+    tickets = []
+    for (n, d, k, nb), x in zip(specs, xs):
+        out = (torch.empty((n, k), dtype=torch.int32, device="cuda"), torch.empty((n, k), dtype=torch.float32, device="cuda"))
+        tickets.append((srv.submit(x, E.KnnGraphOptions(n_neighbors=k), E.NumpyRandomState(5), out=out), out))
+    for t, _ in tickets:
+        t.wait(timeout=300.0)
+    srv.close()
+    slabs = [t.stats.slabs for t, _ in tickets]
+    print(f"[gpu] serving c5: slabs per request {slabs}", flush=True)
+    # the 200k x 1024 request is cut into time-bounded slabs, smaller while others run
+    assert slabs[0] >= 8
+    for (a_i, a_d), (t, (m_i, m_d)) in zip(alone, tickets):
+        assert torch.equal(a_i, m_i)
+        assert torch.equal(a_d, m_d)
