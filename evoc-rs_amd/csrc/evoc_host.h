@@ -16,6 +16,12 @@ struct evoc_mi355x_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   hipEvent_t ev_score0 = nullptr, ev_score1 = nullptr;  // around the exact search's scoring launch
+  // Arena mode (evoc_mi355x_knn_graph_noalloc): while `arena` is set, every
+  // DevBuf is carved from [arena, arena + arena_size) by a stack (a released
+  // block returns its bytes when it is the top one), budget is the arena size
+  // and in_use its top; nothing is allocated from or freed to the pool.
+  uint8_t* arena = nullptr;
+  size_t arena_size = 0, arena_top = 0;
   // stream-ordered device memory of this context: frees do not synchronise the
   // device, so contexts on other streams keep running (serving slots)
   hipMemPool_t pool = nullptr;
@@ -71,6 +77,7 @@ struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
   Ctx* owner = nullptr;
+  bool in_arena = false;
   int alloc(Ctx* c, size_t nbytes, const char* what);
   void release();
   DevBuf() = default;
@@ -78,6 +85,14 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf();
 };
+
+// The allocation-free scratch (noalloc.hip): a file-scope device array of
+// EVOC_MI355X_NOALLOC_SCRATCH_BYTES per GPU.  noalloc_scratch_init looks up
+// this GPU's address of it once per device (with the device current);
+// noalloc_scratch returns that address (nullptr if the lookup failed) and the
+// lock that serialises its users on that device within this process.
+int noalloc_scratch_init(int device);
+void* noalloc_scratch(int device, std::mutex** lock);
 
 // Host <-> device copies through the context's pinned staging ring (serving
 // slots); both return when the copy is complete.  staged_download writes

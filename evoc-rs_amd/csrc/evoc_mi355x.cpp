@@ -176,6 +176,17 @@ int Ctx::hip_fail(hipError_t e, const char* what) {
 DevBuf::~DevBuf() { release(); }
 
 void DevBuf::release() {
+  if (ptr && in_arena) {
+    // a stack: the top block gives its bytes back; others wait for the call's end
+    if ((uint8_t*)ptr + bytes == owner->arena + owner->arena_top) {
+      owner->arena_top = (size_t)((uint8_t*)ptr - owner->arena);
+      owner->in_use = owner->arena_top;
+    }
+    ptr = nullptr;
+    bytes = 0;
+    in_arena = false;
+    return;
+  }
   if (ptr) {
     if (owner->serving) {
       (void)hipFreeAsync(ptr, owner->stream);
@@ -205,6 +216,22 @@ int DevBuf::alloc(Ctx* c, size_t nbytes, const char* what) {
   release();
   if (nbytes == 0) nbytes = 256;
   nbytes = (nbytes + 255) & ~(size_t)255;
+  if (c->arena) {
+    if (c->arena_top + nbytes > c->arena_size) {
+      char buf[256];
+      snprintf(buf, sizeof(buf),
+               "allocation-free scratch exhausted allocating %s: %zu bytes requested, %zu of %zu in use",
+               what, nbytes, c->arena_top, c->arena_size);
+      return c->fail(EVOC_ERR_OUT_OF_BUDGET, buf);
+    }
+    ptr = c->arena + c->arena_top;
+    bytes = nbytes;
+    owner = c;
+    in_arena = true;
+    c->arena_top += nbytes;
+    c->in_use = c->arena_top;
+    return EVOC_OK;
+  }
   // a cached block of about this size (at most 1/8 + 1 MiB larger)
   size_t best = SIZE_MAX;
   for (size_t i = 0; i < c->cache.size(); i++) {
@@ -354,6 +381,9 @@ int evoc_mi355x_create(int32_t device, uint64_t hbm_budget_bytes, evoc_mi355x_ct
     delete c;
     return EVOC_ERR_HIP;
   }
+  // the allocation-free entry point's scratch address on this device (looked up
+  // once per device; a failure is reported by that entry point, not here)
+  (void)noalloc_scratch_init(device);
   // private pool; freed blocks stay cached for the next call until destroy
   hipMemPoolProps props = {};
   props.allocType = hipMemAllocationTypePinned;
@@ -590,7 +620,8 @@ extern "C" int evoc_mi355x_exact_plan(const evoc_knn_input* in, const evoc_knn_o
 }
 
 // The whole of knn_graph_seeded once the context is locked: validation, input
-// preparation, search, re-score, transform, output.
+// preparation, search, re-score, transform, output.  Every device buffer is a
+// DevBuf of the context (its pool, or its arena in allocation-free calls).
 static int knn_graph_run(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const evoc_knn_options* o,
                          const int64_t tau_state[3], const int64_t* tree_states, int64_t n_tree_rows,
                          evoc_knn_output* out) {
@@ -805,5 +836,60 @@ extern "C" int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn
   if (!ctx || !in || !o || !out || !tau_state) return EVOC_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(ctx->mu);
   CallScope call(ctx);
+  return knn_graph_run(ctx, in, o, tau_state, tree_states, n_tree_rows, out);
+}
+
+namespace {
+// The context in arena mode for one call: budget = the scratch, nothing taken
+// from or returned to the pool, and the stream drained before the scratch is
+// handed to the next call.
+struct ArenaScope {
+  Ctx* c;
+  uint64_t budget, in_use;
+  ArenaScope(Ctx* ctx, void* base, size_t bytes) : c(ctx), budget(ctx->budget), in_use(ctx->in_use) {
+    c->arena = (uint8_t*)base;
+    c->arena_size = bytes;
+    c->arena_top = 0;
+    c->budget = bytes;
+    c->in_use = 0;
+  }
+  ~ArenaScope() {
+    (void)hipStreamSynchronize(c->stream);
+    c->arena = nullptr;
+    c->arena_size = c->arena_top = 0;
+    c->budget = budget;
+    c->in_use = in_use;
+  }
+};
+}  // namespace
+
+extern "C" int evoc_mi355x_knn_graph_noalloc(evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
+                                             const evoc_knn_options* o, const int64_t tau_state[3],
+                                             const int64_t* tree_states, int64_t n_tree_rows,
+                                             evoc_knn_output* out) {
+  if (!ctx || !in || !o || !out || !tau_state) return EVOC_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->last_error.clear();
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(EVOC_ERR_HIP, "hipSetDevice failed");
+  std::mutex* dev_lock = nullptr;
+  void* base = noalloc_scratch(ctx->device, &dev_lock);
+  if (!base && noalloc_scratch_init(ctx->device) == EVOC_OK) base = noalloc_scratch(ctx->device, &dev_lock);
+  if (!base || !dev_lock)
+    return ctx->fail(EVOC_ERR_HIP, "the allocation-free scratch has no address on this device");
+  // refuse before any device work a call whose smallest working set cannot fit
+  if (o->n_neighbors > 0 && in->n > 0 && in->d > 0 && in->dtype >= EVOC_DTYPE_F32 && in->dtype <= EVOC_DTYPE_U8) {
+    uint64_t full = 0, minimum = 0;
+    if (evoc_mi355x_request_bytes(ctx, in, o, out->location, &full, &minimum) == EVOC_OK &&
+        minimum > EVOC_MI355X_NOALLOC_SCRATCH_BYTES) {
+      char buf[256];
+      snprintf(buf, sizeof(buf),
+               "the allocation-free entry point has %llu bytes of scratch; this call needs at least %llu "
+               "(use evoc_mi355x_knn_graph_seeded)",
+               (unsigned long long)EVOC_MI355X_NOALLOC_SCRATCH_BYTES, (unsigned long long)minimum);
+      return ctx->fail(EVOC_ERR_OUT_OF_BUDGET, buf);
+    }
+  }
+  std::lock_guard<std::mutex> dev_guard(*dev_lock);
+  ArenaScope arena(ctx, base, (size_t)EVOC_MI355X_NOALLOC_SCRATCH_BYTES);
   return knn_graph_run(ctx, in, o, tau_state, tree_states, n_tree_rows, out);
 }

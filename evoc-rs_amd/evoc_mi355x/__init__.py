@@ -128,6 +128,8 @@ SIGNATURES = {
                               C.c_int),
     "evoc_mi355x_knn_graph_seeded": ([_P, C.POINTER(_Input), C.POINTER(_Options), _P, _P, C.c_int64,
                                       C.POINTER(_Output)], C.c_int),
+    "evoc_mi355x_knn_graph_noalloc": ([_P, C.POINTER(_Input), C.POINTER(_Options), _P, _P, C.c_int64,
+                                       C.POINTER(_Output)], C.c_int),
     "evoc_mi355x_request_bytes": ([_P, C.POINTER(_Input), C.POINTER(_Options), C.c_int32,
                                    C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
     "evoc_mi355x_exact_plan": ([C.POINTER(_Input), C.POINTER(_Options), C.c_int32, C.c_int32, _P], C.c_int),
@@ -470,9 +472,15 @@ def knn_graph(data, options: KnnGraphOptions, rng: NumpyRandomState,
     return idx, dist
 
 
+NOALLOC_SCRATCH_BYTES = 256 << 20  # EVOC_MI355X_NOALLOC_SCRATCH_BYTES
+
+
 def knn_graph_seeded(data, options: KnnGraphOptions, tau_state, tree_states=None,
-                     ctx: Optional[Context] = None, stats: Optional[CallStats] = None, out=None):
-    """evoc_mi355x_knn_graph_seeded: the caller made the RNG draws itself."""
+                     ctx: Optional[Context] = None, stats: Optional[CallStats] = None, out=None,
+                     noalloc: bool = False):
+    """evoc_mi355x_knn_graph_seeded: the caller made the RNG draws itself.  With
+    noalloc, evoc_mi355x_knn_graph_noalloc: the same result, every device buffer of
+    the call carved from the library's fixed per-GPU scratch (no device allocation)."""
     ctx = ctx or default_context()
     inp, keep = _input_from(data)
     o = options._c()
@@ -495,7 +503,8 @@ def knn_graph_seeded(data, options: KnnGraphOptions, tau_state, tree_states=None
         res.distances = C.c_void_p(dist.ctypes.data)
     res.capacity_k = k
     _sync_torch(keep, *(out or ()))
-    rc = lib().evoc_mi355x_knn_graph_seeded(
+    fn = lib().evoc_mi355x_knn_graph_noalloc if noalloc else lib().evoc_mi355x_knn_graph_seeded
+    rc = fn(
         ctx.handle, C.byref(inp), C.byref(o), C.c_void_p(tau.ctypes.data),
         C.c_void_p(trees.ctypes.data) if trees is not None else None,
         0 if trees is None else trees.shape[0], C.byref(res))

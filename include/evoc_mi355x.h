@@ -154,6 +154,27 @@ int evoc_mi355x_knn_graph_seeded(evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
                                  const int64_t* tree_states, int64_t n_tree_rows,
                                  evoc_knn_output* out);
 
+/* Allocation-free knn_graph.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It computes exactly what
+ * evoc_mi355x_knn_graph_seeded computes, with every device buffer of the call
+ * carved from one fixed scratch array of EVOC_MI355X_NOALLOC_SCRATCH_BYTES per
+ * GPU (a file-scope device array the GPU holds until the process ends), and it
+ * calls no hipMalloc, hipFree or allocating library routine.  The caller passes
+ * inputs and outputs as for the seeded call (device or host; host ones are
+ * staged through the scratch).  A call whose minimum working set
+ * (evoc_mi355x_request_bytes) exceeds the scratch returns
+ * EVOC_ERR_OUT_OF_BUDGET before any device work; the exact search runs in query
+ * slabs that fit it.  Calls on one GPU share its scratch: within a process they
+ * are serialised by a per-device lock (they would race otherwise), and the
+ * scratch keeps the last call's contents (every buffer a call reads is written
+ * or cleared by that call first). */
+#define EVOC_MI355X_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int evoc_mi355x_knn_graph_noalloc(evoc_mi355x_ctx* ctx, const evoc_knn_input* in,
+                                  const evoc_knn_options* opts, const int64_t tau_state[3],
+                                  const int64_t* tree_states, int64_t n_tree_rows,
+                                  evoc_knn_output* out);
+
 /* Device bytes a call with these arguments holds at its peak (`full`) and the
  * least it can run in (`minimum`: exact search in small query slabs).  No
  * counterpart in the reference; used for admission by HBM budget. */
