@@ -33,6 +33,8 @@ MEASURED_TRAFFIC = {
                                                          "copy-calibrated (profiles/pass2_bf16_100k_768_pmc.md)"),
     (100000, 768, "pass2_sym_kernel<bf16>"): (17.4e9, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated "
                                                       "(profiles/pass2_sym_bf16_100k_768_traffic.md)"),
+    (1000000, 768, "pass2_sym_kernel<bf16>"): (2.61e12, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated "
+                                                        "(profiles/pass2_sym_bf16_1m_768_traffic.md)"),
 }
 
 
