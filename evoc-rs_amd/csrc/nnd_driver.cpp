@@ -285,3 +285,104 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
   if (ck.e != hipSuccess) return ck.fail();
   return EVOC_OK;
 }
+
+// make_forest (src/knn.rs:115-154) on the device: the forest NN-descent mode
+// starts from, returned in the reference's leaf-array layout.
+extern "C" int evoc_mi355x_make_forest(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const evoc_knn_options* o,
+                                       const int64_t* tree_states, int64_t n_trees, int32_t* leaf_array,
+                                       int64_t capacity_rows, int64_t* rows_out, int64_t* cols_out,
+                                       int64_t* tree_leaves, int32_t* depths) {
+  if (!ctx || !in || !o || !rows_out || !cols_out) return EVOC_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  CallScope call(ctx);
+  ctx->last_error.clear();
+  *rows_out = *cols_out = 0;
+  if (n_trees < 0 || (n_trees > 0 && !tree_states)) return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "tree states");
+  if (in->n <= 0) return ctx->fail(EVOC_ERR_EMPTY_DATA, "empty embedding matrix");
+  if (in->n > (int64_t)INT32_MAX - 1) return ctx->fail(EVOC_ERR_UNSUPPORTED, "more than 2^31-2 rows");
+  if (in->dtype < EVOC_DTYPE_F32 || in->dtype > EVOC_DTYPE_U8 || in->d <= 0 || in->ld < in->d || !in->data)
+    return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "bad matrix (dtype, d > 0, ld >= d, data != NULL)");
+  if (o->n_neighbors <= 0) return ctx->fail(EVOC_ERR_INVALID_NEIGHBORS, "n_neighbors must be positive");
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(EVOC_ERR_HIP, "hipSetDevice failed");
+  const int64_t n = in->n, d = in->d;
+  const int64_t k = std::max<int64_t>(1, std::min<int64_t>(o->n_neighbors, n - 1));
+  // leaf_size as knn_graph derives it (src/knn.rs:408-411, float.rs:148-151), at most 64 on the device
+  const int64_t ls_ref = std::max<int64_t>(o->leaf_size > 0 ? o->leaf_size : std::max<int64_t>(10, k),
+                                           std::max<int64_t>(10, k));
+  const int leaf_size = (int)std::min<int64_t>(std::max<int64_t>(ls_ref, 2), 64);
+  if (ls_ref > 64) return ctx->fail(EVOC_ERR_UNSUPPORTED, "the device forest keeps leaves of at most 64 rows");
+  const int64_t max_depth = o->max_rptree_depth;
+  hipStream_t s = ctx->stream;
+  const bool f32 = in->dtype == EVOC_DTYPE_F32;
+  const int kind = f32 ? EVOC_ELEM_F32 : in->dtype == EVOC_DTYPE_I8 ? EVOC_ELEM_I8 : EVOC_ELEM_U8BITS;
+  const int64_t elem = f32 ? 4 : 1;
+  const int64_t row_bytes = ((d * elem + 127) / 128) * 128;
+  const int64_t f32_ld = ((d + 31) / 32) * 32;
+  // the input on the device, then the rows the forest reads (normalised f32 as
+  // prepare_float leaves them, src/knn.rs:95-113; padded i8 / binary rows)
+  DevBuf src, rows, norms, flag;
+  const uint8_t* d_src = (const uint8_t*)in->data;
+  int rc;
+  if (in->location != EVOC_MEM_DEVICE) {
+    const size_t span = (size_t)((n - 1) * in->ld + d) * (size_t)elem;
+    if ((rc = src.alloc(ctx, span, "input copy"))) return rc;
+    if (hipMemcpyAsync(src.ptr, in->data, span, hipMemcpyHostToDevice, s) != hipSuccess)
+      return ctx->fail(EVOC_ERR_HIP, "input upload");
+    d_src = (const uint8_t*)src.ptr;
+  }
+  hipError_t e;
+  if (f32) {
+    if ((rc = rows.alloc(ctx, (size_t)(n * f32_ld * 4), "prepared f32 rows")) ||
+        (rc = norms.alloc(ctx, (size_t)n * 4, "row norms")) || (rc = flag.alloc(ctx, 4, "renorm flag")))
+      return rc;
+    e = launch_prepare_float((const float*)d_src, in->ld, n, (int)d, (float*)rows.ptr, f32_ld, nullptr, 0,
+                             (float*)norms.ptr, (int32_t*)flag.ptr, s);
+  } else {
+    if ((rc = rows.alloc(ctx, (size_t)(n * row_bytes), "scored rows"))) return rc;
+    e = launch_pad_rows(d_src, in->ld, n, (int)d, (uint8_t*)rows.ptr, row_bytes, s);
+  }
+  if (e != hipSuccess) return ctx->hip_fail(e, "input preparation");
+  // each tree: its leaves in the order of their first row in the tree's permutation
+  std::vector<std::vector<int32_t>> tree_rows((size_t)n_trees);  // concatenated leaf members
+  std::vector<std::vector<int2>> tree_lv((size_t)n_trees);
+  int64_t total = 0, width = leaf_size;
+  if (n_trees > 0) {
+    ForestScratch fs;
+    if ((rc = forest_alloc(ctx, fs, kind, n, (int)d, row_bytes, leaf_size))) return rc;
+    ForestArgs fa = forest_args(fs, kind, n, (int)d, leaf_size, (const uint8_t*)rows.ptr, row_bytes,
+                                f32 ? (const float*)rows.ptr : nullptr, f32_ld);
+    for (int64_t t = 0; t < n_trees; t++) {
+      // the tree's seed from its three draws, as NN-descent mode takes it
+      const uint64_t seed = ((uint64_t)tree_states[3 * t] << 40) ^ ((uint64_t)tree_states[3 * t + 1] << 20) ^
+                            (uint64_t)tree_states[3 * t + 2];
+      ForestTree tr;
+      if ((rc = forest_tree(ctx, fs, fa, max_depth, mix(seed), &tr))) return rc;
+      std::vector<int2>& lv = tree_lv[(size_t)t];
+      lv.resize((size_t)tr.n_leaves);
+      tree_rows[(size_t)t].resize((size_t)n);
+      e = hipMemcpyAsync(lv.data(), tr.leaves, lv.size() * sizeof(int2), hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(tree_rows[(size_t)t].data(), tr.perm, (size_t)n * 4, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return ctx->hip_fail(e, "forest download");
+      std::sort(lv.begin(), lv.end(), [](const int2& a, const int2& b) { return a.x < b.x; });
+      for (const int2& l : lv) width = std::max<int64_t>(width, l.y);
+      total += (int64_t)lv.size();
+      if (tree_leaves) tree_leaves[t] = (int64_t)lv.size();
+      if (depths) depths[t] = tr.depth;
+    }
+  }
+  *rows_out = total;
+  *cols_out = width;
+  if (!leaf_array || capacity_rows < total)
+    return leaf_array ? ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "leaf_array holds fewer rows than the forest has leaves")
+                      : EVOC_OK;
+  int64_t r = 0;
+  for (int64_t t = 0; t < n_trees; t++)
+    for (const int2& l : tree_lv[(size_t)t]) {
+      int32_t* dst = leaf_array + r * width;
+      for (int64_t j = 0; j < width; j++) dst[j] = j < l.y ? tree_rows[(size_t)t][(size_t)(l.x + j)] : -1;
+      r++;
+    }
+  return EVOC_OK;
+}

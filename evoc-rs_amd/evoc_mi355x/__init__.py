@@ -133,6 +133,8 @@ SIGNATURES = {
     "evoc_mi355x_request_bytes": ([_P, C.POINTER(_Input), C.POINTER(_Options), C.c_int32,
                                    C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
     "evoc_mi355x_exact_plan": ([C.POINTER(_Input), C.POINTER(_Options), C.c_int32, C.c_int32, _P], C.c_int),
+    "evoc_mi355x_make_forest": ([_P, C.POINTER(_Input), C.POINTER(_Options), _P, C.c_int64, _P, C.c_int64,
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64), _P, _P], C.c_int),
     "evoc_mi355x_autotune": ([_P, C.POINTER(_Input), C.POINTER(_Options), C.c_int64, C.POINTER(C.c_int32),
                               C.POINTER(C.c_double)], C.c_int),
     "evoc_mi355x_server_create": ([C.c_int32, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)], C.c_int),
@@ -515,6 +517,33 @@ def knn_graph_seeded(data, options: KnnGraphOptions, tau_state, tree_states=None
     if out is not None:
         return out
     return idx, dist
+
+
+def make_forest(data, options: KnnGraphOptions, tree_states, ctx: Optional[Context] = None):
+    """The device random-projection forest (evoc_mi355x_make_forest; make_forest,
+    src/knn.rs:115-154): (leaf_array [leaves, cols] int32 padded with -1,
+    leaves per tree [n_trees] int64, depth per tree [n_trees] int32)."""
+    ctx = ctx or default_context()
+    inp, keep = _input_from(data)
+    o = options._c()
+    ts = np.ascontiguousarray(tree_states, np.int64).reshape(-1, 3)
+    nt = ts.shape[0]
+    rows, cols = C.c_int64(), C.c_int64()
+    per_tree = np.zeros(max(nt, 1), np.int64)
+    depths = np.zeros(max(nt, 1), np.int32)
+    _sync_torch(keep)
+    rc = lib().evoc_mi355x_make_forest(ctx.handle, C.byref(inp), C.byref(o), C.c_void_p(ts.ctypes.data), nt, None, 0,
+                                       C.byref(rows), C.byref(cols), None, None)
+    if rc != OK:
+        _raise(rc, ctx.handle)
+    leaf = np.empty((rows.value, cols.value), np.int32)
+    rc = lib().evoc_mi355x_make_forest(ctx.handle, C.byref(inp), C.byref(o), C.c_void_p(ts.ctypes.data), nt,
+                                       C.c_void_p(leaf.ctypes.data), rows.value, C.byref(rows), C.byref(cols),
+                                       C.c_void_p(per_tree.ctypes.data), C.c_void_p(depths.ctypes.data))
+    del keep
+    if rc != OK:
+        _raise(rc, ctx.handle)
+    return leaf, per_tree[:nt], depths[:nt]
 
 
 def request_bytes(data, options: KnnGraphOptions, out_on_device: bool = False,

@@ -20,7 +20,8 @@
 extern "C" {
 #endif
 
-/* 6: adds evoc_mi355x_exact_plan and evoc_knn_output.slabs; rayon_threads = 0 follows the host's
+/* 6: adds evoc_mi355x_exact_plan, evoc_mi355x_make_forest, evoc_mi355x_knn_graph_noalloc and
+ *    evoc_knn_output.slabs; rayon_threads = 0 follows the host's
  *    thread count
  * 5: adds evoc_npy_read / evoc_npz_read / evoc_npy_write / evoc_npy_free (struct layouts of 4 unchanged) */
 #define EVOC_MI355X_ABI_VERSION 6
@@ -203,6 +204,25 @@ typedef struct evoc_exact_plan {
 /* serving != 0: the plan of a serving slot (per-split pass over time-bounded query slabs) */
 int evoc_mi355x_exact_plan(const evoc_knn_input* in, const evoc_knn_options* opts, int32_t n_cu,
                            int32_t serving, evoc_exact_plan* out);
+
+/* make_forest (src/knn.rs:115-154) on the device: the random-projection forest
+ * NN-descent mode starts from, n_trees trees seeded by tree_states (n_trees x 3,
+ * the make_forest draws), leaf size and depth from opts as knn_graph derives them
+ * (leaves of at most 64 rows on the device), rows prepared as knn_graph prepares
+ * them.  Returned in the reference's leaf-array layout: one row per leaf, each
+ * tree's leaves in the order of their first row in the tree's permutation,
+ * padded with -1 to *cols = max(leaf_size, widest leaf) (float.rs:148-151);
+ * *rows = leaves in all trees.  tree_leaves[t] (may be NULL) = leaves of tree t,
+ * depths[t] (may be NULL) = levels split in tree t (at most max_rptree_depth).
+ * With leaf_array NULL only the sizes are returned; with capacity_rows < *rows,
+ * EVOC_ERR_INVALID_ARGUMENT.  The split rule is the reference's (hyperplane
+ * between two random members, int8 pivots normalised, binary margins by
+ * popcount); the random choices come from the device's counter-based hash, not
+ * the reference's tau RNG, so trees differ from the reference's on the same seeds. */
+int evoc_mi355x_make_forest(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const evoc_knn_options* opts,
+                            const int64_t* tree_states, int64_t n_trees, int32_t* leaf_array,
+                            int64_t capacity_rows, int64_t* rows, int64_t* cols, int64_t* tree_leaves,
+                            int32_t* depths);
 
 /* Tile-shape autotuner.  No counterpart in the reference.  Times the exact
  * search on the first min(n, max_rows) rows of `in` (max_rows <= 0: all) once
