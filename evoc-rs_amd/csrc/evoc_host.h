@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include "evoc_mi355x.h"
+
 struct evoc_mi355x_ctx {
   int device = -1;
   int n_cu = 0;
@@ -192,6 +194,47 @@ struct ExactPlanInfo {
   uint64_t scratch_bytes = 0;
 };
 int exact_plan_info(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, bool serving, ExactPlanInfo* out);
+
+// The plan of one per-split slab of all n queries (threshold sample, splits,
+// buffer slots) under the calling thread's tile shape: what exact_search uses
+// for a slab of the whole matrix outside the symmetric pass.
+struct SlabPlan {
+  int n_splits = 1, tiles_per_split = 1, caps = 0;
+  bool pass1 = false;
+  int stride = 1, n_sample = 0, ng = 0, p1_tiles_per_split = 1, p1_splits = 1;
+};
+SlabPlan exact_slab_plan(int kind, int64_t n, int K, int n_cu);
+
+// Batched exact search of small requests (batch.cpp): many requests of one
+// element kind per launch of each pass, from one fixed device workspace; the
+// launch sequence is captured once per kind as a hipGraph and replayed for
+// every batch.
+struct BatchJob {
+  evoc_knn_input in{};
+  evoc_knn_options opts{};
+  evoc_knn_output* out = nullptr;
+  int status = EVOC_ERR_PENDING;
+  bool overflow = false;  // a query's buffers overflowed: run the request alone instead
+  std::string error;
+  size_t bytes = 0;       // workspace bytes (batch_fit)
+  int kind = -1;
+};
+struct BatchEngine;
+// workspace_bytes: the device workspace (held by the engine's context from the
+// first batch on); use_graph: replay captured launch sequences (else launch them)
+BatchEngine* batch_engine_create(int device, size_t workspace_bytes, int max_requests, bool use_graph);
+void batch_engine_destroy(BatchEngine* e);
+// Whether a request can run batched (exact mode, f32 or int8 rows, n <= max_rows,
+// k + margin <= 64) and fits the workspace; sets job->kind and job->bytes.
+bool batch_fit(const BatchEngine* e, BatchJob* job, int64_t max_rows);
+// Runs the jobs (one kind, workspace bytes within the workspace) as one batch
+// on the stream of `run` (the engine's workspace, the caller's stream; one
+// batch at a time per engine).  Each job's status is set; jobs with overflow
+// set need the per-request path.
+int batch_run(BatchEngine* e, Ctx* run, const std::vector<BatchJob*>& jobs, double* device_ms);
+size_t batch_workspace_bytes(const BatchEngine* e);
+int batch_max_requests(const BatchEngine* e);
+int64_t batch_graph_replays(const BatchEngine* e);  // batches run from a captured graph
 
 // Exact K best candidates per query over all rows (see knn_exact.hip).
 // rows: n x row_bytes scored rows; pop: per-row popcounts (binary only).

@@ -74,6 +74,40 @@ struct SelectArgs {
   int32_t pad0;
 };
 
+// One request of a batched exact search (the persistent batch kernels of
+// knn_exact.hip and prep.hip).  Every pointer is into the batch workspace or the
+// caller's device buffers.  The table of requests and the work-item lists live
+// in device memory, so one captured launch sequence serves any batch.
+struct BatchReq {
+  TileArgs p1;            // threshold pass (has_p1)
+  TileArgs p2;            // scoring pass (per-split)
+  SelectArgs sel;         // K best keys per query (sel.out_keys)
+  const uint8_t* x_in;    // caller rows on the device, ld_in elements apart
+  int64_t ld_in;
+  float* xf32;            // f32: prepared rows, f32_ld apart (the scored rows in the FP32 form)
+  int64_t f32_ld;
+  uint8_t* rows;          // BF16 / int8 scored rows, row_bytes apart (FP32 form: == xf32)
+  int64_t row_bytes;
+  float* norms;           // f32: row norms
+  int32_t* flag;          // f32: prepare_float's renormalise-all flag
+  float* gmax;            // threshold pass: [n][ng] group maxima
+  float* tau;             // threshold pass: [n] score bounds
+  uint64_t* rkeys;        // BF16 form: [n][k] FP32-re-scored keys
+  int32_t* out_idx;       // [n][k]
+  float* out_dist;        // [n][k]
+  int32_t* ovf;           // set to 1 when any query's buffers overflowed
+  int32_t n, d, K, k;
+  int32_t ng, has_p1, dtype, bf16;
+};
+// Work-item lists of one batch: int4 items, one list per pass, at offsets into
+// the item array.  rows items: (request, first row) of 64 rows; tile items:
+// (request, query block, split); query items: (request, first query) of 4.
+struct BatchHeader {
+  int32_t n_reqs, n_rows, n_p1, n_p2, n_q;
+  int32_t off_rows, off_p1, off_p2, off_q;
+  int32_t pad[3];
+};
+
 // ---- launchers (knn_exact.hip) ----
 int tile_shape(int kind);         // MFMA tile shape in force on this thread (-1 for binary rows)
 void set_tile_shape(int shape);   // this thread's shape for the following launches (-1: default)
@@ -81,6 +115,7 @@ int tile_rows_c(int kind);        // candidate rows per tile
 int tile_rows_q(int kind);        // query rows per tile
 int groups_per_tile(int kind);    // pass-1 group maxima per candidate tile
 size_t tile_lds_bytes(int kind, int pass);
+size_t tile_lds_bytes_shape(int shape, int pass);  // LDS of the MFMA tile kernels of one shape
 hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);
 hipError_t launch_pass2(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);
 hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int K, float* tau, hipStream_t s);
@@ -95,6 +130,20 @@ hipError_t launch_rescore(const float* X, int64_t ld, int d, int n_q, int q_offs
                           const int32_t* order = nullptr);
 hipError_t launch_finalize(const uint64_t* keys, int n_q, int q_offset, int k, int dtype,
                            int32_t* idx, float* dist, hipStream_t s);
+// The batched exact search of element kind `kind` (BF16, FP32 or I8; 128 x 128
+// tiles; K <= 64) over the requests in `reqs`, every stage one persistent
+// launch over its work items: clear flags, prepare rows, threshold pass,
+// thresholds, scoring pass, selection, FP32 re-score (BF16 requests), output.
+// Every kernel argument is fixed (the table, items and header live at fixed
+// device addresses; `hdr` there, `h` its host copy sets the grids), so the
+// sequence can be captured once and replayed for any batch, with
+// batch_set_grids fitting the captured grids to each batch's item counts.
+// Before any launch the dynamic LDS limits must have been raised
+// (batch_prepare_attributes, outside any capture).
+hipError_t batch_prepare_attributes(int kind);
+hipError_t launch_batch(int kind, int n_cu, BatchReq* reqs, const int4* items, const BatchHeader* hdr,
+                        const BatchHeader& h, hipStream_t s);
+hipError_t batch_set_grids(hipGraph_t graph, hipGraphExec_t exec, int kind, int n_cu, const BatchHeader& h);
 
 // ---- launchers (prep.hip) ----
 // prepare_float (src/knn.rs:78-113) on the device, fused with padding and the
@@ -102,6 +151,12 @@ hipError_t launch_finalize(const uint64_t* keys, int n_q, int q_offset, int k, i
 hipError_t launch_prepare_float(const float* X, int64_t ld, int64_t n, int d, float* out,
                                 int64_t out_ld, uint16_t* out_bf16, int64_t bf_ld, float* norms,
                                 int32_t* flag, hipStream_t s);
+// batched input preparation (BatchReq): prepare_float for f32 requests (norms,
+// renormalise-all flag, prepared FP32 and BF16 rows), padding for int8 ones
+hipError_t launch_batch_prepare(unsigned norm_grid, unsigned write_grid, BatchReq* reqs, const int4* items,
+                                const BatchHeader* hdr, hipStream_t s);
+const void* batch_norm_kernel();   // the two preparation kernels (graph node identification)
+const void* batch_write_kernel();
 hipError_t launch_pad_rows(const uint8_t* X, int64_t ld_bytes, int64_t n, int row_bytes,
                            uint8_t* out, int64_t out_ld, hipStream_t s);
 hipError_t launch_row_popcount(const uint8_t* X, int64_t ld, int64_t n, int row_bytes,

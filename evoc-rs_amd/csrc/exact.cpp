@@ -474,3 +474,18 @@ int exact_plan_info(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, boo
   out->scratch_bytes = p.scratch_bytes(nq, cdiv(nq, TQ));
   return 0;
 }
+
+SlabPlan exact_slab_plan(int kind, int64_t n, int K, int n_cu) {
+  const Plan p = make_plan(kind, n, n, K, n_cu);
+  SlabPlan o;
+  o.n_splits = p.n_splits;
+  o.tiles_per_split = p.tiles_per_split;
+  o.caps = p.caps;
+  o.pass1 = p.pass1;
+  o.stride = p.stride;
+  o.n_sample = p.n_sample;
+  o.ng = p.ng;
+  o.p1_tiles_per_split = p.p1_tiles_per_split;
+  o.p1_splits = p.p1_splits;
+  return o;
+}

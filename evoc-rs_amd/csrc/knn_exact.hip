@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "evoc_device.h"
 #include "evoc_kernels.h"
 #include "mfma_traits.h"
@@ -314,16 +316,14 @@ __device__ __forceinline__ uint32_t valid_rows(int cb, int lane, int n_c) {
 // Pass 1: group maxima over a strided candidate sample (GROUPS per tile:
 // wave row half x BM blocks x lane half, 16 candidates each).
 // ---------------------------------------------------------------------------
+// The work of one (query block, candidate split): group maxima of its tiles.
 template <int KIND, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void pass1_groupmax_kernel(TileArgs a) {
-  extern __shared__ __align__(16) uint8_t lds[];
+__device__ __forceinline__ void pass1_work(const TileArgs& a, int qb, int split, uint8_t* lds) {
   using En = Engine<KIND, BM, BN, WM, WN>;
   using M = Mfma<KIND>;
   typename En::acc_t acc[BM][BN];
   const int lane = threadIdx.x & 63;
   const int wm = En::wave_m(), wn = En::wave_n();
-  int qb, split;
-  if (!tile_coords(a, qb, split)) return;
   const int q0 = qb * En::TQ;
   // candidate tiles of this split; group maxima are per tile
   const int t_begin = split * a.tiles_per_split;
@@ -374,14 +374,23 @@ __global__ __launch_bounds__(64 * WM * WN) void pass1_groupmax_kernel(TileArgs a
   }
 }
 
+template <int KIND, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void pass1_groupmax_kernel(TileArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  int qb, split;
+  if (!tile_coords(a, qb, split)) return;
+  pass1_work<KIND, BM, BN, WM, WN>(a, qb, split, lds);
+}
+
 // ---------------------------------------------------------------------------
 // Pass 2: threshold filter + append, one candidate split per workgroup.  Each
 // accumulator value is tested against a cheap bound first; the exact test and
 // the append run only in lanes whose value passes (exec-masked, inline).
 // ---------------------------------------------------------------------------
+// The work of one (query block, candidate split): appends of its tiles and the
+// split's per-query counts.
 template <int KIND, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) {
-  extern __shared__ __align__(16) uint8_t lds[];
+__device__ __forceinline__ void pass2_work(const TileArgs& a, int qb, int split, uint8_t* lds) {
   using En = Engine<KIND, BM, BN, WM, WN>;
   using M = Mfma<KIND>;
   // LDS after the stage ring: per-query append counters and bounds (the bounds
@@ -393,8 +402,6 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
   typename En::acc_t acc[BM][BN];
   const int lane = threadIdx.x & 63;
   const int wm = En::wave_m(), wn = En::wave_n();
-  int qb, split;
-  if (!tile_coords(a, qb, split)) return;
   const int q0 = qb * En::TQ;
   for (int i = threadIdx.x; i < En::TQ; i += En::THREADS) {
     const int q = q0 + i;
@@ -479,6 +486,14 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
     const int q = q0 + i;
     if (q < a.n_q) a.counts[(int64_t)split * a.n_q + q] = cnt[i];
   }
+}
+
+template <int KIND, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  int qb, split;
+  if (!tile_coords(a, qb, split)) return;
+  pass2_work<KIND, BM, BN, WM, WN>(a, qb, split, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1030,11 +1045,9 @@ __global__ __launch_bounds__(256) void jac_pass2_sym_kernel(TileArgs a) {
 // One wave per query, ng <= 64 * E, found by bisection (no sort).
 // ---------------------------------------------------------------------------
 template <int E>
-__global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict__ gmax, int n_q,
-                                                         int ng, int K, float* __restrict__ tau) {
+__device__ __forceinline__ void tau_select_one(const float* __restrict__ gmax, int ng, int K,
+                                               float* __restrict__ tau, int q) {
   const int lane = threadIdx.x & 63;
-  const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (q >= n_q) return;
   uint32_t v[E];
 #pragma unroll
   for (int e = 0; e < E; e++) {
@@ -1045,6 +1058,14 @@ __global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict
   // the -inf padding past ng decides no count that fixes the K-th when K <= ng
   const uint32_t kth = wave_kth_largest<E>(v, K, ng);
   if (lane == 0) tau[q] = (K <= ng) ? unord_f32(kth) : -INFINITY;
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict__ gmax, int n_q,
+                                                         int ng, int K, float* __restrict__ tau) {
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= n_q) return;
+  tau_select_one<E>(gmax, ng, K, tau, q);
 }
 
 // ---------------------------------------------------------------------------
@@ -1217,15 +1238,11 @@ __device__ __forceinline__ int select_query(const SelectArgs& a, int qi) {
 // (width from the query's own key count, a wave-uniform branch) and selected
 // by bisection.  select_stream_kernel takes the others.
 template <int EK>
-__global__ __launch_bounds__(256) void select_topk_kernel(SelectArgs a) {
-  extern __shared__ uint64_t sel_lds[];
-  const int q = select_query(a, blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (q < 0) return;
+__device__ __forceinline__ void select_topk_one(const SelectArgs& a, int q, uint64_t* sel) {
   const int nseg = a.n_splits + (a.cbufs ? 1 : 0);
   if (nseg > 64) return;
   const SegLane g = segment_lanes(a, q);
   const int total = g.total;
-  uint64_t* sel = sel_lds + (threadIdx.x >> 6) * 64 * EK;
   if (total <= 64) select_bisect<1, EK>(a, q, g.overflow, total, nseg, g.kept, g.base, g.src, sel);
   else if (total <= 128) select_bisect<2, EK>(a, q, g.overflow, total, nseg, g.kept, g.base, g.src, sel);
   else if (total <= 256) select_bisect<4, EK>(a, q, g.overflow, total, nseg, g.kept, g.base, g.src, sel);
@@ -1234,11 +1251,17 @@ __global__ __launch_bounds__(256) void select_topk_kernel(SelectArgs a) {
   else if (total <= SELECT_REG_KEYS) select_bisect<32, EK>(a, q, g.overflow, total, nseg, g.kept, g.base, g.src, sel);
 }
 
-// The queries select_topk_kernel leaves: their keys streamed through a
-// 2048-key wave sort.
-__global__ __launch_bounds__(256) void select_stream_kernel(SelectArgs a) {
+template <int EK>
+__global__ __launch_bounds__(256) void select_topk_kernel(SelectArgs a) {
+  extern __shared__ uint64_t sel_lds[];
   const int q = select_query(a, blockIdx.x * 4 + (threadIdx.x >> 6));
   if (q < 0) return;
+  select_topk_one<EK>(a, q, sel_lds + (threadIdx.x >> 6) * 64 * EK);
+}
+
+// The queries select_topk_kernel leaves: their keys streamed through a
+// 2048-key wave sort.
+__device__ __forceinline__ void select_stream_one(const SelectArgs& a, int q) {
   const int nseg = a.n_splits + (a.cbufs ? 1 : 0);
   if (nseg <= 64) {
     const SegLane g = segment_lanes(a, q);
@@ -1260,17 +1283,22 @@ __global__ __launch_bounds__(256) void select_stream_kernel(SelectArgs a) {
   select_body<32>(a, q, overflow, total);
 }
 
+__global__ __launch_bounds__(256) void select_stream_kernel(SelectArgs a) {
+  const int q = select_query(a, blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (q < 0) return;
+  select_stream_one(a, q);
+}
+
 // ---------------------------------------------------------------------------
 // FP32 re-score of the BF16 survivors with a fixed reduction order: lane l
 // accumulates elements l, l+64, ... with fmaf, then a butterfly sum.  The k
 // best by exact FP32 score are kept (Kc <= 64*E candidates per query).
 // ---------------------------------------------------------------------------
 template <int E>
-__global__ __launch_bounds__(256) void rescore_kernel(const float* __restrict__ X, int64_t ld, int d,
-                                                      int n_q, int q_offset,
-                                                      const uint64_t* __restrict__ cand_keys, int Kc,
-                                                      int k, uint64_t* __restrict__ out_keys,
-                                                      const int32_t* __restrict__ order) {
+__device__ __forceinline__ void rescore_one(const float* __restrict__ X, int64_t ld, int d, int q_offset,
+                                            const uint64_t* __restrict__ cand_keys, int Kc, int k,
+                                            uint64_t* __restrict__ out_keys, const int32_t* __restrict__ order,
+                                            int qi) {
   // four candidates at a time, one 16-lane group each; lane t of a group sums
   // the float4 chunks t, t + 16, ... of the row with fmaf (fixed order), then a
   // fixed xor-butterfly over the 16 lanes.  Queries are taken in `order` when
@@ -1282,8 +1310,6 @@ __global__ __launch_bounds__(256) void rescore_kernel(const float* __restrict__ 
   constexpr int CH = 16;
   const int lane = threadIdx.x & 63;
   const int grp = lane >> 4, t = lane & 15;
-  const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (qi >= n_q) return;
   const int ql = order ? order[qi] : qi;
   const int q = ql + q_offset;
   const float* xq = X + (int64_t)q * ld;
@@ -1362,15 +1388,23 @@ __global__ __launch_bounds__(256) void rescore_kernel(const float* __restrict__ 
   }
 }
 
+template <int E>
+__global__ __launch_bounds__(256) void rescore_kernel(const float* __restrict__ X, int64_t ld, int d,
+                                                      int n_q, int q_offset,
+                                                      const uint64_t* __restrict__ cand_keys, int Kc,
+                                                      int k, uint64_t* __restrict__ out_keys,
+                                                      const int32_t* __restrict__ order) {
+  const int qi = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (qi >= n_q) return;
+  rescore_one<E>(X, ld, d, q_offset, cand_keys, Kc, k, out_keys, order, qi);
+}
+
 // ---------------------------------------------------------------------------
 // Keys -> (indices, transformed distances) (src/knn.rs:156-187).
 // ---------------------------------------------------------------------------
-__global__ void finalize_kernel(const uint64_t* __restrict__ keys, int n_q, int q_offset, int k,
-                                int dtype, int32_t* __restrict__ out_idx,
-                                float* __restrict__ out_dist) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)n_q * k) return;
-  const int q = (int)(i / k) + q_offset;
+// key i of query q (global index) -> out_idx[i], out_dist[i]
+__device__ __forceinline__ void finalize_elem(const uint64_t* __restrict__ keys, int64_t i, int q, int dtype,
+                                              int32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
   const uint64_t key = keys[i];
   if (key == EVOC_KEY_NONE) {
     out_idx[i] = -1;
@@ -1400,6 +1434,121 @@ __global__ void finalize_kernel(const uint64_t* __restrict__ keys, int n_q, int 
     d = raw >= 0.0f ? 0.0f : (float)(-log2((double)(-raw)));
   }
   out_dist[i] = d;
+}
+
+__global__ void finalize_kernel(const uint64_t* __restrict__ keys, int n_q, int q_offset, int k,
+                                int dtype, int32_t* __restrict__ out_idx,
+                                float* __restrict__ out_dist) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n_q * k) return;
+  finalize_elem(keys, i, (int)(i / k) + q_offset, dtype, out_idx, out_dist);
+}
+
+// ---------------------------------------------------------------------------
+// Batched exact search: the passes above over many requests per launch.  Each
+// kernel is persistent: a fixed grid walks its work-item list (device memory)
+// with a stride of the grid, so the same launches serve any batch.  Per item
+// the work is exactly the single-request kernel's, with that request's
+// arguments from the table, so every request's keys are the ones it gets alone.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void batch_begin_kernel(BatchReq* __restrict__ reqs,
+                                                          const BatchHeader* __restrict__ hdr) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < hdr->n_reqs; r += gridDim.x * blockDim.x) {
+    *reqs[r].ovf = 0;
+    if (reqs[r].flag) *reqs[r].flag = 0;
+  }
+}
+
+template <int KIND, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void pass1_batch_kernel(const BatchReq* __restrict__ reqs,
+                                                                   const int4* __restrict__ items,
+                                                                   const BatchHeader* __restrict__ hdr) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const int n = hdr->n_p1;
+  const int4* it = items + hdr->off_p1;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int4 w = it[i];
+    __syncthreads();  // the previous item's waves are done with the LDS
+    pass1_work<KIND, BM, BN, WM, WN>(reqs[w.x].p1, w.y, w.z, lds);
+  }
+}
+
+template <int KIND, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void pass2_batch_kernel(const BatchReq* __restrict__ reqs,
+                                                                   const int4* __restrict__ items,
+                                                                   const BatchHeader* __restrict__ hdr) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  const int n = hdr->n_p2;
+  const int4* it = items + hdr->off_p2;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int4 w = it[i];
+    __syncthreads();  // the previous item's counters and bounds are written out
+    pass2_work<KIND, BM, BN, WM, WN>(reqs[w.x].p2, w.y, w.z, lds);
+  }
+}
+
+// per-query stages: one wave per query, four queries per item
+__global__ __launch_bounds__(256) void tau_batch_kernel(const BatchReq* __restrict__ reqs,
+                                                        const int4* __restrict__ items,
+                                                        const BatchHeader* __restrict__ hdr) {
+  const int n = hdr->n_q;
+  const int4* it = items + hdr->off_q;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int4 w = it[i];
+    const BatchReq& r = reqs[w.x];
+    const int q = w.y + (int)(threadIdx.x >> 6);
+    if (r.has_p1 && q < r.n) tau_select_one<32>(r.gmax, r.ng, r.K, r.tau, q);
+  }
+}
+
+__global__ __launch_bounds__(256) void select_batch_kernel(const BatchReq* __restrict__ reqs,
+                                                           const int4* __restrict__ items,
+                                                           const BatchHeader* __restrict__ hdr) {
+  __shared__ uint64_t sel[4 * 64];  // K <= 64: one 64-key slice per wave
+  const int n = hdr->n_q;
+  const int4* it = items + hdr->off_q;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int4 w = it[i];
+    const BatchReq& r = reqs[w.x];
+    const int q = w.y + (int)(threadIdx.x >> 6);
+    if (q >= r.n) continue;
+    // the register path takes a query with at most 64 segments and
+    // SELECT_REG_KEYS keys, the streaming path every other one
+    select_topk_one<1>(r.sel, q, sel + (threadIdx.x >> 6) * 64);
+    select_stream_one(r.sel, q);
+    if ((threadIdx.x & 63) == 0 && r.sel.overflow[q]) atomicOr(r.ovf, 1);
+  }
+}
+
+__global__ __launch_bounds__(256) void rescore_batch_kernel(const BatchReq* __restrict__ reqs,
+                                                            const int4* __restrict__ items,
+                                                            const BatchHeader* __restrict__ hdr) {
+  const int n = hdr->n_q;
+  const int4* it = items + hdr->off_q;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int4 w = it[i];
+    const BatchReq& r = reqs[w.x];
+    const int q = w.y + (int)(threadIdx.x >> 6);
+    if (r.bf16 && q < r.n) rescore_one<1>(r.xf32, r.f32_ld, r.d, 0, r.sel.out_keys, r.K, r.k, r.rkeys, nullptr, q);
+  }
+}
+
+__global__ __launch_bounds__(256) void finalize_batch_kernel(const BatchReq* __restrict__ reqs,
+                                                             const int4* __restrict__ items,
+                                                             const BatchHeader* __restrict__ hdr) {
+  const int n = hdr->n_q;
+  const int4* it = items + hdr->off_q;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int4 w = it[i];
+    const BatchReq& r = reqs[w.x];
+    const int q = w.y + (int)(threadIdx.x >> 6);
+    if (q >= r.n) continue;
+    // the BF16 form outputs its re-scored keys (k per query), the others the
+    // selected ones (K = k per query)
+    const uint64_t* keys = r.bf16 ? r.rkeys : r.sel.out_keys;
+    for (int j = (int)(threadIdx.x & 63); j < r.k; j += 64)
+      finalize_elem(keys, (int64_t)q * r.k + j, q, r.dtype, r.out_idx, r.out_dist);
+  }
 }
 
 }  // namespace evoc
@@ -1450,6 +1599,19 @@ int tile_rows_c(int kind) { int tc, tq, g; shape_dims(kind, tc, tq, g); return t
 int tile_rows_q(int kind) { int tc, tq, g; shape_dims(kind, tc, tq, g); return tq; }
 int groups_per_tile(int kind) { int tc, tq, g; shape_dims(kind, tc, tq, g); return g; }
 
+// LDS bytes of the MFMA tile kernels of `shape` for pass 1 / 2 / 3
+size_t tile_lds_bytes_shape(int shape, int pass) {
+  int tc, tq;
+  switch (shape) {
+    case 1: tc = 128; tq = 256; break;
+    case 2: case 3: tc = 256; tq = 256; break;
+    default: tc = 128; tq = 128; break;
+  }
+  const size_t ring = (size_t)MFMA_NS * (tc + tq) * MFMA_SB;
+  if (pass == 3) return ring + (size_t)tq * 40 + (size_t)tc * SYM_SLOTS * 8 + 16;
+  return ring + (pass == 2 ? (size_t)tq * (4 + 4 + 4 + 8) : 0);
+}
+
 size_t tile_lds_bytes(int kind, int pass) {
   if (kind == EVOC_ELEM_U8BITS) {
     // pass 2: per-query counters; symmetric pass (3): also the candidate rows'
@@ -1457,14 +1619,10 @@ size_t tile_lds_bytes(int kind, int pass) {
     if (pass == 3) return 2 * TILE * JAC_PITCH + TILE * (4 + 4 + 4 + 8) + TILE * SYM_SLOTS * 8;
     return 2 * TILE * JAC_PITCH + (pass == 2 ? TILE * sizeof(int) : 0);
   }
-  int tc, tq, g;
-  shape_dims(kind, tc, tq, g);
-  const size_t ring = (size_t)MFMA_NS * (tc + tq) * MFMA_SB;
   // pass 2 adds per-query counters, score bounds, fast bounds and key bounds;
   // the symmetric pass (3) adds the candidate rows' bounds, staged counts,
   // SYM_SLOTS staged keys per candidate row and four exactness flags
-  if (pass == 3) return ring + (size_t)tq * 40 + (size_t)tc * SYM_SLOTS * 8 + 16;
-  return ring + (pass == 2 ? (size_t)tq * (4 + 4 + 4 + 8) : 0);
+  return tile_lds_bytes_shape(tile_shape(kind), pass);
 }
 
 template <class Kern>
@@ -1616,6 +1774,103 @@ hipError_t launch_rescore(const float* X, int64_t ld, int d, int n_q, int q_offs
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// The batch kernels run 128 x 128 tiles (shape 0: 4 waves, two workgroups per CU).
+template <int KIND>
+static hipError_t batch_attributes_kind() {
+  const int l1 = (int)tile_lds_bytes_shape(0, 1), l2 = (int)tile_lds_bytes_shape(0, 2);
+  hipError_t e = hipFuncSetAttribute((const void*)pass1_batch_kernel<KIND, 2, 2, 2, 2>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, l1);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)pass2_batch_kernel<KIND, 2, 2, 2, 2>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, l2);
+  return e;
+}
+
+hipError_t batch_prepare_attributes(int kind) {
+  switch (kind) {
+    case EVOC_ELEM_BF16: return batch_attributes_kind<EVOC_ELEM_BF16>();
+    case EVOC_ELEM_F32: return batch_attributes_kind<EVOC_ELEM_F32>();
+    case EVOC_ELEM_I8: return batch_attributes_kind<EVOC_ELEM_I8>();
+  }
+  return hipErrorInvalidValue;
+}
+
+// Grid of each batch pass: one workgroup per item up to the persistent size
+// (two tile workgroups or four query workgroups per CU).  A grid never changes
+// what is computed, only how many workgroups wait for a CU: a small batch
+// beside a large running search would otherwise queue hundreds of empty ones.
+static unsigned item_grid(int items, int cap) { return (unsigned)max(1, min(items, cap)); }
+
+template <int KIND>
+static const void* batch_tile_kernel(bool pass1) {
+  return pass1 ? (const void*)pass1_batch_kernel<KIND, 2, 2, 2, 2> : (const void*)pass2_batch_kernel<KIND, 2, 2, 2, 2>;
+}
+
+static const void* batch_tile_kernel_of(int kind, bool pass1) {
+  switch (kind) {
+    case EVOC_ELEM_BF16: return batch_tile_kernel<EVOC_ELEM_BF16>(pass1);
+    case EVOC_ELEM_F32: return batch_tile_kernel<EVOC_ELEM_F32>(pass1);
+    default: return batch_tile_kernel<EVOC_ELEM_I8>(pass1);
+  }
+}
+
+// the grid this batch gives kernel `f` of the launch sequence (0: not one of them)
+static unsigned batch_grid_of(const void* f, int kind, int n_cu, const BatchHeader& h) {
+  if (f == (const void*)batch_begin_kernel) return 1;
+  if (f == batch_norm_kernel()) return item_grid(h.n_rows, 4 * n_cu);
+  if (f == batch_write_kernel()) return item_grid(h.n_rows, 2 * n_cu);
+  if (f == batch_tile_kernel_of(kind, true)) return item_grid(h.n_p1, 2 * n_cu);
+  if (f == batch_tile_kernel_of(kind, false)) return item_grid(h.n_p2, 2 * n_cu);
+  if (f == (const void*)tau_batch_kernel || f == (const void*)select_batch_kernel ||
+      f == (const void*)rescore_batch_kernel || f == (const void*)finalize_batch_kernel)
+    return item_grid(h.n_q, 4 * n_cu);
+  return 0;
+}
+
+hipError_t launch_batch(int kind, int n_cu, BatchReq* reqs, const int4* items, const BatchHeader* hdr,
+                        const BatchHeader& h, hipStream_t s) {
+  if (kind != EVOC_ELEM_BF16 && kind != EVOC_ELEM_F32 && kind != EVOC_ELEM_I8) return hipErrorInvalidValue;
+  const dim3 b(256), gq(item_grid(h.n_q, 4 * n_cu));
+  hipLaunchKernelGGL(batch_begin_kernel, dim3(1), b, 0, s, reqs, hdr);
+  hipError_t e = launch_batch_prepare(item_grid(h.n_rows, 4 * n_cu), item_grid(h.n_rows, 2 * n_cu), reqs, items,
+                                      hdr, s);
+  if (e != hipSuccess) return e;
+  const void* args[3] = {&reqs, &items, &hdr};
+  e = hipLaunchKernel(batch_tile_kernel_of(kind, true), dim3(item_grid(h.n_p1, 2 * n_cu)), b, (void**)args,
+                      tile_lds_bytes_shape(0, 1), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(tau_batch_kernel, gq, b, 0, s, reqs, items, hdr);
+  e = hipLaunchKernel(batch_tile_kernel_of(kind, false), dim3(item_grid(h.n_p2, 2 * n_cu)), b, (void**)args,
+                      tile_lds_bytes_shape(0, 2), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(select_batch_kernel, gq, b, 0, s, reqs, items, hdr);
+  if (kind == EVOC_ELEM_BF16) hipLaunchKernelGGL(rescore_batch_kernel, gq, b, 0, s, reqs, items, hdr);
+  hipLaunchKernelGGL(finalize_batch_kernel, gq, b, 0, s, reqs, items, hdr);
+  return hipGetLastError();
+}
+
+hipError_t batch_set_grids(hipGraph_t graph, hipGraphExec_t exec, int kind, int n_cu, const BatchHeader& h) {
+  size_t count = 0;
+  hipError_t e = hipGraphGetNodes(graph, nullptr, &count);
+  if (e != hipSuccess) return e;
+  std::vector<hipGraphNode_t> nodes(count);
+  e = hipGraphGetNodes(graph, nodes.data(), &count);
+  for (size_t i = 0; i < count && e == hipSuccess; i++) {
+    hipGraphNodeType type;
+    e = hipGraphNodeGetType(nodes[i], &type);
+    if (e != hipSuccess || type != hipGraphNodeTypeKernel) continue;
+    hipKernelNodeParams p;
+    e = hipGraphKernelNodeGetParams(nodes[i], &p);
+    if (e != hipSuccess) break;
+    const unsigned g = batch_grid_of(p.func, kind, n_cu, h);
+    if (g == 0) return hipErrorInvalidValue;  // a node this sequence does not launch
+    if (p.gridDim.x == g) continue;
+    p.gridDim = dim3(g);
+    e = hipGraphExecKernelNodeSetParams(exec, nodes[i], &p);
+  }
+  return e;
 }
 
 hipError_t launch_finalize(const uint64_t* keys, int n_q, int q_offset, int k, int dtype,

@@ -97,6 +97,88 @@ static inline dim3 grid_rows(int64_t n, int64_t cols, int bs) {
   return dim3((unsigned)((cols + bs - 1) / bs), (unsigned)n);
 }
 
+// Batched preparation: one persistent pass per stage over (request, 64-row)
+// items, each item the work of the single-request kernels above on its rows.
+// Norms: 64 threads per item, as row_norm_serial_kernel (same serial order).
+__global__ __launch_bounds__(NORM_ROWS) void norm_batch_kernel(const BatchReq* __restrict__ reqs,
+                                                               const int4* __restrict__ items,
+                                                               const BatchHeader* __restrict__ hdr) {
+  __shared__ float tile[NORM_ROWS][NORM_ROWS + 1];
+  const int t = threadIdx.x;
+  const int n_items = hdr->n_rows;
+  const int4* it = items + hdr->off_rows;
+  for (int i = blockIdx.x; i < n_items; i += gridDim.x) {
+    const int4 w = it[i];
+    const BatchReq& r = reqs[w.x];
+    if (r.dtype != EVOC_DTYPE_F32) continue;  // (uniform: the whole block skips)
+    const float* X = (const float*)r.x_in;
+    const int64_t r0 = w.y;
+    const int rows = (int)(r.n - r0 < NORM_ROWS ? r.n - r0 : NORM_ROWS);
+    float acc = 0.0f;
+    for (int c0 = 0; c0 < r.d; c0 += NORM_ROWS) {
+      const int cols = min(NORM_ROWS, r.d - c0);
+      __syncthreads();  // the last chunk's (or item's) sums are done with the tile
+      if (t < cols)
+        for (int j = 0; j < rows; j++) tile[j][t] = X[(r0 + j) * r.ld_in + c0 + t];
+      __syncthreads();
+      if (t < rows)
+        for (int j = 0; j < cols; j++) acc = __fadd_rn(acc, __fmul_rn(tile[t][j], tile[t][j]));
+    }
+    if (t < rows) {
+      const float nr = __fsqrt_rn(acc);
+      if (fabsf(__fsub_rn(nr, 1.0f)) > 1e-5f) atomicOr(r.flag, 1);
+      r.norms[r0 + t] = nr == 0.0f ? 1.0f : nr;
+    }
+  }
+}
+
+// Rows: f32 -> prepared FP32 (f32_ld apart) and, for the BF16 form, the BF16
+// scored rows, as normalize_rows_kernel; int8 -> padded scored rows, as
+// pad_rows_kernel.  256 threads per item, columns across the threads.
+__global__ __launch_bounds__(256) void write_batch_kernel(const BatchReq* __restrict__ reqs,
+                                                          const int4* __restrict__ items,
+                                                          const BatchHeader* __restrict__ hdr) {
+  const int n_items = hdr->n_rows;
+  const int4* it = items + hdr->off_rows;
+  for (int i = blockIdx.x; i < n_items; i += gridDim.x) {
+    const int4 w = it[i];
+    const BatchReq& r = reqs[w.x];
+    const int64_t r0 = w.y;
+    const int rows = (int)(r.n - r0 < 64 ? r.n - r0 : 64);
+    if (r.dtype == EVOC_DTYPE_F32) {
+      const float* X = (const float*)r.x_in;
+      const bool renorm = *r.flag != 0;
+      const int64_t bf_ld = r.bf16 ? r.row_bytes / 2 : 0;
+      const int64_t cols = r.f32_ld > bf_ld ? r.f32_ld : bf_ld;
+      for (int j = 0; j < rows; j++) {
+        const int64_t row = r0 + j;
+        for (int64_t c = threadIdx.x; c < cols; c += blockDim.x) {
+          float v = c < r.d ? X[row * r.ld_in + c] : 0.0f;
+          if (renorm && c < r.d) v = __fdiv_rn(v, r.norms[row]);
+          if (c < r.f32_ld) r.xf32[row * r.f32_ld + c] = v;
+          if (r.bf16 && c < bf_ld) ((uint16_t*)r.rows)[row * bf_ld + c] = bf16_rne(v);
+        }
+      }
+    } else {
+      for (int j = 0; j < rows; j++) {
+        const int64_t row = r0 + j;
+        for (int64_t c = threadIdx.x; c < r.row_bytes; c += blockDim.x)
+          r.rows[row * r.row_bytes + c] = c < r.d ? r.x_in[row * r.ld_in + c] : (uint8_t)0;
+      }
+    }
+  }
+}
+
+hipError_t launch_batch_prepare(unsigned norm_grid, unsigned write_grid, BatchReq* reqs, const int4* items,
+                                const BatchHeader* hdr, hipStream_t s) {
+  hipLaunchKernelGGL(norm_batch_kernel, dim3(norm_grid), dim3(NORM_ROWS), 0, s, reqs, items, hdr);
+  hipLaunchKernelGGL(write_batch_kernel, dim3(write_grid), dim3(256), 0, s, reqs, items, hdr);
+  return hipGetLastError();
+}
+
+const void* batch_norm_kernel() { return (const void*)norm_batch_kernel; }
+const void* batch_write_kernel() { return (const void*)write_batch_kernel; }
+
 hipError_t launch_prepare_float(const float* X, int64_t ld, int64_t n, int d, float* out,
                                 int64_t out_ld, uint16_t* out_bf16, int64_t bf_ld, float* norms,
                                 int32_t* flag, hipStream_t s) {

@@ -145,6 +145,9 @@ SIGNATURES = {
     "evoc_mi355x_wait": ([_P, C.c_uint64, C.c_double, C.c_char_p, C.c_int64], C.c_int),
     "evoc_mi355x_server_request_bytes": ([_P, C.POINTER(_Input), C.POINTER(_Options), C.c_int32,
                                           C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+    "evoc_mi355x_server_set_batching": ([_P, C.c_int64], C.c_int),
+    "evoc_mi355x_server_batch_stats": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int64)], None),
     "evoc_mi355x_server_stats": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)], None),
     "evoc_mi355x_fuzzy_graph": ([_P, C.POINTER(_FuzzyInput), C.POINTER(_FuzzyOutput)], C.c_int),
@@ -685,6 +688,17 @@ class Server:
         if rc != OK:
             raise BackendError(rc, "invalid request")
         return int(full.value), int(low.value)
+
+    def set_batching(self, max_rows: int) -> None:
+        """Batch requests of at most max_rows rows (0: off); see evoc_mi355x_server_set_batching."""
+        rc = lib().evoc_mi355x_server_set_batching(self._h, max_rows)
+        if rc != OK:
+            raise BackendError(rc, "evoc_mi355x_server_set_batching")
+
+    def batch_stats(self) -> dict:
+        b, r, m, g = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        lib().evoc_mi355x_server_batch_stats(self._h, C.byref(b), C.byref(r), C.byref(m), C.byref(g))
+        return {"batches": b.value, "requests": r.value, "largest": m.value, "graph_replays": g.value}
 
     def stats(self):
         q, r, g, b = C.c_int64(), C.c_int64(), C.c_uint64(), C.c_uint64()

@@ -20,8 +20,8 @@
 extern "C" {
 #endif
 
-/* 6: adds evoc_mi355x_exact_plan, evoc_mi355x_make_forest, evoc_mi355x_knn_graph_noalloc and
- *    evoc_knn_output.slabs; rayon_threads = 0 follows the host's
+/* 6: adds evoc_mi355x_exact_plan, evoc_mi355x_make_forest, evoc_mi355x_knn_graph_noalloc,
+ *    evoc_mi355x_server_set_batching / _batch_stats and evoc_knn_output.slabs; rayon_threads = 0 follows the host's
  *    thread count
  * 5: adds evoc_npy_read / evoc_npz_read / evoc_npy_write / evoc_npy_free (struct layouts of 4 unchanged) */
 #define EVOC_MI355X_ABI_VERSION 6
@@ -268,6 +268,21 @@ int evoc_mi355x_wait(evoc_mi355x_server* srv, uint64_t ticket, double timeout_se
 int evoc_mi355x_server_request_bytes(const evoc_mi355x_server* srv, const evoc_knn_input* in,
                                      const evoc_knn_options* opts, int32_t out_location,
                                      uint64_t* full_bytes, uint64_t* min_bytes);
+/* Continuous batching of small requests (no counterpart in the reference):
+ * requests of at most max_rows rows in exact mode over f32 or int8 rows with
+ * k + margin <= 64 are queued for a batch lane, which runs every queued one of
+ * an element kind together, each pass one persistent launch over all of them
+ * (replayed from a hipGraph captured once per kind), on its own stream beside
+ * the slots.  Each result equals the request's result alone, bit for bit.
+ * Applies to requests submitted afterwards; max_rows = 0 turns it off.  On by
+ * default with max_rows = EVOC_MI355X_BATCH_ROWS or 16384 when the budget
+ * leaves the lane a workspace (min(4 GiB, budget / 8), at least 256 MiB,
+ * reserved from the budget at creation); EVOC_ERR_UNSUPPORTED without one. */
+int evoc_mi355x_server_set_batching(evoc_mi355x_server* srv, int64_t max_rows);
+/* batches run, requests they held, the largest batch, and batches run by replaying
+ * a captured hipGraph (the others launched pass by pass) */
+void evoc_mi355x_server_batch_stats(const evoc_mi355x_server* srv, int64_t* batches, int64_t* requests,
+                                    int64_t* largest, int64_t* graph_replays);
 /* requests queued / running and bytes granted to running requests */
 void evoc_mi355x_server_stats(const evoc_mi355x_server* srv, int64_t* queued, int64_t* running,
                               uint64_t* granted_bytes, uint64_t* budget_bytes);
