@@ -35,6 +35,9 @@ MEASURED_TRAFFIC = {
                                                       "(profiles/pass2_sym_bf16_100k_768_traffic.md)"),
     (1000000, 768, "pass2_sym_kernel<bf16>"): (2.61e12, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated "
                                                         "(profiles/pass2_sym_bf16_1m_768_traffic.md)"),
+    # the NN-descent local join: the mean over a call's 27 launches
+    (100000, 768, "nnd_join_mfma_kernel<bf16>"): (1.666e9, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated, "
+                                                           "mean per launch (profiles/nndescent_100k_768_k30_kernel_trace.md)"),
 }
 
 
@@ -557,17 +560,31 @@ def run_nndescent(args, torch, E):
         torch.cuda.synchronize()
         rec = recall_gpu(torch, idx, truth)
         sec, stats = _timed(torch, step, args.steps, args.warmup)
+        jb = float(np.mean([s_.join_bytes for s_ in stats]))
+        js = float(np.mean([s_.join_seconds for s_ in stats]))
+        jl = float(np.mean([s_.join_launches for s_ in stats]))
         out[name] = {"points_per_s": n / sec, "ms_per_step": sec * 1e3, "recall_at_k": rec,
-                     "iterations": stats[-1].iterations, "offers_dropped": stats[-1].offers_dropped}
+                     "iterations": stats[-1].iterations, "offers_dropped": stats[-1].offers_dropped,
+                     "join_bytes_per_call": jb, "join_ms_per_call": js * 1e3, "join_launches_per_call": jl,
+                     "join_gb_per_s": jb / js / 1e9 if js > 0 else None}
         log(f"[bench] nndescent {name}: {out[name]}")
     ctx.close()
     best = out["bf16"]
+    kern = "nnd_join_mfma_kernel<bf16>"
+    traffic, tsrc = MEASURED_TRAFFIC.get((n, d, kern), (None, None))
     res = _base_line(args, best["points_per_s"], best["ms_per_step"] / 1e3,
                      {"workload": f"{n}x{d} f32 blobs (50 centres, std 0.25, Zipf 1.1), cosine k={k}, "
                                   f"device NN-descent 8 iterations (delta=0), BF16 + FP32 re-score of k+{args.margin}",
                       "n": n, "d": d, "k": k, "method": "nndescent"},
                      "bf16 (fp32 accumulate) + fp32 rescore", {"recall_at_k": best["recall_at_k"], "fp32": out["fp32"],
                                                                "bf16": out["bf16"]})
+    # the local join is bound by its gathers of candidate rows: algorithmic bytes = each work
+    # item's candidate rows once (counted by the kernel), over the join launches' event time
+    per_launch = best["join_bytes_per_call"] / max(best["join_launches_per_call"], 1.0)
+    res["roofline"] = {"bound": "hbm", "kernel": kern, "achieved": best["join_gb_per_s"], "peak": 8000.0,
+                       "unit": "GB/s", "frac": (best["join_gb_per_s"] or 0.0) / 8000.0, "traffic": traffic,
+                       "traffic_source": tsrc, "algorithmic_bytes_per_launch": per_launch,
+                       "avg_launch_ms": best["join_ms_per_call"] / max(best["join_launches_per_call"], 1.0)}
     if not args.no_cpu_baseline:
         keep = {}
         res["cpu_baseline"] = cpu_baseline(x.cpu().numpy(), k, args.cpu_seconds, keep=keep)
@@ -577,6 +594,18 @@ def run_nndescent(args, torch, E):
             res["reference_recall_at_k"] = recall_gpu(torch, torch.from_numpy(ref_idx).cuda(), truth)
             log(f"[bench] reference NN-descent recall@{k} at 8 iterations: {res['reference_recall_at_k']:.5f}")
     return res
+
+
+def copy_calibration(torch):
+    """One 1 GiB device-to-device copy (1 GiB read, 1 GiB written): the known traffic that
+    scales FETCH_SIZE / WRITE_SIZE counter readings."""
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("[bench] copied 1 GiB device to device")
+    del src, dst
 
 
 def main():
@@ -619,7 +648,10 @@ def main():
         print(json.dumps(run_million(args, torch, E, "i8" if args.workload == "c3" else "u8")), flush=True)
         return
     if args.workload == "c2-nndescent":
-        print(json.dumps(run_nndescent(args, torch, E)), flush=True)
+        res = run_nndescent(args, torch, E)
+        if args.copy_calibration:
+            copy_calibration(torch)
+        print(json.dumps(res), flush=True)
         return
     if args.workload == "c2":
         r = run_c2_exact(args, torch, E, args.n)
@@ -641,13 +673,7 @@ def main():
         result["latency_ms"] = {"p50": result["serving"]["p50_ms"], "p99": result["serving"]["p99_ms"],
                                 "source": "serving stream"}
     if args.copy_calibration:
-        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("[bench] copied 1 GiB device to device")
-        del src, dst
+        copy_calibration(torch)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         from evoc_mi355x import synth

@@ -18,6 +18,10 @@ struct evoc_mi355x_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   hipEvent_t ev_score0 = nullptr, ev_score1 = nullptr;  // around the exact search's scoring launch
+  // event pairs around NN-descent's local-join launches (created with the
+  // context; their times are collected after each host synchronisation)
+  static constexpr int JOIN_EVENTS = 32;
+  hipEvent_t ev_join[2 * JOIN_EVENTS] = {};
   // Arena mode (evoc_mi355x_knn_graph_noalloc): while `arena` is set, every
   // DevBuf is carved from [arena, arena + arena_size) by a stack (a released
   // block returns its bytes when it is the top one), budget is the arena size
@@ -138,6 +142,9 @@ struct NndStats {
   int64_t leaves = 0;
   uint32_t dropped = 0;
   std::vector<uint32_t> changed;
+  uint64_t join_bytes = 0;   // algorithmic bytes the local joins gathered
+  double join_ms = 0.0;      // device time of the local-join launches
+  int64_t join_launches = 0;
 };
 
 // Device random-projection forest (nnd_driver.cpp over nndescent.hip's level

@@ -150,6 +150,8 @@ evoc_mi355x_ctx::~evoc_mi355x_ctx() {
     if (ev_end) (void)hipEventDestroy(ev_end);
     if (ev_score0) (void)hipEventDestroy(ev_score0);
     if (ev_score1) (void)hipEventDestroy(ev_score1);
+    for (hipEvent_t ev : ev_join)
+      if (ev) (void)hipEventDestroy(ev);
     while (!cache.empty()) free_cached(cache.size() - 1);
     if (stream) {
       (void)hipStreamSynchronize(stream);
@@ -381,6 +383,11 @@ int evoc_mi355x_create(int32_t device, uint64_t hbm_budget_bytes, evoc_mi355x_ct
     delete c;
     return EVOC_ERR_HIP;
   }
+  for (hipEvent_t& ev : c->ev_join)
+    if (hipEventCreate(&ev) != hipSuccess) {
+      delete c;
+      return EVOC_ERR_HIP;
+    }
   // the allocation-free entry point's scratch address on this device (looked up
   // once per device; a failure is reported by that entry point, not here)
   (void)noalloc_scratch_init(device);
@@ -648,6 +655,9 @@ static int knn_graph_run(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const e
   out->offers_dropped = 0;
   out->slabs = 0;
   out->score_pairs = 0;
+  out->join_bytes = 0;
+  out->join_seconds = 0.0;
+  out->join_launches = 0;
   out->score_seconds = 0.0;
   out->iterations = 0;
   out->rescore_rounds = 0;
@@ -760,6 +770,9 @@ static int knn_graph_run(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const e
     if (rc) return rc;
     out->iterations = ns.iterations;
     out->offers_dropped = ns.dropped;
+    out->join_bytes = (int64_t)ns.join_bytes;
+    out->join_seconds = ns.join_ms * 1e-3;
+    out->join_launches = ns.join_launches;
   } else {
     ExactStats xs;
     rc = exact_search(ctx, kind, (const uint8_t*)rows.ptr, row_bytes, (const uint32_t*)pop.ptr, n, (int)K,

@@ -40,6 +40,32 @@ struct Check {
   int fail() { return ctx->hip_fail(e, what); }
 };
 
+// Device time of the local-join launches: each launch between a pair of the
+// context's events; the times are read after a host synchronisation (or, when
+// every pair is in use, after waiting for the last one).
+struct JoinTimer {
+  Ctx* ctx;
+  int used = 0;
+  double ms = 0.0;
+  int64_t launches = 0;
+  void collect() {
+    for (int i = 0; i < used; i++) {
+      float t = 0.0f;
+      if (hipEventElapsedTime(&t, ctx->ev_join[2 * i], ctx->ev_join[2 * i + 1]) == hipSuccess) ms += t;
+    }
+    launches += used;
+    used = 0;
+  }
+  void begin(hipStream_t s) {
+    if (used == Ctx::JOIN_EVENTS) {
+      (void)hipEventSynchronize(ctx->ev_join[2 * used - 1]);
+      collect();
+    }
+    (void)hipEventRecord(ctx->ev_join[2 * used], s);
+  }
+  void end(hipStream_t s) { (void)hipEventRecord(ctx->ev_join[2 * used++ + 1], s); }
+};
+
 }  // namespace
 
 int forest_alloc(Ctx* ctx, ForestScratch& fs, int kind, int64_t n, int dim, int64_t row_bytes, int leaf_size) {
@@ -170,7 +196,14 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
     return rc;
   uint32_t* d_changed = (uint32_t*)counters.ptr;
   uint32_t* d_dropped = d_changed + 1;
+  unsigned long long* d_jbytes = (unsigned long long*)(d_changed + 2);  // 8-byte aligned
   Check ck{ctx};
+  JoinTimer jt{ctx};
+  auto join = [&](const JoinArgs& args, const char* what) {
+    jt.begin(s);
+    ck(nnd_launch_join(kind, args, s), what);
+    jt.end(s);
+  };
   ck(launch_fill_i32((int32_t*)counters.ptr, 4, 0, s), "counters");
   ck(nnd_launch_init((uint64_t*)keys.ptr, (float*)thr.ptr, (int32_t*)umark.ptr, n, Kg, s), "list init");
   ck(launch_fill_i32((int32_t*)ubuf.ptr, (int64_t)n * UCAP * 2, 0, s), "offer slots");  // EVOC_KEY_NONE
@@ -188,6 +221,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
   ja.newc = (const int32_t*)newc.ptr;
   ja.oldc = (const int32_t*)oldc.ptr;
   ja.mc = mc;
+  ja.bytes = d_jbytes;
 
   auto merge_all = [&]() {
     return ck(nnd_launch_merge((uint64_t*)keys.ptr, (float*)thr.ptr, (int32_t*)umark.ptr,
@@ -209,7 +243,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
       jl.leaves = tr.leaves;
       jl.perm = tr.perm;
       jl.salt = 1;
-      if (tr.n_leaves > 0) ck(nnd_launch_join(kind, jl, s), "leaf join");
+      if (tr.n_leaves > 0) join(jl, "leaf join");
       merge_all();
       if (st) st->leaves += tr.n_leaves;
     }
@@ -225,7 +259,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
     jr.v0 = 0;
     jr.n_items = n;
     jr.salt = 2;
-    ck(nnd_launch_join(kind, jr, s), "random join");
+    join(jr, "random join");
     merge_all();
   }
 
@@ -246,7 +280,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
       jj.v0 = (int32_t)v0;
       jj.n_items = (int32_t)std::min<int64_t>(block, n - v0);
       jj.salt = 3 + (uint32_t)it * 4096u + (uint32_t)(v0 / 1024);
-      ck(nnd_launch_join(kind, jj, s), "local join");
+      join(jj, "local join");
       merge_all();
     }
     block = std::min<int64_t>(block, n) * 2;  // float.rs:872-873
@@ -279,9 +313,17 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
   if (st) st->iterations = it;
   ck(nnd_launch_export((const uint64_t*)keys.ptr, n, Kg, out_keys, s), "export");
   uint32_t dropped = 0;
+  unsigned long long jbytes = 0;
   ck(hipMemcpyAsync(&dropped, d_dropped, 4, hipMemcpyDeviceToHost, s), "counters");
+  ck(hipMemcpyAsync(&jbytes, d_jbytes, 8, hipMemcpyDeviceToHost, s), "counters");
   ck(hipStreamSynchronize(s), "NN-descent");
-  if (st) st->dropped = dropped;
+  jt.collect();
+  if (st) {
+    st->dropped = dropped;
+    st->join_bytes = jbytes;
+    st->join_ms = jt.ms;
+    st->join_launches = jt.launches;
+  }
   if (ck.e != hipSuccess) return ck.fail();
   return EVOC_OK;
 }

@@ -31,6 +31,7 @@ struct JoinArgs {
   int32_t n;
   uint32_t salt;            // varies the slot hash between launches
   uint32_t* dropped;        // offers that met a better offer on both slots when they arrived
+  unsigned long long* bytes;  // algorithmic bytes gathered: each item's candidate rows once (may be null)
 };
 
 struct ForestArgs {

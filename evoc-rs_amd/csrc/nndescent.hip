@@ -136,6 +136,9 @@ __global__ __launch_bounds__(256, 2) void nnd_join_mfma_kernel(JoinArgs a) {
   __threadfence_block();
   __builtin_amdgcn_wave_barrier();
   if (nN == 0 || nC < 2) return;
+  // the item's algorithmic traffic: its nC candidate rows (the new rows are
+  // the first nN of them), each read once
+  if (a.bytes && lane == 0) atomicAdd(a.bytes, (unsigned long long)nC * (unsigned long long)a.row_bytes);
   const int nbm = (nN + 31) >> 5, nbn = (nC + 31) >> 5;
   // fragment row pointers (invalid rows read a valid row; masked below)
   const uint8_t* arow[2];
@@ -224,6 +227,8 @@ __global__ __launch_bounds__(256) void nnd_join_jaccard_kernel(JoinArgs a) {
   load_item(a, item, sh, nN, nC);
   __threadfence_block();
   __builtin_amdgcn_wave_barrier();
+  if (a.bytes && lane == 0 && nN > 0 && nC >= 2)
+    atomicAdd(a.bytes, (unsigned long long)nC * (unsigned long long)a.row_bytes);
   const int words = (int)(a.row_bytes / 4);
   for (int pi = lane; pi < nN * nC; pi += 64) {
     const int r = pi / nC, c = pi % nC;

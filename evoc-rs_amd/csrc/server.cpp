@@ -231,8 +231,11 @@ int evoc_mi355x_server_create(int32_t device, int32_t n_slots, uint64_t hbm_budg
     }
     s->slots.push_back(c);
   }
-  // the budget is shared by all slots: by default 90% of what is free now
-  s->budget = hbm_budget_bytes ? hbm_budget_bytes : s->slots[0]->budget;
+  // the budget is shared by all slots.  Between requests each slot's pool keeps
+  // up to budget / (4 slots) bytes cached (see work()), so granted and cached
+  // bytes together reach 1.25 budgets: by default the budget is 90% of what is
+  // free now divided by 1.25, so both fit the device
+  s->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)((double)s->slots[0]->budget / 1.25);
   for (auto* c : s->slots) {
     c->serving = true;
     c->contention = &s->contention;

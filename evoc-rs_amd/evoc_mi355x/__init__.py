@@ -87,7 +87,8 @@ class _Output(C.Structure):
                 ("seconds", C.c_double), ("device_seconds", C.c_double), ("score_seconds", C.c_double),
                 ("iterations", C.c_int64), ("rescore_rounds", C.c_int64), ("offers_dropped", C.c_int64),
                 ("queue_seconds", C.c_double),
-                ("score_pairs", C.c_int64), ("slabs", C.c_int64)]
+                ("score_pairs", C.c_int64), ("slabs", C.c_int64), ("join_bytes", C.c_int64),
+                ("join_seconds", C.c_double), ("join_launches", C.c_int64)]
 
 
 class _ExactPlan(C.Structure):
@@ -377,6 +378,9 @@ class CallStats:
     queue_seconds: float = 0.0
     score_pairs: int = 0
     slabs: int = 0
+    join_bytes: int = 0
+    join_seconds: float = 0.0
+    join_launches: int = 0
     extra: dict = field(default_factory=dict)
 
 
@@ -403,6 +407,9 @@ def _fill_stats(stats, res) -> None:
     stats.queue_seconds = res.queue_seconds
     stats.score_pairs = res.score_pairs
     stats.slabs = res.slabs
+    stats.join_bytes = res.join_bytes
+    stats.join_seconds = res.join_seconds
+    stats.join_launches = res.join_launches
 
 
 def _input_from(data, dtype_code=None):

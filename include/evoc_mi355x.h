@@ -115,6 +115,10 @@ typedef struct evoc_knn_output {
   int64_t score_pairs;     /* exact: row pairs the scoring kernel scored (one per unordered pair
                               in the symmetric pass, one per query and candidate otherwise) */
   int64_t slabs;           /* exact: query slabs the search ran in (1 = the whole matrix at once) */
+  int64_t join_bytes;      /* NN-descent: algorithmic bytes the local joins gathered (each work
+                              item's candidate rows once, rows padded to 128 bytes) */
+  double join_seconds;     /* NN-descent: device time of the local-join launches (HIP events) */
+  int64_t join_launches;   /* NN-descent: local-join launches */
 } evoc_knn_output;
 
 typedef struct evoc_mi355x_ctx evoc_mi355x_ctx;
@@ -246,7 +250,9 @@ int evoc_mi355x_autotune(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const e
  * computes for it alone.  Input and output buffers stay owned by the caller and
  * must stay valid until the request is waited for. */
 typedef struct evoc_mi355x_server evoc_mi355x_server;
-/* slots <= 0: 4; hbm_budget_bytes = 0: 90% of free device memory */
+/* slots <= 0: 4; hbm_budget_bytes = 0: 72% of free device memory (the slots'
+ * memory pools keep up to a quarter of the budget cached between requests, so
+ * 1.25 budgets must fit the device) */
 int evoc_mi355x_server_create(int32_t device, int32_t slots, uint64_t hbm_budget_bytes,
                               evoc_mi355x_server** out);
 /* waits for running requests; queued ones finish with EVOC_ERR_CANCELLED */
