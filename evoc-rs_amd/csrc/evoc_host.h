@@ -197,6 +197,7 @@ uint64_t exact_scratch_bytes(int kind, int64_t n, int64_t row_bytes, int K, int 
 // the symmetric pass (a host-only diagnostic; evoc_mi355x_exact_plan).
 struct ExactPlanInfo {
   int symmetric = 0, n_splits = 0, caps = 0, ccap = 0, stride = 0, n_sample = 0, segments = 0, tile_shape = -1;
+  int bound_rank = 0;  // J of the plan (the K-th group maximum when J == K)
   int64_t work_items = 0;
   uint64_t scratch_bytes = 0;
 };

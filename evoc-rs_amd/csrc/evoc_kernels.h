@@ -65,13 +65,16 @@ struct SelectArgs {
   int32_t n_list;
   int32_t q_offset;
   uint64_t* out_keys;     // [n_q][K]
-  int32_t* overflow;      // [n_q]
+  int32_t* overflow;      // [n_q] re-score flag: a buffer overflowed, or fewer than min_count keys
   uint64_t* tau_key_out;  // [n_q]
   // symmetric pass: the candidate-side list of each row, read as one more split
   const uint64_t* cbufs;  // [n_q][ccap] or null
   const int32_t* ccounts;
   int32_t ccap;
-  int32_t pad0;
+  // a query with fewer appended keys than this (over all segments, before the
+  // caps) is flagged for re-scoring: its bound was an estimate that missed
+  // (0: every bound of the pass was a guaranteed lower bound)
+  int32_t min_count;
 };
 
 // One request of a batched exact search (the persistent batch kernels of
@@ -118,7 +121,10 @@ size_t tile_lds_bytes(int kind, int pass);
 size_t tile_lds_bytes_shape(int shape, int pass);  // LDS of the MFMA tile kernels of one shape
 hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);
 hipError_t launch_pass2(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);
-hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int K, float* tau, hipStream_t s);
+// tau[q] = J-th largest group maximum of query q (the bound pass 2 tests);
+// with tau_safe, also tau_safe[q] = the K-th largest (a guaranteed lower bound)
+hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int J, float* tau, hipStream_t s, int K = 0,
+                             float* tau_safe = nullptr);
 hipError_t launch_select(const SelectArgs& a, hipStream_t s);
 // symmetric pass 2 over a.work (square tiles only: tile_rows_c == tile_rows_q)
 hipError_t launch_pass2_sym(int kind, const TileArgs& a, hipStream_t s);

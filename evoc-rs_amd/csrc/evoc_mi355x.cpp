@@ -621,6 +621,7 @@ extern "C" int evoc_mi355x_exact_plan(const evoc_knn_input* in, const evoc_knn_o
   out->segments = p.segments;
   out->tile_shape = p.tile_shape;
   out->candidates = (int32_t)c.K;
+  out->bound_rank = p.bound_rank;
   out->work_items = p.work_items;
   out->scratch_bytes = p.scratch_bytes;
   return EVOC_OK;

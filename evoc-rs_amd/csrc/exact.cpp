@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -53,15 +54,17 @@ struct Plan {
   int n_splits = 1, tiles_per_split = 1, caps = 0;
   bool pass1 = false;
   int stride = 1, n_sample = 0, ng = 0;
+  int J = 0;  // pass 2 tests the J-th largest group maximum (J < K: an estimate, see bound_rank)
   int p1_tiles_per_split = 1, p1_splits = 1;
   int group_q = 1, p1_group_q = 1;  // query blocks per XCD group (tile_coords)
+  int K = 0;
   // device bytes exact_search allocates for a slab of nq queries (each buffer
   // rounded to 256 bytes by DevBuf)
   size_t scratch_bytes(int64_t nq, int64_t nqb) const {
     auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
     size_t b = r((size_t)n_splits * nq * caps * 8) + r((size_t)n_splits * nq * 4) + r((size_t)nq * 4) +
                r((size_t)nq * 8) + r((size_t)(nqb + nq) * 4);
-    if (pass1) b += r((size_t)nq * ng * 4) + r((size_t)nq * 4);
+    if (pass1) b += r((size_t)nq * ng * 4) + r((size_t)nq * 4) + (J < K ? r((size_t)nq * 4) : 0);
     return b;
   }
 };
@@ -84,8 +87,52 @@ int64_t sample_rows(bool symmetric) {
   return env ? env : symmetric ? 8192 : 16384;
 }
 
+// Per-query probability that a query's estimated bound misses (see bound_rank):
+// 1e-5 by default; EVOC_MI355X_BOUND_EPS overrides it (a tuning knob, read at
+// every plan: results do not depend on it, a missed bound only costs a re-score;
+// 1 makes the bound the guaranteed one).
+double bound_eps() {
+  const char* e = std::getenv("EVOC_MI355X_BOUND_EPS");
+  const double v = e ? std::atof(e) : 0.0;
+  return v > 0.0 ? v : 1e-5;
+}
+
+// The rank J of the group maximum that bounds pass 2.  The K-th largest of a
+// query's group maxima is a guaranteed lower bound on its K-th best score, but
+// a loose one: the strided sample holds about K / stride of the query's K best,
+// so that bound sits near rank K * stride of all candidates, and every
+// candidate above it is appended.  The J-th largest group maximum exceeds the
+// K-th best score only if J sampled candidates rank among the query's K - 1
+// best; with every stride-th row sampled that count is about Binomial(K - 1,
+// 1 / stride).  J is the smallest rank whose tail probability is at most
+// bound_eps(): appended keys fall from about K * stride to J * stride per query
+// (at 1,000,000 rows and K = 46: rank 46 -> 10), and a query the estimate
+// misses ends pass 2 with fewer than K keys, is flagged by selection and
+// scored again against the guaranteed bound.  The result does not depend on J.
+int bound_rank(int K, int stride) {
+  if (stride <= 1 || K <= 1) return K;
+  const int m = K - 1;
+  const double p = 1.0 / stride, eps = bound_eps();
+  if (eps >= 1.0) return K;
+  // tail(j) = P(X >= j), X ~ Binomial(m, p), summed from the top down
+  std::vector<double> pmf((size_t)m + 1);
+  for (int i = 0; i <= m; i++)
+    pmf[(size_t)i] = std::exp(std::lgamma(m + 1.0) - std::lgamma(i + 1.0) - std::lgamma(m - i + 1.0) +
+                              i * std::log(p) + (m - i) * std::log1p(-p));
+  double tail = 0.0;
+  int J = K;  // P(X >= K) = 0: the guaranteed bound
+  for (int j = m; j >= 1; j--) {
+    tail += pmf[(size_t)j];
+    if (tail > eps) break;
+    J = j;
+  }
+  return J;
+}
+
 Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu, bool symmetric = false) {
   Plan p;
+  p.K = K;
+  p.J = K;
   const int TC = tile_rows_c(kind), TQ = tile_rows_q(kind);
   const int64_t n_ctiles = cdiv(n, TC);
   const int64_t nqb = cdiv(nq, TQ);
@@ -116,9 +163,10 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu, bool symmetric 
     }
     stride++;
   }
-  // a query's bound ranks near its K-th .. 2K-th sample group, i.e. about
-  // stride * 2K candidates of the full set pass it; keep 4x headroom
-  const int64_t expect = (int64_t)p.stride * 2 * K;
+  p.J = bound_rank(K, p.stride);
+  // a query's bound ranks near its J-th .. 2J-th sample group, i.e. about
+  // stride * 2J candidates of the full set pass it; keep 4x headroom
+  const int64_t expect = (int64_t)p.stride * 2 * p.J;
   int64_t caps = std::max<int64_t>(pow2ceil(cdiv(4 * expect, p.n_splits)), 128);
   // the kept subset must hold 2K keys so an overflow round tightens the bound
   caps = std::max<int64_t>(caps, pow2ceil(cdiv(2LL * K, p.n_splits)));
@@ -175,7 +223,7 @@ SymPlan make_sym_plan(int kind, int64_t n, int K, int n_cu) {
   // group_q items, each query tile once for its whole row of splits
   p.group_q = (int)std::max<int64_t>(1, std::max<int64_t>(slots / 8, 1) / std::min<int64_t>(p.n_splits, 8));
   if (const char* e = std::getenv("EVOC_MI355X_SYM_GROUP")) p.group_q = std::max(1, std::atoi(e));
-  const int64_t expect = (int64_t)p.stride * 2 * K;
+  const int64_t expect = (int64_t)p.stride * 2 * p.J;
   int64_t caps = std::max<int64_t>(pow2ceil(cdiv(4 * expect, p.n_splits)), 128);
   caps = std::max<int64_t>(caps, pow2ceil(cdiv(2LL * K, p.n_splits)));
   p.caps = (int)std::min<int64_t>(caps, MAX_SORT);
@@ -294,10 +342,12 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
     nq = std::min(slab, n - q0);
     const int64_t nqb = cdiv(nq, TQ);
     const Plan p = sym ? sp.p : make_plan(kind, n, nq, K, ctx->n_cu);
-    DevBuf gmax, tau, bufs, counts, ovf, tkey, list, cbufs, ccounts, work;
+    DevBuf gmax, tau, tau_safe, bufs, counts, ovf, tkey, list, cbufs, ccounts, work;
+    const bool estimate = p.pass1 && p.J < K;  // pass 2 tests an estimated bound
     if (p.pass1) {
       if ((rc = gmax.alloc(ctx, (size_t)nq * p.ng * 4, "group maxima"))) break;
       if ((rc = tau.alloc(ctx, (size_t)nq * 4, "score bounds"))) break;
+      if (estimate && (rc = tau_safe.alloc(ctx, (size_t)nq * 4, "guaranteed score bounds"))) break;
     }
     if ((rc = bufs.alloc(ctx, (size_t)p.n_splits * nq * p.caps * 8, "candidate buffers"))) break;
     if ((rc = counts.alloc(ctx, (size_t)p.n_splits * nq * 4, "candidate counts"))) break;
@@ -329,7 +379,9 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       a1.ng = p.ng;
       a1.group_q = p.p1_group_q;
       e = launch_pass1(kind, a1, (int)nqb, p.p1_splits, s);
-      if (e == hipSuccess) e = launch_tau_select((const float*)gmax.ptr, (int)nq, p.ng, K, (float*)tau.ptr, s);
+      if (e == hipSuccess)
+        e = launch_tau_select((const float*)gmax.ptr, (int)nq, p.ng, p.J, (float*)tau.ptr, s, K,
+                              estimate ? (float*)tau_safe.ptr : nullptr);
       if (e != hipSuccess) {
         rc = ctx->hip_fail(e, "threshold pass");
         break;
@@ -356,6 +408,9 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
     sa.out_keys = keys + q0 * K;
     sa.overflow = (int32_t*)ovf.ptr;
     sa.tau_key_out = (uint64_t*)tkey.ptr;
+    // with an estimated bound, a query left with fewer than K keys (or every
+    // candidate it has, if fewer) is scored again against the guaranteed one
+    if (estimate) sa.min_count = (int32_t)std::min<int64_t>(K, n - (seed_self ? 0 : 1));
 
     if (sym) {
       // query blocks skip the splits before their own tile: those counts stay 0
@@ -422,13 +477,17 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       if (e == hipSuccess) e = upload_sync(d_q, qlist.data(), qlist.size() * 4, s);
       // the re-scored queries' candidate-side lists are covered by pass 2 now
       if (e == hipSuccess && sym) e = launch_zero_list((int32_t*)ccounts.ptr, d_q, (int)qlist.size(), s);
+      // the round's bounds are guaranteed ones: the K-th kept key of an
+      // overflowed query (the K-th best of a subset), the K-th group maximum
+      // (a query whose estimated bound missed has no K-th kept key)
       TileArgs ar = a2;
       ar.work = nullptr;
       ar.block_list = d_blocks;
-      ar.tau = nullptr;
+      ar.tau = estimate ? (const float*)tau_safe.ptr : nullptr;
       ar.tau_key = (const uint64_t*)tkey.ptr;
       if (e == hipSuccess) e = launch_pass2(kind, ar, (int)blocks.size(), p.n_splits, s);
       SelectArgs sr = sa;
+      sr.min_count = 0;
       sr.query_list = d_q;
       sr.n_list = (int32_t)qlist.size();
       if (e == hipSuccess) e = launch_select(sr, s);
@@ -457,6 +516,7 @@ int exact_plan_info(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, boo
       out->ccap = sp.ccap;
       out->stride = sp.p.pass1 ? sp.p.stride : 0;
       out->n_sample = sp.p.pass1 ? sp.p.n_sample : 0;
+      out->bound_rank = sp.p.pass1 ? sp.p.J : 0;
       out->segments = sp.p.n_splits + 1;
       out->work_items = (int64_t)sp.work.size();
       out->scratch_bytes = sp.scratch_bytes(n);
@@ -469,6 +529,7 @@ int exact_plan_info(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, boo
   out->caps = p.caps;
   out->stride = p.pass1 ? p.stride : 0;
   out->n_sample = p.pass1 ? p.n_sample : 0;
+  out->bound_rank = p.pass1 ? p.J : 0;
   out->segments = p.n_splits;
   out->work_items = cdiv(nq, TQ) * p.n_splits;
   out->scratch_bytes = p.scratch_bytes(nq, cdiv(nq, TQ));

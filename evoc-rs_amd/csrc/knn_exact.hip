@@ -1041,12 +1041,21 @@ __global__ __launch_bounds__(256) void jac_pass2_sym_kernel(TileArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Threshold from group maxima: tau_q = K-th largest of gmax[q, 0..ng).
-// One wave per query, ng <= 64 * E, found by bisection (no sort).
+// Thresholds from group maxima, one wave per query, ng <= 64 * E, found by
+// bisection (no sort):
+//   tau_q      = J-th largest of gmax[q, 0..ng), the bound pass 2 tests;
+//   tau_safe_q = K-th largest (when asked for), a guaranteed lower bound on
+//                the query's K-th best score: K groups each hold a candidate
+//                at least that good.
+// With J < K, tau_q is an estimate: it stays below the K-th best score unless
+// J of the sampled candidates rank among the query's K - 1 best, which the
+// sample rate makes rare (the host picks J from that binomial tail).  A query
+// it misses ends pass 2 with fewer than K keys; selection flags it and it is
+// scored again against tau_safe.
 // ---------------------------------------------------------------------------
 template <int E>
-__device__ __forceinline__ void tau_select_one(const float* __restrict__ gmax, int ng, int K,
-                                               float* __restrict__ tau, int q) {
+__device__ __forceinline__ void tau_select_one(const float* __restrict__ gmax, int ng, int J, float* __restrict__ tau,
+                                               int q, int K = 0, float* __restrict__ tau_safe = nullptr) {
   const int lane = threadIdx.x & 63;
   uint32_t v[E];
 #pragma unroll
@@ -1055,17 +1064,21 @@ __device__ __forceinline__ void tau_select_one(const float* __restrict__ gmax, i
     const float m = g < ng ? gmax[(int64_t)q * ng + g] : -INFINITY;
     v[e] = ord_f32(m);
   }
-  // the -inf padding past ng decides no count that fixes the K-th when K <= ng
-  const uint32_t kth = wave_kth_largest<E>(v, K, ng);
-  if (lane == 0) tau[q] = (K <= ng) ? unord_f32(kth) : -INFINITY;
+  // the -inf padding past ng decides no count that fixes the J-th when J <= ng
+  const uint32_t jth = wave_kth_largest<E>(v, J, ng);
+  if (lane == 0) tau[q] = (J <= ng) ? unord_f32(jth) : -INFINITY;
+  if (tau_safe) {
+    const uint32_t kth = wave_kth_largest<E>(v, K, ng);
+    if (lane == 0) tau_safe[q] = (K <= ng) ? unord_f32(kth) : -INFINITY;
+  }
 }
 
 template <int E>
-__global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict__ gmax, int n_q,
-                                                         int ng, int K, float* __restrict__ tau) {
+__global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict__ gmax, int n_q, int ng, int J,
+                                                         float* __restrict__ tau, int K, float* __restrict__ tau_safe) {
   const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (q >= n_q) return;
-  tau_select_one<E>(gmax, ng, K, tau, q);
+  tau_select_one<E>(gmax, ng, J, tau, q, K, tau_safe);
 }
 
 // ---------------------------------------------------------------------------
@@ -1213,6 +1226,14 @@ __device__ __forceinline__ SegLane segment_lanes(const SelectArgs& a, int q) {
   }
   SegLane r;
   r.overflow = __ballot(c > cap) != 0;
+  if (a.min_count > 0) {
+    // every appended key counts, kept or not: fewer than min_count means the
+    // query's estimated bound was above its true K-th score
+    int all = c;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) all += __shfl_xor(all, o, 64);
+    r.overflow = r.overflow || all < a.min_count;
+  }
   r.kept = min(c, cap);
   int incl = r.kept;  // inclusive prefix over the lanes (segment order)
 #pragma unroll
@@ -1268,18 +1289,21 @@ __device__ __forceinline__ void select_stream_one(const SelectArgs& a, int q) {
     if (g.total > SELECT_REG_KEYS) select_body<32>(a, q, g.overflow, g.total);
     return;
   }
-  int total = 0;
+  int total = 0, all = 0;
   bool overflow = false;
   for (int s = 0; s < a.n_splits; s++) {
     const int c = a.counts[(int64_t)s * a.n_q + q];
     overflow |= c > a.caps;
     total += min(c, a.caps);
+    all += c;
   }
   if (a.cbufs) {
     const int c = a.ccounts[q];
     overflow |= c > a.ccap;
     total += min(c, a.ccap);
+    all += c;
   }
+  if (a.min_count > 0 && all < a.min_count) overflow = true;
   select_body<32>(a, q, overflow, total);
 }
 
@@ -1730,15 +1754,19 @@ static int pow2_per_lane(int entries) {
   return e;
 }
 
-hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int K, float* tau, hipStream_t s) {
+hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int J, float* tau, hipStream_t s, int K,
+                             float* tau_safe) {
   const dim3 g((unsigned)((n_q + 3) / 4)), b(256);
   switch (pow2_per_lane(ng)) {
-    case 1: hipLaunchKernelGGL(tau_select_kernel<1>, g, b, 0, s, gmax, n_q, ng, K, tau); break;
-    case 2: hipLaunchKernelGGL(tau_select_kernel<2>, g, b, 0, s, gmax, n_q, ng, K, tau); break;
-    case 4: hipLaunchKernelGGL(tau_select_kernel<4>, g, b, 0, s, gmax, n_q, ng, K, tau); break;
-    case 8: hipLaunchKernelGGL(tau_select_kernel<8>, g, b, 0, s, gmax, n_q, ng, K, tau); break;
-    case 16: hipLaunchKernelGGL(tau_select_kernel<16>, g, b, 0, s, gmax, n_q, ng, K, tau); break;
-    case 32: hipLaunchKernelGGL(tau_select_kernel<32>, g, b, 0, s, gmax, n_q, ng, K, tau); break;
+#define EVOC_TAU_CASE(E) \
+  case E: hipLaunchKernelGGL(tau_select_kernel<E>, g, b, 0, s, gmax, n_q, ng, J, tau, K, tau_safe); break;
+    EVOC_TAU_CASE(1)
+    EVOC_TAU_CASE(2)
+    EVOC_TAU_CASE(4)
+    EVOC_TAU_CASE(8)
+    EVOC_TAU_CASE(16)
+    EVOC_TAU_CASE(32)
+#undef EVOC_TAU_CASE
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -94,7 +94,7 @@ class _Output(C.Structure):
 class _ExactPlan(C.Structure):
     _fields_ = [("symmetric", C.c_int32), ("n_splits", C.c_int32), ("caps", C.c_int32), ("ccap", C.c_int32),
                 ("stride", C.c_int32), ("n_sample", C.c_int32), ("segments", C.c_int32),
-                ("tile_shape", C.c_int32), ("candidates", C.c_int32), ("reserved0", C.c_int32),
+                ("tile_shape", C.c_int32), ("candidates", C.c_int32), ("bound_rank", C.c_int32),
                 ("work_items", C.c_int64), ("scratch_bytes", C.c_uint64)]
 
 
@@ -313,7 +313,7 @@ def exact_plan(dtype_code: int, n: int, d: int, options: KnnGraphOptions, n_cu: 
     rc = lib().evoc_mi355x_exact_plan(C.byref(i), C.byref(o), n_cu, int(serving), C.byref(p))
     if rc != OK:
         raise BackendError(rc, "evoc_mi355x_exact_plan")
-    return {name: getattr(p, name) for name, _ in _ExactPlan._fields_ if name != "reserved0"}
+    return {name: getattr(p, name) for name, _ in _ExactPlan._fields_}
 
 
 def _raise(code: int, ctx) -> None:

@@ -201,7 +201,8 @@ typedef struct evoc_exact_plan {
   int32_t segments;    /* key lists the selection merges per query */
   int32_t tile_shape;  /* MFMA tile shape (-1: binary rows) */
   int32_t candidates;  /* K: keys kept per query (k, or k + margin for the BF16 form) */
-  int32_t reserved0;
+  int32_t bound_rank;  /* threshold pass: scoring tests the bound_rank-th largest sample group maximum
+                          (< candidates: an estimated bound; queries it misses are scored again) */
   int64_t work_items;  /* workgroup work items of the scoring pass */
   uint64_t scratch_bytes; /* device scratch of the search (outside the rows and results) */
 } evoc_exact_plan;

@@ -129,3 +129,50 @@ def test_symmetric_plan_keeps_selection_in_registers(dtype, d, prec):
     # serving slots never take the symmetric pass
     s = E.exact_plan(E.DTYPE_F32, 100000, 768, E.KnnGraphOptions(n_neighbors=30), serving=True)
     assert s["symmetric"] == 0 and s["segments"] == s["n_splits"] <= 16
+
+
+def _binom_tail(m, p, j):
+    """P(X >= j), X ~ Binomial(m, p), with exact rationals (independent of the library's
+    log-domain sum)."""
+    from fractions import Fraction
+    from math import comb
+
+    q = Fraction(p)
+    return sum(comb(m, i) * q ** i * (1 - q) ** (m - i) for i in range(j, m + 1))
+
+
+@pytest.mark.parametrize("n,d,k,prec", [(1000000, 768, 30, E.PREC_BF16_RESCORE), (100000, 768, 30, E.PREC_BF16_RESCORE),
+                                         (100000, 768, 30, E.PREC_FP32), (30000, 384, 15, E.PREC_FP32),
+                                         (20000, 128, 10, E.PREC_FP32)])
+def test_bound_rank_is_the_smallest_rank_within_the_miss_budget(n, d, k, prec, monkeypatch):
+    """The scoring pass tests the J-th largest group maximum of the strided sample.  It
+    exceeds a query's K-th best score only if J sampled rows rank among its K - 1 best,
+    about Binomial(K - 1, 1 / stride): J is the smallest rank with that tail <= 1e-5
+    (EVOC_MI355X_BOUND_EPS), checked here with exact rational arithmetic; eps = 1 gives
+    the guaranteed rank K."""
+    from fractions import Fraction
+
+    o = E.KnnGraphOptions(n_neighbors=k, precision=prec)
+    p = E.exact_plan(E.DTYPE_F32, n, d, o)
+    K, st, J = p["candidates"], p["stride"], p["bound_rank"]
+    assert st > 1 and 1 <= J <= K
+    eps = Fraction(1, 100000)
+    assert _binom_tail(K - 1, Fraction(1, st), J) <= eps
+    if J > 1:
+        assert _binom_tail(K - 1, Fraction(1, st), J - 1) > eps
+    if n == 1000000 and K == 46:
+        assert (st, J) == (31, 10)  # rank 46 -> 10: about 4.6x fewer appended keys
+    monkeypatch.setenv("EVOC_MI355X_BOUND_EPS", "1")
+    assert E.exact_plan(E.DTYPE_F32, n, d, o)["bound_rank"] == K
+    monkeypatch.setenv("EVOC_MI355X_BOUND_EPS", "0.5")
+    j_loose = E.exact_plan(E.DTYPE_F32, n, d, o)["bound_rank"]
+    assert j_loose < J and _binom_tail(K - 1, Fraction(1, st), j_loose) <= Fraction(1, 2)
+
+
+def test_bound_rank_without_a_strided_sample():
+    # no threshold pass at all
+    p = E.exact_plan(E.DTYPE_F32, 2000, 128, E.KnnGraphOptions(n_neighbors=15))
+    assert p["stride"] == 0 and p["bound_rank"] == 0
+    # every row sampled: only the guaranteed rank K is safe
+    p = E.exact_plan(E.DTYPE_F32, 5000, 128, E.KnnGraphOptions(n_neighbors=15, precision=E.PREC_FP32))
+    assert p["stride"] == 1 and p["bound_rank"] == p["candidates"] == 15

@@ -269,6 +269,45 @@ def test_symmetric_pass_matches_split_pass(ctx, dtype, prec_name, n):
         np.testing.assert_array_equal(out["1"][0], ei)
 
 
+@pytest.mark.parametrize("sym", ["1", "0"])
+@pytest.mark.parametrize("dtype,prec_name", [("f32", "bf16"), ("f32", "fp32"), ("i8", None), ("u8", None)])
+def test_estimated_bound_misses_are_scored_again(ctx, dtype, prec_name, sym):
+    """The scoring pass tests an estimated bound (the J-th largest sample group maximum,
+    J < K); a query it misses ends with fewer than K keys and is scored again against the
+    guaranteed bound (the K-th group maximum).  With EVOC_MI355X_BOUND_EPS = 0.9 the
+    estimate misses for many queries (re-score rounds run); the graph is bit-identical to
+    the one the guaranteed bound gives (EVOC_MI355X_BOUND_EPS = 1), in the symmetric and
+    the per-split pass."""
+    n = 20000
+    print(f"[gpu] estimated bound {dtype} {prec_name} sym={sym} n={n}", flush=True)
+    prec = dict(PRECS).get(prec_name, E.PREC_FP32)
+    if dtype == "f32":
+        x = synth.blobs_f32(n, 256, 40, 0.3, seed=77)
+    elif dtype == "i8":
+        x = synth.blobs_i8(n, 256, 40, 0.3, seed=77)
+    else:
+        x = synth.bits_u8(n, 512, 60, 0.3, 0.08, seed=77)
+    o = E.KnnGraphOptions(n_neighbors=30, precision=prec)
+    out, rounds = {}, {}
+    os.environ["EVOC_MI355X_SYMMETRIC"] = sym
+    try:
+        for eps in ("1", "0.9", "1e-5"):
+            os.environ["EVOC_MI355X_BOUND_EPS"] = eps
+            plan = E.exact_plan({"f32": E.DTYPE_F32, "i8": E.DTYPE_I8, "u8": E.DTYPE_U8}[dtype], n, x.shape[1], o)
+            st = E.CallStats()
+            out[eps] = E.knn_graph(x, o, E.NumpyRandomState(3), ctx=ctx, stats=st)
+            rounds[eps] = (st.rescore_rounds, plan["bound_rank"], plan["candidates"])
+    finally:
+        os.environ.pop("EVOC_MI355X_SYMMETRIC", None)
+        os.environ.pop("EVOC_MI355X_BOUND_EPS", None)
+    print(f"[gpu] (rounds, J, K) by eps: {rounds}", flush=True)
+    assert rounds["1"][1] == rounds["1"][2] and rounds["0.9"][1] < rounds["1e-5"][1] < rounds["1"][1]
+    assert rounds["0.9"][0] >= 1  # the loose estimate missed and the misses were scored again
+    for eps in ("0.9", "1e-5"):
+        np.testing.assert_array_equal(out[eps][0], out["1"][0])
+        np.testing.assert_array_equal(out[eps][1].view(np.uint32), out["1"][1].view(np.uint32))
+
+
 def test_scratch_kept_between_calls_and_trimmed(ctx):
     """A context reuses its last call's scratch blocks (same shape: same graph), frees the
     ones a call of another shape does not reuse, and gives all back on trim."""
