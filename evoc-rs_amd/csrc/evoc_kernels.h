@@ -29,7 +29,7 @@ struct TileArgs {
   int32_t n_qblocks;       // query blocks (entries of block_list when given)
   int32_t n_splits;        // candidate splits
   int32_t group_q;         // query blocks per XCD group (they share candidate tiles)
-  int32_t pad1;
+  int32_t early_out;       // symmetric pass: 1 when most 32 x 32 blocks are expected to hold no key
   const int32_t* block_list;  // optional list of query blocks (re-scoring)
   const uint32_t* row_pop;    // binary rows: popcount per row
   // pass 1

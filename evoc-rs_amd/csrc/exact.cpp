@@ -423,6 +423,12 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       }
       a2.work = (const int4*)work.ptr;
       a2.n_work = (int32_t)sp.work.size();
+      // keys expected per 32 x 32 block (both sides): about stride * J per row
+      // spread over n candidates, 2 x 1024 tests per block; the early-out test
+      // pays where fewer than about one block in four holds a key (e^-1.5)
+      a2.early_out = p.pass1 && 2048.0 * p.stride * p.J / (double)n < 1.5 ? 1 : 0;
+      // EVOC_MI355X_EARLY_OUT=0/1 forces it (results do not depend on it)
+      if (const char* e = std::getenv("EVOC_MI355X_EARLY_OUT")) a2.early_out = e[0] == '1' ? 1 : 0;
       a2.cbufs = (uint64_t*)cbufs.ptr;
       a2.ccounts = (int32_t*)ccounts.ptr;
       a2.ccap = sp.ccap;

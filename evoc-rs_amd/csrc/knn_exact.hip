@@ -21,6 +21,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <climits>
+
+#include <type_traits>
 #include <vector>
 
 #include "evoc_device.h"
@@ -44,7 +47,9 @@ namespace evoc {
 // ---------------------------------------------------------------------------
 constexpr int STAGE_BYTES = 128;     // row bytes per stage of the Jaccard tiles
 constexpr int MFMA_SB = 64;          // row bytes per stage of the MFMA tiles
-constexpr int MFMA_NS = 4;           // stages in flight (LDS ring buffers)
+// Stages in flight (LDS ring buffers): four, three for the 128 x 256 tile,
+// whose ring then fits two workgroups in a CU's 160 KiB.
+constexpr int mfma_ring(int stage_rows) { return stage_rows == 384 ? 3 : 4; }
 
 // MFMA stage loads go straight from global memory into LDS
 // (global_load_lds_dwordx4: lane i of a wave writes 16 bytes at M0 + 16 * i),
@@ -124,7 +129,8 @@ struct Engine {
   static constexpr int TC = 32 * BM * WM;               // candidate rows per tile
   static constexpr int TQ = 32 * BN * WN;               // query rows per tile
   static constexpr int STAGE_LDS = (TC + TQ) * MFMA_SB;
-  static constexpr int LDS = MFMA_NS * STAGE_LDS;       // ring of MFMA_NS stages
+  static constexpr int NS = mfma_ring(TC + TQ);         // ring depth
+  static constexpr int LDS = NS * STAGE_LDS;            // ring of NS stages
   static constexpr int GROUPS = WM * BM * 2;            // pass-1 groups per tile
   // DMA instructions each wave issues per stage
   static constexpr int DMA_PER_STAGE = (TC + TQ) / 16 / WAVES;
@@ -139,23 +145,23 @@ struct Engine {
   }
 
   // Whether a workgroup's next tile can be primed by this one: its first
-  // MFMA_NS stages are loaded during this tile's last MFMA_NS stages, which
-  // needs at least MFMA_NS stages per tile.
-  __device__ static bool can_prime(const TileArgs& a) { return a.row_bytes / MFMA_SB >= MFMA_NS; }
+  // NS stages are loaded during this tile's last NS stages, which
+  // needs at least NS stages per tile.
+  __device__ static bool can_prime(const TileArgs& a) { return a.row_bytes / MFMA_SB >= NS; }
 
   // Accumulate the TC x TQ score tile (candidates c0.., queries q0..) over all
-  // stages.  Stage st of the tile lives in ring buffer (ph + st) % MFMA_NS;
+  // stages.  Stage st of the tile lives in ring buffer (ph + st) % NS;
   // `ph` carries the ring phase from tile to tile.  The stage boundary is
   // crossed inside the previous stage: once a wave has issued (and waited
   // for) its LDS reads of the last K step of stage s, it waits for its own
   // loads of stage s + 1, and one barrier then publishes stage s + 1 and
-  // retires buffer (ph + s) % MFMA_NS, which the loads of stage s + MFMA_NS
-  // refill (loads run MFMA_NS stages ahead).  The first reads of stage s + 1
+  // retires buffer (ph + s) % NS, which the loads of stage s + NS
+  // refill (loads run NS stages ahead).  The first reads of stage s + 1
   // then overlap the last MFMAs of stage s, so no stage starts on an LDS
   // bubble.  Within a K step, the reads of the next step alternate with the
   // MFMAs.
-  // With `has_next` the loads of the last MFMA_NS crossings are the first
-  // MFMA_NS stages of the next tile (nc0, nq0): they land during this tile's
+  // With `has_next` the loads of the last NS crossings are the first
+  // NS stages of the next tile (nc0, nq0): they land during this tile's
   // epilogue, and the next call, `primed`, starts on them.  Memory operations
   // issued between two tiles (the epilogue's) only make the vmcnt waits
   // stricter: everything older than a stage's loads has landed with them.
@@ -164,10 +170,8 @@ struct Engine {
                                                     int nq0) {
     const int lane = threadIdx.x & 63;
     const int wm = wave_m(), wn = wave_n();
-#pragma unroll
-    for (int i = 0; i < BM; i++)
-#pragma unroll
-      for (int j = 0; j < BN; j++) acc[i][j] = M::zero();
+    // (the accumulators start at zero through the first K step's MFMAs,
+    // which add to an inline zero instead of the previous tile's values)
     const int nst = (int)(a.row_bytes / MFMA_SB);
     // fragment rows of this lane; every block starts at a multiple of 32 rows,
     // so the swizzle key of the lane's row is the same in every block
@@ -179,7 +183,7 @@ struct Engine {
     constexpr int KS = MFMA_SB / M::KSTEP_BYTES;
     static_assert(KS % 2 == 0, "fragment double buffer parity must carry across stages");
     typename M::frag_t fa[2][BM], fb[2][BN];
-    auto ring = [&](int st) __attribute__((always_inline)) { return lds + ((ph + st) % MFMA_NS) * STAGE_LDS; };
+    auto ring = [&](int st) __attribute__((always_inline)) { return lds + ((ph + st) % NS) * STAGE_LDS; };
     auto read_frags = [&](const uint8_t* buf, int ks, int par) __attribute__((always_inline)) {
       const int kb = ks * M::KSTEP_BYTES + (lane >> 5) * M::LANE_BYTES;
       const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
@@ -206,33 +210,45 @@ struct Engine {
       // (kept rolled: unrolled, the compiler precomputes each stage's 64-bit
       // addresses, runs out of registers and spills them)
 #pragma unroll 1
-      for (int st = 0; st < MFMA_NS; st++)
+      for (int st = 0; st < NS; st++)
         if (st < nst) issue(st);
     }
-    wait_stage(min(MFMA_NS - 1, nst - 1));
+    wait_stage(min(NS - 1, nst - 1));
     ring_barrier();
     read_frags(ring(0), 0, 0);
     // MFMAs of one K step from fragment buffer `par`: one MFMA, then all the
     // fragment reads of the next step issued just before (the first MFMA's
     // operand wait covers the previous step's reads only), then the remaining
     // MFMAs, which give the reads the whole step to land
-    auto mma_step = [&](int par) __attribute__((always_inline)) {
+    // (`first`: the tile's first K step, whose MFMAs start from zero)
+    auto mma_step = [&](int par, auto first) __attribute__((always_inline)) {
 #pragma unroll
       for (int bm = 0; bm < BM; bm++)
 #pragma unroll
-        for (int bn = 0; bn < BN; bn++) M::mma(acc[bm][bn], fa[par][bm], fb[par][bn]);
+        for (int bn = 0; bn < BN; bn++) {
+          if constexpr (decltype(first)::value) M::mma_zero(acc[bm][bn], fa[par][bm], fb[par][bn]);
+          else M::mma(acc[bm][bn], fa[par][bm], fb[par][bn]);
+        }
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);        // MFMA
       __builtin_amdgcn_sched_group_barrier(0x100, BM + BN, 0);  // DS reads
       __builtin_amdgcn_sched_group_barrier(0x008, BM * BN - 1, 0);
       __builtin_amdgcn_sched_barrier(0);
     };
-    for (int s = 0; s < nst; s++) {
+    using no_t = std::integral_constant<bool, false>;
+    using yes_t = std::integral_constant<bool, true>;
+    // one stage; the first is peeled so that its first K step starts from zero
+    auto stage = [&](int s, auto first_stage) __attribute__((always_inline)) {
       const uint8_t* cur = ring(s);
 #pragma unroll
       for (int ks = 0; ks + 1 < KS; ks++) {
         const int p = ks & 1;
         read_frags(cur, ks + 1, p ^ 1);
-        mma_step(p);
+        if constexpr (decltype(first_stage)::value) {
+          if (ks == 0) mma_step(p, yes_t{});
+          else mma_step(p, no_t{});
+        } else {
+          mma_step(p, no_t{});
+        }
       }
       constexpr int PL = (KS - 1) & 1;  // buffer of the last K step
       // this wave's reads of stage s are complete (lgkmcnt)
@@ -240,22 +256,24 @@ struct Engine {
       if (s + 1 < nst) {
         // cross into stage s + 1: its loads have landed (vmcnt; two later
         // stages stay in flight while the stream continues)
-        wait_stage(has_next ? MFMA_NS - 2 : min(MFMA_NS - 2, nst - 2 - s));
+        wait_stage(has_next ? NS - 2 : min(NS - 2, nst - 2 - s));
         ring_barrier();
-        if (s + MFMA_NS < nst || has_next) issue(s + MFMA_NS);
+        if (s + NS < nst || has_next) issue(s + NS);
         read_frags(ring(s + 1), 0, PL ^ 1);
       } else {
         // last stage: every wave is done reading the ring
         ring_barrier();
-        if (has_next) issue(s + MFMA_NS);
+        if (has_next) issue(s + NS);
       }
-      mma_step(PL);
-    }
+      mma_step(PL, no_t{});
+    };
+    stage(0, yes_t{});
+    for (int s = 1; s < nst; s++) stage(s, no_t{});
     // without a prefetch every stage load has landed (the last crossing
     // waited for all); saying so lets the epilogue's LDS accesses go without
     // further vmcnt waits
     if (!has_next) wait_vmcnt<0>();
-    ph = (ph + nst) % MFMA_NS;
+    ph = (ph + nst) % NS;
   }
 };
 
@@ -375,7 +393,7 @@ __device__ __forceinline__ void pass1_work(const TileArgs& a, int qb, int split,
 }
 
 template <int KIND, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void pass1_groupmax_kernel(TileArgs a) {
+__global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass1_groupmax_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   int qb, split;
   if (!tile_coords(a, qb, split)) return;
@@ -489,7 +507,7 @@ __device__ __forceinline__ void pass2_work(const TileArgs& a, int qb, int split,
 }
 
 template <int KIND, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) {
+__global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass2_append_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   int qb, split;
   if (!tile_coords(a, qb, split)) return;
@@ -510,8 +528,20 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_append_kernel(TileArgs a) 
 // ---------------------------------------------------------------------------
 constexpr int SYM_SLOTS = 8;
 
+// An integer b with acc >= b whenever (float)acc >= t, for an i32
+// accumulator: floor(t), less 256 where floats of that size are more than one
+// apart (|(float)acc - acc| <= 128 below 2^31); INT_MIN / INT_MAX at the ends
+// (a NaN bound lets everything through to the exact test).
+__device__ __forceinline__ int early_int_bound(float t) {
+  if (!(t > -2147483520.0f)) return INT_MIN;
+  if (t >= 2147483520.0f) return INT_MAX;
+  int b = (int)floorf(t);
+  if (fabsf(t) >= 16777216.0f) b = b > INT_MIN + 256 ? b - 256 : INT_MIN;
+  return b;
+}
+
 template <int KIND, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
+__global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass2_sym_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   using En = Engine<KIND, BM, BN, WM, WN>;
   using M = Mfma<KIND>;
@@ -532,6 +562,10 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
   int* scnt = (int*)(ckey_l + T);
   uint64_t* stage = (uint64_t*)(scnt + T);
   int* cexact = (int*)(stage + T * SYM_SLOTS);
+  // int8: the fast bounds of query and candidate rows as integer lower bounds
+  // of the accumulators (early_int_bound), for the blocks' early-out test
+  int* tfast_i = cexact + 4;
+  int* cfast_i = tfast_i + T;
   typename En::acc_t acc[BM][BN];
   const int lane = threadIdx.x & 63;
   const int wm = En::wave_m(), wn = En::wave_n();
@@ -559,6 +593,7 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
     cnt[i] = 0;
     tau_l[i] = tau;
     tfast_l[i] = fast_of(tau, key);
+    tfast_i[i] = early_int_bound(tfast_l[i]);
     tkey_l[i] = key;
   }
   // candidate-row bounds of a tile (row threadIdx.x of it), into LDS with the
@@ -568,6 +603,7 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
     scnt[i] = 0;
     ctau_l[i] = tau;
     cfast_l[i] = fast_of(tau, key);
+    cfast_i[i] = early_int_bound(cfast_l[i]);
     ckey_l[i] = key;
     const bool ex = c0 + i >= a.n_q || bound_is_exact<KIND>(tau, key_bound);
     const uint64_t all = __ballot(ex);
@@ -600,12 +636,57 @@ __global__ __launch_bounds__(64 * WM * WN) void pass2_sym_kernel(TileArgs a) {
     bool c_fast = !diag;
 #pragma unroll
     for (int g = 0; g < T / 64; g++) c_fast = c_fast && cexact[g] != 0;
-    // One accumulator block at a time, rolled: the block's 16 values are copied
-    // out under a uniform branch, so the epilogue's code is one block's worth
-    // (unrolled over all BM x BN blocks it outgrows the instruction cache and
-    // is fetched again on every tile).
+    // Early-out, unrolled over the blocks: a block takes the full epilogue
+    // below only if some value of it reaches the cheap bound of its query row
+    // or of its candidate row (ballots of compares straight on the
+    // accumulators, ORed in scalar registers), or if it lies on the diagonal
+    // tile.  With the estimated bounds most blocks of a large matrix hold no
+    // key, and this test costs a fifth of the masks it saves; the host turns
+    // it off (early_out = 0) where most blocks hold keys.
+    uint32_t need = 0;
+    if (diag || !a.early_out) {
+      need = (1u << (BM * BN)) - 1u;
+    } else {
+      static_for<BM>([&](auto bmc) __attribute__((always_inline)) {
+        constexpr int bm = decltype(bmc)::value;
+        using B = typename std::conditional<KIND == EVOC_ELEM_I8, int, float>::type;
+        // bounds of the lane's 16 candidate rows 8 (j / 4) + 4 (lane / 32) + j % 4
+        const int rl = wm_e * 32 * BM + bm * 32 + 4 * (lane_e >> 5);
+        B cb[16];
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          if constexpr (KIND == EVOC_ELEM_I8) {
+            const int4 f = *(const int4*)(cfast_i + rl + 8 * g);
+            cb[4 * g] = f.x; cb[4 * g + 1] = f.y; cb[4 * g + 2] = f.z; cb[4 * g + 3] = f.w;
+          } else {
+            const float4 f = *(const float4*)(cfast_l + rl + 8 * g);
+            cb[4 * g] = f.x; cb[4 * g + 1] = f.y; cb[4 * g + 2] = f.z; cb[4 * g + 3] = f.w;
+          }
+        }
+        static_for<BN>([&](auto bnc) __attribute__((always_inline)) {
+          constexpr int bn = decltype(bnc)::value;
+          const int ql = wn_e * 32 * BN + bn * 32 + (lane_e & 31);
+          B tb;
+          if constexpr (KIND == EVOC_ELEM_I8) tb = tfast_i[ql];
+          else tb = tfast_l[ql];
+          uint64_t any = 0;
+#pragma unroll
+          for (int j = 0; j < 16; j++) {
+            const B x = acc[bm][bn][j];
+            any |= __ballot(x >= tb) | __ballot(x >= cb[j]);
+          }
+          if (any) need |= 1u << (bm * BN + bn);
+        });
+      });
+    }
+    // The blocks that need it, one at a time, rolled: the block's 16 values are
+    // copied out under a uniform branch, so the epilogue's code is one block's
+    // worth (unrolled over all BM x BN blocks it outgrows the instruction cache
+    // and is fetched again on every tile).
 #pragma unroll 1
-    for (int sb = 0; sb < BM * BN; sb++) {
+    while (need) {
+      const int sb = __builtin_ctz(need);
+      need &= need - 1u;
       const int bm = sb / BN, bn = sb % BN;
       float v[16];  // raw accumulator values of this block
       static_for<BM * BN>([&](auto sbc) __attribute__((always_inline)) {
@@ -1484,7 +1565,7 @@ __global__ __launch_bounds__(256) void batch_begin_kernel(BatchReq* __restrict__
 }
 
 template <int KIND, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void pass1_batch_kernel(const BatchReq* __restrict__ reqs,
+__global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass1_batch_kernel(const BatchReq* __restrict__ reqs,
                                                                    const int4* __restrict__ items,
                                                                    const BatchHeader* __restrict__ hdr) {
   extern __shared__ __align__(16) uint8_t lds[];
@@ -1498,7 +1579,7 @@ __global__ __launch_bounds__(64 * WM * WN) void pass1_batch_kernel(const BatchRe
 }
 
 template <int KIND, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN) void pass2_batch_kernel(const BatchReq* __restrict__ reqs,
+__global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass2_batch_kernel(const BatchReq* __restrict__ reqs,
                                                                    const int4* __restrict__ items,
                                                                    const BatchHeader* __restrict__ hdr) {
   extern __shared__ __align__(16) uint8_t lds[];
@@ -1631,8 +1712,8 @@ size_t tile_lds_bytes_shape(int shape, int pass) {
     case 2: case 3: tc = 256; tq = 256; break;
     default: tc = 128; tq = 128; break;
   }
-  const size_t ring = (size_t)MFMA_NS * (tc + tq) * MFMA_SB;
-  if (pass == 3) return ring + (size_t)tq * 40 + (size_t)tc * SYM_SLOTS * 8 + 16;
+  const size_t ring = (size_t)mfma_ring(tc + tq) * (tc + tq) * MFMA_SB;
+  if (pass == 3) return ring + (size_t)tq * 40 + (size_t)tc * SYM_SLOTS * 8 + 16 + (size_t)(tq + tc) * 4;
   return ring + (pass == 2 ? (size_t)tq * (4 + 4 + 4 + 8) : 0);
 }
 

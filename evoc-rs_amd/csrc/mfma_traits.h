@@ -32,6 +32,11 @@ template <> struct Mfma<EVOC_ELEM_BF16> {
     c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
                                                 0, 0, 0);
   }
+  // c = a b, accumulating into zero (the block's first K step)
+  __device__ static void mma_zero(acc_t& c, frag_t a, frag_t b) {
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                acc_t{}, 0, 0, 0);
+  }
   __device__ static acc_t zero() { return acc_t{}; }
   __device__ static void step(acc_t& c, const uint8_t* a, const uint8_t* b) {
     bf16x8 av = __builtin_bit_cast(bf16x8, *(const i32x4*)a);
@@ -50,6 +55,9 @@ template <> struct Mfma<EVOC_ELEM_F32> {
   __device__ static void mma(acc_t& c, frag_t a, frag_t b) {
     c = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
   }
+  __device__ static void mma_zero(acc_t& c, frag_t a, frag_t b) {
+    c = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc_t{}, 0, 0, 0);
+  }
   __device__ static acc_t zero() { return acc_t{}; }
   __device__ static void step(acc_t& c, const uint8_t* a, const uint8_t* b) {
     c = __builtin_amdgcn_mfma_f32_32x32x2f32(*(const float*)a, *(const float*)b, c, 0, 0, 0);
@@ -65,6 +73,9 @@ template <> struct Mfma<EVOC_ELEM_I8> {
   __device__ static frag_t load(const uint8_t* p) { return *(const i32x4*)p; }
   __device__ static void mma(acc_t& c, frag_t a, frag_t b) {
     c = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+  }
+  __device__ static void mma_zero(acc_t& c, frag_t a, frag_t b) {
+    c = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc_t{}, 0, 0, 0);
   }
   __device__ static acc_t zero() { return acc_t{}; }
   __device__ static void step(acc_t& c, const uint8_t* a, const uint8_t* b) {

@@ -256,14 +256,20 @@ def test_symmetric_pass_matches_split_pass(ctx, dtype, prec_name, n):
     else:
         x = synth.bits_u8(n, 512, 40, 0.3, 0.08, seed=n)
     out = {}
-    for sym in ("1", "0"):
+    # the symmetric pass with its per-block early-out test on and off
+    for sym, early in (("1", "1"), ("1", "0"), ("0", None)):
         os.environ["EVOC_MI355X_SYMMETRIC"] = sym
+        if early is not None:
+            os.environ["EVOC_MI355X_EARLY_OUT"] = early
         try:
-            out[sym] = run(ctx, x, 5, n_neighbors=25, precision=prec)
+            out[sym + str(early)] = run(ctx, x, 5, n_neighbors=25, precision=prec)
         finally:
             del os.environ["EVOC_MI355X_SYMMETRIC"]
-    np.testing.assert_array_equal(out["1"][0], out["0"][0])
-    np.testing.assert_array_equal(out["1"][1].view(np.uint32), out["0"][1].view(np.uint32))
+            os.environ.pop("EVOC_MI355X_EARLY_OUT", None)
+    for key in ("11", "10"):
+        np.testing.assert_array_equal(out[key][0], out["0None"][0])
+        np.testing.assert_array_equal(out[key][1].view(np.uint32), out["0None"][1].view(np.uint32))
+    out["1"] = out["11"]
     if dtype in ("i8", "u8"):
         ei, _ = O.exact_knn(x, 25)
         np.testing.assert_array_equal(out["1"][0], ei)
