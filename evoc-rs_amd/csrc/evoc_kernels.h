@@ -121,10 +121,10 @@ size_t tile_lds_bytes(int kind, int pass);
 size_t tile_lds_bytes_shape(int shape, int pass);  // LDS of the MFMA tile kernels of one shape
 hipError_t launch_pass1(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);
 hipError_t launch_pass2(int kind, const TileArgs& a, int n_blocks, int n_splits, hipStream_t s);
-// tau[q] = J-th largest group maximum of query q (the bound pass 2 tests);
-// with tau_safe, also tau_safe[q] = the K-th largest (a guaranteed lower bound)
-hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int J, float* tau, hipStream_t s, int K = 0,
-                             float* tau_safe = nullptr);
+// out[q] = the rank-th largest group maximum of query q, for q < n_q or for
+// the n_q queries of `list`
+hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int rank, float* out, hipStream_t s,
+                             const int32_t* list = nullptr);
 hipError_t launch_select(const SelectArgs& a, hipStream_t s);
 // symmetric pass 2 over a.work (square tiles only: tile_rows_c == tile_rows_q)
 hipError_t launch_pass2_sym(int kind, const TileArgs& a, hipStream_t s);

@@ -88,13 +88,13 @@ int64_t sample_rows(bool symmetric) {
 }
 
 // Per-query probability that a query's estimated bound misses (see bound_rank):
-// 1e-5 by default; EVOC_MI355X_BOUND_EPS overrides it (a tuning knob, read at
+// 1e-6 by default; EVOC_MI355X_BOUND_EPS overrides it (a tuning knob, read at
 // every plan: results do not depend on it, a missed bound only costs a re-score;
 // 1 makes the bound the guaranteed one).
 double bound_eps() {
   const char* e = std::getenv("EVOC_MI355X_BOUND_EPS");
   const double v = e ? std::atof(e) : 0.0;
-  return v > 0.0 ? v : 1e-5;
+  return v > 0.0 ? v : 1e-6;
 }
 
 // The rank J of the group maximum that bounds pass 2.  The K-th largest of a
@@ -106,7 +106,7 @@ double bound_eps() {
 // best; with every stride-th row sampled that count is about Binomial(K - 1,
 // 1 / stride).  J is the smallest rank whose tail probability is at most
 // bound_eps(): appended keys fall from about K * stride to J * stride per query
-// (at 1,000,000 rows and K = 46: rank 46 -> 10), and a query the estimate
+// (at 1,000,000 rows and K = 46: rank 46 -> 11), and a query the estimate
 // misses ends pass 2 with fewer than K keys, is flagged by selection and
 // scored again against the guaranteed bound.  The result does not depend on J.
 int bound_rank(int K, int stride) {
@@ -379,9 +379,7 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       a1.ng = p.ng;
       a1.group_q = p.p1_group_q;
       e = launch_pass1(kind, a1, (int)nqb, p.p1_splits, s);
-      if (e == hipSuccess)
-        e = launch_tau_select((const float*)gmax.ptr, (int)nq, p.ng, p.J, (float*)tau.ptr, s, K,
-                              estimate ? (float*)tau_safe.ptr : nullptr);
+      if (e == hipSuccess) e = launch_tau_select((const float*)gmax.ptr, (int)nq, p.ng, p.J, (float*)tau.ptr, s);
       if (e != hipSuccess) {
         rc = ctx->hip_fail(e, "threshold pass");
         break;
@@ -483,6 +481,10 @@ int exact_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, con
       if (e == hipSuccess) e = upload_sync(d_q, qlist.data(), qlist.size() * 4, s);
       // the re-scored queries' candidate-side lists are covered by pass 2 now
       if (e == hipSuccess && sym) e = launch_zero_list((int32_t*)ccounts.ptr, d_q, (int)qlist.size(), s);
+      // their guaranteed bounds (the K-th group maximum), for those whose
+      // estimated bound missed; every round's bounds are guaranteed ones
+      if (e == hipSuccess && estimate)
+        e = launch_tau_select((const float*)gmax.ptr, (int)qlist.size(), p.ng, K, (float*)tau_safe.ptr, s, d_q);
       // the round's bounds are guaranteed ones: the K-th kept key of an
       // overflowed query (the K-th best of a subset), the K-th group maximum
       // (a query whose estimated bound missed has no K-th kept key)

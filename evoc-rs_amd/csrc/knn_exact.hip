@@ -1122,21 +1122,20 @@ __global__ __launch_bounds__(256) void jac_pass2_sym_kernel(TileArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Thresholds from group maxima, one wave per query, ng <= 64 * E, found by
-// bisection (no sort):
-//   tau_q      = J-th largest of gmax[q, 0..ng), the bound pass 2 tests;
-//   tau_safe_q = K-th largest (when asked for), a guaranteed lower bound on
-//                the query's K-th best score: K groups each hold a candidate
-//                at least that good.
-// With J < K, tau_q is an estimate: it stays below the K-th best score unless
-// J of the sampled candidates rank among the query's K - 1 best, which the
-// sample rate makes rare (the host picks J from that binomial tail).  A query
-// it misses ends pass 2 with fewer than K keys; selection flags it and it is
-// scored again against tau_safe.
+// Thresholds from group maxima: out_q = the rank-th largest of gmax[q, 0..ng),
+// one wave per query, ng <= 64 * E, found by bisection (no sort).
+//   rank K: a guaranteed lower bound on the query's K-th best score (K groups
+//           each hold a candidate at least that good);
+//   rank J < K (the scoring pass's estimate): below the K-th best score unless
+//           J of the sampled candidates rank among the query's K - 1 best,
+//           which the sample rate makes rare (the host picks J from that
+//           binomial tail).  A query it misses ends the scoring pass with
+//           fewer than K keys; selection flags it, its guaranteed bound is
+//           taken (rank K, for the flagged queries only) and it is scored again.
 // ---------------------------------------------------------------------------
 template <int E>
-__device__ __forceinline__ void tau_select_one(const float* __restrict__ gmax, int ng, int J, float* __restrict__ tau,
-                                               int q, int K = 0, float* __restrict__ tau_safe = nullptr) {
+__device__ __forceinline__ void tau_select_one(const float* __restrict__ gmax, int ng, int rank,
+                                               float* __restrict__ out, int q) {
   const int lane = threadIdx.x & 63;
   uint32_t v[E];
 #pragma unroll
@@ -1145,21 +1144,18 @@ __device__ __forceinline__ void tau_select_one(const float* __restrict__ gmax, i
     const float m = g < ng ? gmax[(int64_t)q * ng + g] : -INFINITY;
     v[e] = ord_f32(m);
   }
-  // the -inf padding past ng decides no count that fixes the J-th when J <= ng
-  const uint32_t jth = wave_kth_largest<E>(v, J, ng);
-  if (lane == 0) tau[q] = (J <= ng) ? unord_f32(jth) : -INFINITY;
-  if (tau_safe) {
-    const uint32_t kth = wave_kth_largest<E>(v, K, ng);
-    if (lane == 0) tau_safe[q] = (K <= ng) ? unord_f32(kth) : -INFINITY;
-  }
+  // the -inf padding past ng decides no count that fixes the rank-th when rank <= ng
+  const uint32_t kth = wave_kth_largest<E>(v, rank, ng);
+  if (lane == 0) out[q] = (rank <= ng) ? unord_f32(kth) : -INFINITY;
 }
 
+// queries 0 .. n_q - 1, or list[0 .. n_q) when a list is given
 template <int E>
-__global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict__ gmax, int n_q, int ng, int J,
-                                                         float* __restrict__ tau, int K, float* __restrict__ tau_safe) {
-  const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (q >= n_q) return;
-  tau_select_one<E>(gmax, ng, J, tau, q, K, tau_safe);
+__global__ __launch_bounds__(256) void tau_select_kernel(const float* __restrict__ gmax, int n_q, int ng, int rank,
+                                                         float* __restrict__ out, const int32_t* __restrict__ list) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n_q) return;
+  tau_select_one<E>(gmax, ng, rank, out, list ? list[i] : i);
 }
 
 // ---------------------------------------------------------------------------
@@ -1835,12 +1831,13 @@ static int pow2_per_lane(int entries) {
   return e;
 }
 
-hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int J, float* tau, hipStream_t s, int K,
-                             float* tau_safe) {
+hipError_t launch_tau_select(const float* gmax, int n_q, int ng, int rank, float* out, hipStream_t s,
+                             const int32_t* list) {
+  if (n_q <= 0) return hipSuccess;
   const dim3 g((unsigned)((n_q + 3) / 4)), b(256);
   switch (pow2_per_lane(ng)) {
 #define EVOC_TAU_CASE(E) \
-  case E: hipLaunchKernelGGL(tau_select_kernel<E>, g, b, 0, s, gmax, n_q, ng, J, tau, K, tau_safe); break;
+  case E: hipLaunchKernelGGL(tau_select_kernel<E>, g, b, 0, s, gmax, n_q, ng, rank, out, list); break;
     EVOC_TAU_CASE(1)
     EVOC_TAU_CASE(2)
     EVOC_TAU_CASE(4)

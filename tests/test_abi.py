@@ -147,7 +147,7 @@ def _binom_tail(m, p, j):
 def test_bound_rank_is_the_smallest_rank_within_the_miss_budget(n, d, k, prec, monkeypatch):
     """The scoring pass tests the J-th largest group maximum of the strided sample.  It
     exceeds a query's K-th best score only if J sampled rows rank among its K - 1 best,
-    about Binomial(K - 1, 1 / stride): J is the smallest rank with that tail <= 1e-5
+    about Binomial(K - 1, 1 / stride): J is the smallest rank with that tail <= 1e-6
     (EVOC_MI355X_BOUND_EPS), checked here with exact rational arithmetic; eps = 1 gives
     the guaranteed rank K."""
     from fractions import Fraction
@@ -156,12 +156,12 @@ def test_bound_rank_is_the_smallest_rank_within_the_miss_budget(n, d, k, prec, m
     p = E.exact_plan(E.DTYPE_F32, n, d, o)
     K, st, J = p["candidates"], p["stride"], p["bound_rank"]
     assert st > 1 and 1 <= J <= K
-    eps = Fraction(1, 100000)
+    eps = Fraction(1, 1000000)
     assert _binom_tail(K - 1, Fraction(1, st), J) <= eps
     if J > 1:
         assert _binom_tail(K - 1, Fraction(1, st), J - 1) > eps
     if n == 1000000 and K == 46:
-        assert (st, J) == (31, 10)  # rank 46 -> 10: about 4.6x fewer appended keys
+        assert (st, J) == (31, 11)  # rank 46 -> 11: about 4x fewer appended keys
     monkeypatch.setenv("EVOC_MI355X_BOUND_EPS", "1")
     assert E.exact_plan(E.DTYPE_F32, n, d, o)["bound_rank"] == K
     monkeypatch.setenv("EVOC_MI355X_BOUND_EPS", "0.5")
