@@ -148,29 +148,37 @@ struct NndStats {
 };
 
 // Device random-projection forest (nnd_driver.cpp over nndescent.hip's level
-// kernels).  ForestScratch holds one tree's working set; forest_tree builds a
-// tree from a seed and leaves its row permutation and (start, length) leaves in
-// that scratch until the next tree.
+// kernels).  ForestScratch holds the working set of a forest; forest_build
+// builds one tree per seed, all of them level by level together, and leaves
+// their row permutations and (start, length) leaves in that scratch.
 struct ForestScratch {
-  DevBuf perm, perm2, seg_of, side, segs, nsegs, leaves, fcnt, hyper;
+  DevBuf perm, perm2, seg_of, side, segs, nsegs, leaves, fcnt, hyper, seeds, depth;
   int max_seg = 0;
+  int n_trees = 0;
   int64_t hyper_ld = 0;
-  std::vector<int32_t> iota;
 };
-struct ForestTree {
-  const int32_t* perm = nullptr;  // device, n rows in leaf order
-  const int2* leaves = nullptr;   // device, n_leaves (start, length) into perm, in no particular order
+// A forest built by forest_build: tree t's rows in leaf order at positions
+// [t n, t n + n) of perm; leaves are (flat start, length) into perm, in no
+// particular order, tree start / n.
+struct ForestSet {
+  const int32_t* perm = nullptr;  // device, n_trees x n
+  const int2* leaves = nullptr;   // device, n_leaves
   int n_leaves = 0;
-  int depth = 0;                  // levels split (the root is level 0)
+  std::vector<int32_t> depth;     // levels split per tree (the root is level 0)
 };
 namespace evoc {
 struct ForestArgs;
 }
-int forest_alloc(Ctx* ctx, ForestScratch& fs, int kind, int64_t n, int dim, int64_t row_bytes, int leaf_size);
+// scratch for up to n_trees trees of n rows built together (n_trees * n < 2^31)
+int forest_alloc(Ctx* ctx, ForestScratch& fs, int kind, int64_t n, int dim, int64_t row_bytes, int leaf_size,
+                 int n_trees);
 evoc::ForestArgs forest_args(const ForestScratch& fs, int kind, int64_t n, int dim, int leaf_size,
                              const uint8_t* rows, int64_t row_bytes, const float* frows, int64_t frow_ld);
-int forest_tree(Ctx* ctx, ForestScratch& fs, evoc::ForestArgs fa, int64_t max_depth, uint64_t seed,
-                ForestTree* out);
+// builds one tree per seed (at most fs.n_trees), every level of all trees in one set of launches
+int forest_build(Ctx* ctx, ForestScratch& fs, evoc::ForestArgs fa, int64_t max_depth,
+                 const std::vector<uint64_t>& seeds, ForestSet* out);
+// device bytes of forest_alloc
+uint64_t forest_bytes(int kind, int64_t n, int dim, int64_t row_bytes, int leaf_size, int n_trees);
 
 // NN-descent over all rows (nndescent.hip); out_keys: n x Kg exact-format keys.
 int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes, const uint32_t* pop,

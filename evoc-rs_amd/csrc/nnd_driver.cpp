@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <utility>
 #include <vector>
@@ -68,21 +69,38 @@ struct JoinTimer {
 
 }  // namespace
 
-int forest_alloc(Ctx* ctx, ForestScratch& fs, int kind, int64_t n, int dim, int64_t row_bytes, int leaf_size) {
-  fs.max_seg = (int)(n / (leaf_size + 1) + 2);
-  fs.hyper_ld = kind == EVOC_ELEM_U8BITS ? (row_bytes * 2 + 3) / 4 : dim;
+static int64_t forest_hyper_ld(int kind, int64_t row_bytes, int dim) {
+  // binary: positive and negative masks; else floats, padded to a multiple of 4
+  return kind == EVOC_ELEM_U8BITS ? (row_bytes * 2 + 3) / 4 : ((int64_t)dim + 3) / 4 * 4;
+}
+
+uint64_t forest_bytes(int kind, int64_t n, int dim, int64_t row_bytes, int leaf_size, int n_trees) {
+  auto r = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
+  const int64_t T = std::max(1, n_trees);
+  const int64_t max_seg = T * (n / (leaf_size + 1) + 2);
+  return 3 * r(T * n * 4) + r(T * n) + 2 * r(max_seg * 16) + r(T * (2 * n + 2) * 8) + 256 +
+         r(max_seg * forest_hyper_ld(kind, row_bytes, dim) * 4) + r(T * 8) + r(T * 4);
+}
+
+int forest_alloc(Ctx* ctx, ForestScratch& fs, int kind, int64_t n, int dim, int64_t row_bytes, int leaf_size,
+                 int n_trees) {
+  fs.n_trees = std::max(1, n_trees);
+  const int64_t T = fs.n_trees;
+  if (T * n > (int64_t)INT32_MAX - 1) return ctx->fail(EVOC_ERR_UNSUPPORTED, "forest: n_trees x rows >= 2^31");
+  fs.max_seg = (int)(T * (n / (leaf_size + 1) + 2));
+  fs.hyper_ld = forest_hyper_ld(kind, row_bytes, dim);
   int rc;
-  if ((rc = fs.perm.alloc(ctx, (size_t)n * 4, "forest perm")) || (rc = fs.perm2.alloc(ctx, (size_t)n * 4, "forest perm")) ||
-      (rc = fs.seg_of.alloc(ctx, (size_t)n * 4, "forest segments")) ||
-      (rc = fs.side.alloc(ctx, (size_t)n, "forest sides")) ||
+  if ((rc = fs.perm.alloc(ctx, (size_t)(T * n) * 4, "forest perm")) ||
+      (rc = fs.perm2.alloc(ctx, (size_t)(T * n) * 4, "forest perm")) ||
+      (rc = fs.seg_of.alloc(ctx, (size_t)(T * n) * 4, "forest segments")) ||
+      (rc = fs.side.alloc(ctx, (size_t)(T * n), "forest sides")) ||
       (rc = fs.segs.alloc(ctx, (size_t)fs.max_seg * 16, "forest segments")) ||
       (rc = fs.nsegs.alloc(ctx, (size_t)fs.max_seg * 16, "forest segments")) ||
-      (rc = fs.leaves.alloc(ctx, (size_t)(2 * n + 2) * 8, "forest leaves")) ||
+      (rc = fs.leaves.alloc(ctx, (size_t)(T * (2 * n + 2)) * 8, "forest leaves")) ||
       (rc = fs.fcnt.alloc(ctx, 16, "forest counters")) ||
-      (rc = fs.hyper.alloc(ctx, (size_t)fs.max_seg * fs.hyper_ld * 4, "hyperplanes")))
+      (rc = fs.hyper.alloc(ctx, (size_t)fs.max_seg * fs.hyper_ld * 4, "hyperplanes")) ||
+      (rc = fs.seeds.alloc(ctx, (size_t)T * 8, "tree seeds")) || (rc = fs.depth.alloc(ctx, (size_t)T * 4, "tree depths")))
     return rc;
-  fs.iota.resize((size_t)n);
-  for (int64_t i = 0; i < n; i++) fs.iota[(size_t)i] = (int32_t)i;
   return EVOC_OK;
 }
 
@@ -99,37 +117,48 @@ ForestArgs forest_args(const ForestScratch& fs, int kind, int64_t n, int dim, in
   fa.frow_ld = frow_ld;
   fa.hyper = (float*)fs.hyper.ptr;
   fa.hyper_ld = fs.hyper_ld;
+  fa.seeds = (const uint64_t*)fs.seeds.ptr;
+  fa.depth = (int32_t*)fs.depth.ptr;
+  fa.n_trees = fs.n_trees;
   return fa;
 }
 
-// One tree, level by level (float.rs:97-126): every segment longer than
-// leaf_size is split until max_depth levels have run; the leaves are
-// (start, length) ranges of the tree's row permutation.
-int forest_tree(Ctx* ctx, ForestScratch& fs, ForestArgs fa, int64_t max_depth, uint64_t seed, ForestTree* out) {
+// All trees level by level together (float.rs:97-126 per tree): every segment
+// longer than leaf_size is split until max_depth levels have run; the leaves
+// are (flat start, length) ranges of the trees' row permutations.  One set of
+// launches and one host synchronisation (the next level's segment count) per
+// level for the whole forest.
+int forest_build(Ctx* ctx, ForestScratch& fs, ForestArgs fa, int64_t max_depth, const std::vector<uint64_t>& seeds,
+                 ForestSet* out) {
   hipStream_t s = ctx->stream;
   Check ck{ctx};
   const int n = fa.n;
-  fa.seed = seed;
+  const int T = (int)seeds.size();
+  if (T < 1 || T > fs.n_trees) return ctx->fail(EVOC_ERR_INVALID_ARGUMENT, "forest: tree count");
+  fa.n_trees = T;
   int32_t* d_perm = (int32_t*)fs.perm.ptr;
   int32_t* d_perm2 = (int32_t*)fs.perm2.ptr;
   int4* d_segs = (int4*)fs.segs.ptr;
   int4* d_next = (int4*)fs.nsegs.ptr;
   int32_t* d_fc = (int32_t*)fs.fcnt.ptr;  // [0] next segments, [1] leaves
-  ck(upload_sync(d_perm, fs.iota.data(), (size_t)n * 4, s), "forest perm");
+  ck(upload_sync(fs.seeds.ptr, seeds.data(), (size_t)T * 8, s), "tree seeds");
+  ck(launch_fill_i32((int32_t*)fs.depth.ptr, T, 0, s), "tree depths");
   ck(launch_fill_i32(d_fc, 4, 0, s), "forest counters");
+  ck(rp_launch_init(d_perm, fa, s), "forest perm");
   int nseg;
   if (n > fa.leaf_size && max_depth > 0) {
-    const int4 root = make_int4(0, n, 0, 0);
-    ck(upload_sync(d_segs, &root, sizeof(root), s), "forest root");
-    nseg = 1;
+    std::vector<int4> roots((size_t)T);
+    for (int t = 0; t < T; t++) roots[(size_t)t] = make_int4(t * n, n, t, 0);
+    ck(upload_sync(d_segs, roots.data(), roots.size() * sizeof(int4), s), "forest roots");
+    nseg = T;
   } else {
-    const int2 leaf = make_int2(0, n);
-    const int32_t one = 1;
-    ck(upload_sync(fs.leaves.ptr, &leaf, sizeof(leaf), s), "forest leaf");
-    ck(upload_sync(d_fc + 1, &one, 4, s), "forest leaf");
+    std::vector<int2> lv((size_t)T);
+    for (int t = 0; t < T; t++) lv[(size_t)t] = make_int2(t * n, n);
+    const int32_t cnt = T;
+    ck(upload_sync(fs.leaves.ptr, lv.data(), lv.size() * sizeof(int2), s), "forest leaves");
+    ck(upload_sync(d_fc + 1, &cnt, 4, s), "forest leaves");
     nseg = 0;
   }
-  int depth = 0;
   for (int level = 0; nseg > 0 && ck.e == hipSuccess; level++) {
     const int last = level + 1 >= max_depth ? 1 : 0;
     ck(launch_fill_i32(d_fc, 1, 0, s), "forest counters");
@@ -142,16 +171,16 @@ int forest_tree(Ctx* ctx, ForestScratch& fs, ForestArgs fa, int64_t max_depth, u
     ck(hipMemcpyAsync(&nn, d_fc, 4, hipMemcpyDeviceToHost, s), "forest counters");
     ck(hipStreamSynchronize(s), "forest level");
     nseg = nn;
-    depth = level + 1;
   }
   int32_t nl = 0;
+  out->depth.assign((size_t)T, 0);
   ck(hipMemcpyAsync(&nl, d_fc + 1, 4, hipMemcpyDeviceToHost, s), "forest leaves");
+  ck(hipMemcpyAsync(out->depth.data(), fs.depth.ptr, (size_t)T * 4, hipMemcpyDeviceToHost, s), "tree depths");
   ck(hipStreamSynchronize(s), "forest");
   if (ck.e != hipSuccess) return ck.fail();
   out->perm = d_perm;
   out->leaves = (const int2*)fs.leaves.ptr;
   out->n_leaves = nl;
-  out->depth = depth;
   return EVOC_OK;
 }
 
@@ -163,9 +192,7 @@ uint64_t nndescent_bytes(int kind, int64_t n, int Kg, int max_candidates, int64_
   const int64_t nb = (n + 1023) / 1024;
   uint64_t b = r(n * Kg * 8) + 2 * r(n * 4) + r(n * UCAP * 8) + 2 * r(n * mc * 4) + 2 * r(n * 4) +
                r((n + 1) * 4) + r(n * Kg * 4) + r((nb + 2) * 4) + 2 * 256;
-  const int64_t max_seg = n / (ls + 1) + 2;
-  const int64_t hyper_ld = kind == EVOC_ELEM_U8BITS ? (row_bytes * 2 + 3) / 4 : dim;
-  b += 3 * r(n * 4) + r(n) + 2 * r(max_seg * 16) + r((2 * n + 2) * 8) + 256 + r(max_seg * hyper_ld * 4);
+  b += forest_bytes(kind, n, dim, row_bytes, (int)ls, NND_FOREST_TREES);
   return b;
 }
 
@@ -230,22 +257,34 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
   };
 
   // ---- random-projection forest: leaves seed the lists (float.rs:243-348) ----
+  // All trees are split together (one set of launches per level); their leaves
+  // are joined two trees at a time, with a merge after each pair so the later
+  // trees' offers meet the lists the earlier ones left.
   if (P.n_trees > 0 && n > 1 && ck.e == hipSuccess) {
-    ForestScratch fs;
-    if ((rc = forest_alloc(ctx, fs, kind, n, dim, row_bytes, leaf_size))) return rc;
-    ForestArgs fa = forest_args(fs, kind, n, dim, leaf_size, rows, row_bytes, frows, frow_ld);
-    for (int t = 0; t < P.n_trees && ck.e == hipSuccess; t++) {
-      ForestTree tr;
-      if ((rc = forest_tree(ctx, fs, fa, P.max_depth, mix(P.tree_seeds[(size_t)t]), &tr))) return rc;
-      JoinArgs jl = ja;
-      jl.mode = 1;
-      jl.n_items = tr.n_leaves;
-      jl.leaves = tr.leaves;
-      jl.perm = tr.perm;
-      jl.salt = 1;
-      if (tr.n_leaves > 0) join(jl, "leaf join");
-      merge_all();
-      if (st) st->leaves += tr.n_leaves;
+    const int T = std::min<int>(P.n_trees, NND_FOREST_TREES);
+    for (int t0 = 0; t0 < P.n_trees && ck.e == hipSuccess; t0 += T) {
+      const int nt = std::min(T, P.n_trees - t0);
+      ForestScratch fs;
+      if ((rc = forest_alloc(ctx, fs, kind, n, dim, row_bytes, leaf_size, nt))) return rc;
+      ForestArgs fa = forest_args(fs, kind, n, dim, leaf_size, rows, row_bytes, frows, frow_ld);
+      std::vector<uint64_t> seeds;
+      for (int t = 0; t < nt; t++) seeds.push_back(mix(P.tree_seeds[(size_t)(t0 + t)]));
+      ForestSet fo;
+      if ((rc = forest_build(ctx, fs, fa, P.max_depth, seeds, &fo))) return rc;
+      for (int g = 0; g < nt && fo.n_leaves > 0; g += 2) {
+        JoinArgs jl = ja;
+        jl.mode = 1;
+        jl.n_items = fo.n_leaves;
+        jl.leaves = fo.leaves;
+        jl.perm = fo.perm;
+        jl.tree_n = n;
+        jl.tree_lo = g;
+        jl.tree_hi = std::min(nt, g + 2);
+        jl.salt = 1 + (uint32_t)(t0 + g);
+        join(jl, "leaf join");
+        merge_all();
+      }
+      if (st) st->leaves += fo.n_leaves;
     }
   }
 
@@ -267,6 +306,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
   double prev_sum = 0.0;
   bool have_prev = false;
   int64_t block = std::max<int64_t>(std::min<int64_t>(n, 16384), n / 8);
+  if (const char* e = std::getenv("EVOC_MI355X_NND_BLOCK")) block = std::max<int64_t>(1024, std::atoll(e));
   int it = 0;
   for (; it < P.n_iters && ck.e == hipSuccess; it++) {
     ck(launch_fill_i32((int32_t*)d_changed, 1, 0, s), "counters");
@@ -384,34 +424,48 @@ extern "C" int evoc_mi355x_make_forest(evoc_mi355x_ctx* ctx, const evoc_knn_inpu
     e = launch_pad_rows(d_src, in->ld, n, (int)d, (uint8_t*)rows.ptr, row_bytes, s);
   }
   if (e != hipSuccess) return ctx->hip_fail(e, "input preparation");
-  // each tree: its leaves in the order of their first row in the tree's permutation
   std::vector<std::vector<int32_t>> tree_rows((size_t)n_trees);  // concatenated leaf members
   std::vector<std::vector<int2>> tree_lv((size_t)n_trees);
   int64_t total = 0, width = leaf_size;
-  if (n_trees > 0) {
+  for (int64_t t0 = 0; t0 < n_trees; t0 += NND_FOREST_TREES) {
+    const int nt = (int)std::min<int64_t>(NND_FOREST_TREES, n_trees - t0);
     ForestScratch fs;
-    if ((rc = forest_alloc(ctx, fs, kind, n, (int)d, row_bytes, leaf_size))) return rc;
+    if ((rc = forest_alloc(ctx, fs, kind, n, (int)d, row_bytes, leaf_size, nt))) return rc;
     ForestArgs fa = forest_args(fs, kind, n, (int)d, leaf_size, (const uint8_t*)rows.ptr, row_bytes,
                                 f32 ? (const float*)rows.ptr : nullptr, f32_ld);
-    for (int64_t t = 0; t < n_trees; t++) {
-      // the tree's seed from its three draws, as NN-descent mode takes it
-      const uint64_t seed = ((uint64_t)tree_states[3 * t] << 40) ^ ((uint64_t)tree_states[3 * t + 1] << 20) ^
-                            (uint64_t)tree_states[3 * t + 2];
-      ForestTree tr;
-      if ((rc = forest_tree(ctx, fs, fa, max_depth, mix(seed), &tr))) return rc;
-      std::vector<int2>& lv = tree_lv[(size_t)t];
-      lv.resize((size_t)tr.n_leaves);
-      tree_rows[(size_t)t].resize((size_t)n);
-      e = hipMemcpyAsync(lv.data(), tr.leaves, lv.size() * sizeof(int2), hipMemcpyDeviceToHost, s);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(tree_rows[(size_t)t].data(), tr.perm, (size_t)n * 4, hipMemcpyDeviceToHost, s);
-      if (e == hipSuccess) e = hipStreamSynchronize(s);
-      if (e != hipSuccess) return ctx->hip_fail(e, "forest download");
+    // each tree's seed from its three draws, as NN-descent mode takes it
+    std::vector<uint64_t> seeds;
+    for (int t = 0; t < nt; t++) {
+      const int64_t tt = t0 + t;
+      seeds.push_back(mix(((uint64_t)tree_states[3 * tt] << 40) ^ ((uint64_t)tree_states[3 * tt + 1] << 20) ^
+                          (uint64_t)tree_states[3 * tt + 2]));
+    }
+    ForestSet fo;
+    if ((rc = forest_build(ctx, fs, fa, max_depth, seeds, &fo))) return rc;
+    std::vector<int2> all((size_t)fo.n_leaves);
+    std::vector<int32_t> perm((size_t)(nt * n));
+    e = hipMemcpyAsync(all.data(), fo.leaves, all.size() * sizeof(int2), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(perm.data(), fo.perm, perm.size() * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return ctx->hip_fail(e, "forest download");
+    for (int t = 0; t < nt; t++) {
+      const int64_t tt = t0 + t;
+      tree_rows[(size_t)tt].assign(perm.begin() + (int64_t)t * n, perm.begin() + (int64_t)(t + 1) * n);
+    }
+    // leaves of tree t, starts made tree-local
+    for (const int2& l : all) {
+      const int t = (int)(l.x / n);
+      tree_lv[(size_t)(t0 + t)].push_back(make_int2((int)(l.x - (int64_t)t * n), l.y));
+    }
+    for (int t = 0; t < nt; t++) {
+      const int64_t tt = t0 + t;
+      std::vector<int2>& lv = tree_lv[(size_t)tt];
+      // each tree: its leaves in the order of their first row in the tree's permutation
       std::sort(lv.begin(), lv.end(), [](const int2& a, const int2& b) { return a.x < b.x; });
       for (const int2& l : lv) width = std::max<int64_t>(width, l.y);
       total += (int64_t)lv.size();
-      if (tree_leaves) tree_leaves[t] = (int64_t)lv.size();
-      if (depths) depths[t] = tr.depth;
+      if (tree_leaves) tree_leaves[tt] = (int64_t)lv.size();
+      if (depths) depths[tt] = fo.depth[(size_t)t];
     }
   }
   *rows_out = total;

@@ -12,6 +12,10 @@ namespace evoc {
 // the arrival order, and repeated offers of one pair land on the same slots.
 constexpr int NND_OFFER_CAP = 448;
 
+// Trees of the random-projection forest split together (one set of launches
+// per level for all of them); a larger forest is built in groups of this many.
+constexpr int NND_FOREST_TREES = 8;
+
 struct JoinArgs {
   int32_t mode;     // 0: candidate lists of vertices v0.., 1: forest leaves
   int32_t n_items;  // work items (one wave each)
@@ -21,6 +25,9 @@ struct JoinArgs {
   const int32_t* oldc;  // [n][mc]
   const int2* leaves;   // mode 1: (start, len) into perm, len <= 64
   const int32_t* perm;
+  int32_t tree_n;       // mode 1: rows per tree (leaf of tree start / tree_n); 0: no filter
+  int32_t tree_lo, tree_hi;  // mode 1: only leaves of trees [tree_lo, tree_hi)
+  int32_t pad1;
   const uint8_t* rows;  // scored rows, row_bytes apart (multiple of 128)
   int64_t row_bytes;
   const uint32_t* row_pop;  // binary rows
@@ -34,18 +41,25 @@ struct JoinArgs {
   unsigned long long* bytes;  // algorithmic bytes gathered: each item's candidate rows once (may be null)
 };
 
+// The random-projection forest of n_trees trees, built level by level for all
+// trees at once: tree t's rows occupy positions [t n, t n + n) of the flat
+// permutation, a segment is (flat start, length, tree, 0) and a leaf (flat
+// start, length).
 struct ForestArgs {
   int32_t kind;
-  int32_t n;
+  int32_t n;          // rows (per tree)
   int32_t dim;        // elements per row (f32 / i8)
   int32_t leaf_size;
-  const uint8_t* rows;   // i8 / binary rows
+  const uint8_t* rows;   // i8 / binary / BF16 rows, row_bytes apart
   int64_t row_bytes;
   const float* frows;    // prepared f32 rows (f32 data)
   int64_t frow_ld;
   float* hyper;          // [segments][hyper_ld]
   int64_t hyper_ld;
-  uint64_t seed;
+  const uint64_t* seeds;  // [n_trees] per-tree seeds (device)
+  int32_t* depth;         // [n_trees] levels split per tree (device)
+  int32_t n_trees;
+  int32_t pad0;
 };
 
 hipError_t nnd_launch_init(uint64_t* keys, float* thr, int32_t* umark, int n, int Kg, hipStream_t s);
@@ -57,6 +71,8 @@ hipError_t nnd_launch_candidates(uint64_t* keys, int n, int Kg, int32_t* revcnt,
                                  uint64_t seed, int it, int32_t* newc, int32_t* oldc, hipStream_t s);
 hipError_t nnd_launch_export(const uint64_t* keys, int n, int Kg, uint64_t* out, hipStream_t s);
 hipError_t nnd_launch_sum_dist(const uint64_t* keys, int64_t m, double* out, hipStream_t s);
+// perm[t n + i] = i for every tree t
+hipError_t rp_launch_init(int32_t* perm, const ForestArgs& f, hipStream_t s);
 hipError_t rp_launch_level(const int4* segs, int nseg, const int32_t* perm, int32_t* perm2,
                            int32_t* seg_of, int8_t* side, const ForestArgs& f, int level,
                            int4* next_segs, int32_t* n_next, int2* leaves, int32_t* n_leaves,

@@ -102,9 +102,11 @@ __device__ __forceinline__ void load_item(const JoinArgs& a, int item, JoinShare
       ov = a.oldc[(int64_t)v * a.mc + lane];
     }
   } else {
-    // forest leaf: members perm[start .. start + len), len <= 64
+    // forest leaf: members perm[start .. start + len), len <= 64 (only the
+    // leaves of trees [tree_lo, tree_hi) when a tree filter is given)
     const int2 lf = a.leaves[item];
-    if (lane < lf.y) nv = a.perm[lf.x + lane];
+    const int tr = a.tree_n > 0 ? lf.x / a.tree_n : 0;
+    if (lane < lf.y && (a.tree_n == 0 || (tr >= a.tree_lo && tr < a.tree_hi))) nv = a.perm[lf.x + lane];
   }
   const uint64_t mn = __ballot(nv >= 0), mo = __ballot(ov >= 0);
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -252,32 +254,19 @@ __global__ __launch_bounds__(256) void nnd_join_jaccard_kernel(JoinArgs a) {
 // collapse onto the copy already in the list (flagged_heap_push rejects a
 // present index, src/heap.rs:156-160), the first Kg survive.  New entries
 // carry flag 1.  The slots are emptied for the next round of offers.
+// The list and the non-empty slots are first compacted through the wave's LDS
+// stash, so the wave sort runs over as many registers as they fill (one
+// register per 64 keys), not over every slot.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void nnd_merge_kernel(uint64_t* __restrict__ keys,
-                                                        float* __restrict__ thr,
-                                                        int32_t* __restrict__ umark,
-                                                        uint64_t* __restrict__ ubuf, int ucap,
-                                                        int n, int Kg, int v0, int nv,
-                                                        uint32_t* __restrict__ changed) {
-  constexpr int E = 1 + NND_OFFER_CAP / 64;  // list, then the offers
-  static_assert(NND_OFFER_CAP % 64 == 0, "offer buffers fill whole registers of the wave sort");
+template <int E>
+__device__ __forceinline__ void merge_sorted(const uint64_t* st, int cnt, uint64_t* __restrict__ keys,
+                                             float* __restrict__ thr, int v, int Kg, int& inserted) {
   const int lane = threadIdx.x & 63;
-  const int vi = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (vi >= nv) return;
-  const int v = v0 + vi;
-  if (umark[v] == 0) return;
   uint64_t x[E];
-  x[0] = lane < Kg ? (keys[(int64_t)v * Kg + lane] | (keys[(int64_t)v * Kg + lane] ? 2ull : 0ull))
-                   : EVOC_KEY_NONE;
 #pragma unroll
-  for (int e = 1; e < E; e++) {
-    const int u = (e - 1) * 64 + lane;
-    x[e] = u < ucap ? ubuf[(int64_t)v * ucap + u] : EVOC_KEY_NONE;
-  }
-#pragma unroll
-  for (int e = 1; e < E; e++) {
-    const int u = (e - 1) * 64 + lane;
-    if (u < ucap) ubuf[(int64_t)v * ucap + u] = EVOC_KEY_NONE;
+  for (int e = 0; e < E; e++) {
+    const int i = e * 64 + lane;
+    x[e] = i < cnt ? st[i] : EVOC_KEY_NONE;
   }
   wave_sort_desc<E>(x);
   // keep[i]: non-empty and not a repeat of element i - 1 (same score and index)
@@ -292,7 +281,6 @@ __global__ __launch_bounds__(256) void nnd_merge_kernel(uint64_t* __restrict__ k
   }
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int base = 0;
-  int inserted = 0;
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const uint64_t m = __ballot(keep[e]);
@@ -307,6 +295,50 @@ __global__ __launch_bounds__(256) void nnd_merge_kernel(uint64_t* __restrict__ k
   // slots beyond the kept entries stay empty
   for (int pos = base + lane; pos < Kg; pos += 64) keys[(int64_t)v * Kg + pos] = EVOC_KEY_NONE;
   if (base < Kg && lane == 0) thr[v] = -INFINITY;
+}
+
+__global__ __launch_bounds__(256) void nnd_merge_kernel(uint64_t* __restrict__ keys,
+                                                        float* __restrict__ thr,
+                                                        int32_t* __restrict__ umark,
+                                                        uint64_t* __restrict__ ubuf, int ucap,
+                                                        int n, int Kg, int v0, int nv,
+                                                        uint32_t* __restrict__ changed) {
+  constexpr int SLOTS = NND_OFFER_CAP / 64;  // registers of offer slots
+  static_assert(NND_OFFER_CAP % 64 == 0, "offer buffers fill whole registers of the wave sort");
+  __shared__ uint64_t stash[4][64 * (1 + SLOTS)];
+  const int lane = threadIdx.x & 63;
+  const int vi = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (vi >= nv) return;
+  const int v = v0 + vi;
+  if (umark[v] == 0) return;
+  uint64_t* st = stash[threadIdx.x >> 6];
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // the list (from_list bit set, so a listed neighbour sorts before its offers)
+  const uint64_t lk = lane < Kg ? keys[(int64_t)v * Kg + lane] : EVOC_KEY_NONE;
+  const uint64_t l = lk | (lk ? 2ull : 0ull);
+  const uint64_t ml = __ballot(l != EVOC_KEY_NONE);
+  if (l != EVOC_KEY_NONE) st[__popcll(ml & below)] = l;
+  int cnt = __popcll(ml);
+  // then the non-empty offer slots, each emptied for the next round
+#pragma unroll
+  for (int e = 0; e < SLOTS; e++) {
+    const int u = e * 64 + lane;
+    const uint64_t o = u < ucap ? ubuf[(int64_t)v * ucap + u] : EVOC_KEY_NONE;
+    const uint64_t mo = __ballot(o != EVOC_KEY_NONE);
+    if (o != EVOC_KEY_NONE) {
+      st[cnt + __popcll(mo & below)] = o;
+      ubuf[(int64_t)v * ucap + u] = EVOC_KEY_NONE;
+    }
+    cnt += __popcll(mo);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int inserted = 0;
+  if (cnt <= 64) merge_sorted<1>(st, cnt, keys, thr, v, Kg, inserted);
+  else if (cnt <= 128) merge_sorted<2>(st, cnt, keys, thr, v, Kg, inserted);
+  else if (cnt <= 256) merge_sorted<4>(st, cnt, keys, thr, v, Kg, inserted);
+  else merge_sorted<1 + SLOTS>(st, cnt, keys, thr, v, Kg, inserted);
   inserted = wave_sum_i32(inserted);
   if (lane == 0) {
     atomicAdd(changed, (uint32_t)inserted);
@@ -463,8 +495,17 @@ __global__ void nnd_random_lists_kernel(const uint64_t* __restrict__ keys, int n
 // ---------------------------------------------------------------------------
 // Level-synchronous random-projection forest (float.rs:13-177 semantics:
 // hyperplane between two random members, margin sign, random side when
-// |margin| < 1e-8, random halves when one side is empty).
+// |margin| < 1e-8, random halves when one side is empty), all trees of the
+// forest split together: one set of launches per level instead of one per
+// tree and level.  Tree t's rows occupy positions [t n, t n + n) of the flat
+// permutation; random choices hash the tree's seed with the level and the
+// position within the tree, so a tree comes out the same alone or with others.
 // ---------------------------------------------------------------------------
+__global__ void rp_init_perm_kernel(int32_t* __restrict__ perm, int n, int64_t total) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total) perm[i] = (int32_t)(i % n);
+}
+
 __global__ void rp_seg_fill_kernel(const int4* __restrict__ segs, int nseg, int32_t* __restrict__ seg_of) {
   const int s = blockIdx.x;
   if (s >= nseg) return;
@@ -480,7 +521,8 @@ __global__ __launch_bounds__(256) void rp_hyperplane_kernel(const int4* __restri
   const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (s >= nseg) return;
   const int4 g = segs[s];
-  const uint64_t h = mix64(f.seed ^ ((uint64_t)level << 40) ^ (uint64_t)(uint32_t)g.x);
+  const int local = g.x - g.z * f.n;  // start within the tree
+  const uint64_t h = mix64(f.seeds[g.z] ^ ((uint64_t)level << 40) ^ (uint64_t)(uint32_t)local);
   const int li = (int)(h % (uint64_t)g.y);
   int ri = (int)((h >> 32) % (uint64_t)g.y);
   ri += (li == ri);
@@ -526,13 +568,13 @@ __global__ __launch_bounds__(256) void rp_hyperplane_kernel(const int4* __restri
   for (int i = lane; i < f.dim; i += 64) hv[i] = hv[i] / hn;
 }
 
-// one wave per position of an active segment: side (0 left, 1 right)
+// one wave per position of an active segment (over all trees): side (0 left, 1 right)
 __global__ __launch_bounds__(256) void rp_side_kernel(const int32_t* __restrict__ perm,
                                                       const int32_t* __restrict__ seg_of,
-                                                      const int4* __restrict__ segs, int n_pos,
+                                                      const int4* __restrict__ segs, int64_t n_pos,
                                                       ForestArgs f, int level, int8_t* __restrict__ side) {
   const int lane = threadIdx.x & 63;
-  const int pos = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t pos = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (pos >= n_pos) return;
   const int s = seg_of[pos];
   if (s < 0) return;
@@ -549,6 +591,18 @@ __global__ __launch_bounds__(256) void rp_side_kernel(const int32_t* __restrict_
     const int8_t* x = (const int8_t*)(f.rows + (int64_t)p * f.row_bytes);
     for (int i = lane; i < f.dim; i += 64) m = fmaf(hv[i], (float)x[i], m);
     m = wave_sum_f32(m);
+  } else if (f.kind == EVOC_ELEM_BF16) {
+    // the scored BF16 copy of the row (half the bytes of the FP32 rows), two
+    // elements per lane and load; the rows are zero padded to row_bytes
+    const uint32_t* x = (const uint32_t*)(f.rows + (int64_t)p * f.row_bytes);
+    const float2* h2 = (const float2*)hv;
+    for (int i = lane; 2 * i < f.dim; i += 64) {
+      const uint32_t u = x[i];
+      const float2 hh = h2[i];
+      m = fmaf(hh.x, __uint_as_float(u << 16), m);
+      if (2 * i + 1 < f.dim) m = fmaf(hh.y, __uint_as_float(u & 0xffff0000u), m);
+    }
+    m = wave_sum_f32(m);
   } else {
     const float* x = (const float*)(f.frows + (int64_t)p * f.frow_ld);
     for (int i = lane; i < f.dim; i += 64) m = fmaf(hv[i], x[i], m);
@@ -556,10 +610,13 @@ __global__ __launch_bounds__(256) void rp_side_kernel(const int32_t* __restrict_
   }
   if (lane == 0) {
     int8_t sd;
-    if (fabsf(m) < 1e-8f)
-      sd = (int8_t)(mix64(f.seed ^ ((uint64_t)level << 40) ^ ((uint64_t)(uint32_t)p << 1) ^ 0x5bd1e995ull) & 1ull);
-    else
+    if (fabsf(m) < 1e-8f) {
+      const int t = (int)(pos / f.n);
+      sd = (int8_t)(mix64(f.seeds[t] ^ ((uint64_t)level << 40) ^ ((uint64_t)(uint32_t)p << 1) ^ 0x5bd1e995ull) &
+                    1ull);
+    } else {
       sd = m > 0.0f ? 0 : 1;
+    }
     side[pos] = sd;
   }
 }
@@ -581,6 +638,7 @@ __global__ __launch_bounds__(256) void rp_partition_kernel(const int4* __restric
   const int s = blockIdx.x;
   if (s >= nseg) return;
   const int4 g = segs[s];
+  const int local = g.x - g.z * f.n;  // start within the tree
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   for (int attempt = 0; attempt < 2; attempt++) {
     // count the left side
@@ -595,7 +653,7 @@ __global__ __launch_bounds__(256) void rp_partition_kernel(const int4* __restric
     if (nl > 0 && nl < g.y) break;
     // one side empty: random halves (float.rs:77-90)
     for (int i = t; i < g.y; i += 256)
-      side[g.x + i] = (int8_t)(mix64(f.seed ^ ((uint64_t)level << 40) ^ ((uint64_t)(uint32_t)(g.x + i) << 1) ^
+      side[g.x + i] = (int8_t)(mix64(f.seeds[g.z] ^ ((uint64_t)level << 40) ^ ((uint64_t)(uint32_t)(local + i) << 1) ^
                                      0x2545f4914f6cdd1dull) & 1ull);
     __syncthreads();
   }
@@ -625,11 +683,12 @@ __global__ __launch_bounds__(256) void rp_partition_kernel(const int4* __restric
     __syncthreads();
   }
   if (t == 0) {
+    atomicMax(&f.depth[g.z], level + 1);
     const int2 kids[2] = {make_int2(g.x, nl), make_int2(g.x + nl, g.y - nl)};
     for (int k = 0; k < 2; k++) {
       if (kids[k].y > f.leaf_size && !last_level) {
         const int slot = atomicAdd(n_next, 1);
-        next_segs[slot] = make_int4(kids[k].x, kids[k].y, 0, 0);
+        next_segs[slot] = make_int4(kids[k].x, kids[k].y, g.z, 0);
       } else {
         const int slot = atomicAdd(n_leaves, 1);
         leaves[slot] = kids[k];
@@ -640,8 +699,8 @@ __global__ __launch_bounds__(256) void rp_partition_kernel(const int4* __restric
 
 // positions outside active segments keep their order
 __global__ void rp_copy_kernel(const int32_t* __restrict__ perm, int32_t* __restrict__ perm2,
-                               const int32_t* __restrict__ seg_of, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                               const int32_t* __restrict__ seg_of, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && seg_of[i] < 0) perm2[i] = perm[i];
 }
 
@@ -784,15 +843,22 @@ hipError_t nnd_launch_sum_dist(const uint64_t* keys, int64_t m, double* out, hip
   return hipGetLastError();
 }
 
+hipError_t rp_launch_init(int32_t* perm, const ForestArgs& f, hipStream_t s) {
+  const int64_t total = (int64_t)f.n * f.n_trees;
+  hipLaunchKernelGGL(rp_init_perm_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, perm, f.n, total);
+  return hipGetLastError();
+}
+
 hipError_t rp_launch_level(const int4* segs, int nseg, const int32_t* perm, int32_t* perm2, int32_t* seg_of,
                            int8_t* side, const ForestArgs& f, int level, int4* next_segs, int32_t* n_next,
                            int2* leaves, int32_t* n_leaves, int last_level, hipStream_t s) {
-  hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(f.n, 256)), dim3(256), 0, s, seg_of, (int64_t)f.n, -1);
+  const int64_t total = (int64_t)f.n * f.n_trees;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, seg_of, total, -1);
   hipLaunchKernelGGL(rp_seg_fill_kernel, dim3(nseg), dim3(256), 0, s, segs, nseg, seg_of);
   hipLaunchKernelGGL(rp_hyperplane_kernel, dim3(blocks_for(nseg, 4)), dim3(256), 0, s, segs, nseg, perm, f, level);
-  hipLaunchKernelGGL(rp_side_kernel, dim3(blocks_for(f.n, 4)), dim3(256), 0, s, perm, seg_of, segs, f.n, f, level,
-                     side);
-  hipLaunchKernelGGL(rp_copy_kernel, dim3(blocks_for(f.n, 256)), dim3(256), 0, s, perm, perm2, seg_of, f.n);
+  hipLaunchKernelGGL(rp_side_kernel, dim3(blocks_for(total, 4)), dim3(256), 0, s, perm, seg_of, segs, total, f,
+                     level, side);
+  hipLaunchKernelGGL(rp_copy_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, perm, perm2, seg_of, total);
   hipLaunchKernelGGL(rp_partition_kernel, dim3(nseg), dim3(256), 0, s, segs, nseg, perm, perm2, side, f, level,
                      next_segs, n_next, leaves, n_leaves, last_level);
   return hipGetLastError();
