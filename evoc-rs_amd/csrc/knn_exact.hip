@@ -936,6 +936,16 @@ __device__ __forceinline__ float jac_score(uint32_t inter, uint32_t pa, uint32_t
   return uni > 0 ? __fdiv_rn((float)inter, (float)uni) : 0.0f;
 }
 
+// Whether the Jaccard score inter / uni (IEEE quotient, uni counts below 2^24)
+// may reach the bound tau, without the division: false only when
+// inter < fl(fl(tau uni) (1 - 2^-20)), i.e. inter < tau uni (1 - 2^-21), where
+// the rounded quotient is below tau.  Bounds <= 0, -inf and NaN products never
+// rule a pair out; +inf rules every pair out.  The exact test follows for the
+// pairs it lets through.
+__device__ __forceinline__ bool jac_may_reach(uint32_t inter, uint32_t uni, float tau) {
+  return !((float)inter < tau * (float)uni * (1.0f - 0x1p-20f));
+}
+
 __global__ __launch_bounds__(256) void jac_pass1_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   uint32_t inter[8][8];
@@ -1005,6 +1015,7 @@ __global__ __launch_bounds__(256) void jac_pass2_kernel(TileArgs a) {
         const int q = q0 + ql;
         if (q >= a.n_q) continue;
         const bool self = (c == q + a.q_offset);
+        if (!self && !jac_may_reach(inter[i][j], pc + pq[j] - inter[i][j], tau[j])) continue;
         const float s = self ? 1.0f : jac_score(inter[i][j], pc, pq[j]);
         bool take = s >= tau[j] && !(self && a.skip_self);
         if (take && a.tau_key) take = make_key(s, c, self) >= tkey[j];
@@ -1078,6 +1089,9 @@ __global__ __launch_bounds__(256) void jac_pass2_sym_kernel(TileArgs a) {
         const int q = q0 + ql;
         if (q >= a.n_q) continue;
         const bool self = (c == q);
+        const uint32_t uni = pc + pq[j] - inter[i][j];
+        // most pairs reach neither row's bound: they skip the division
+        if (!self && !jac_may_reach(inter[i][j], uni, tau[j]) && !jac_may_reach(inter[i][j], uni, ctau)) continue;
         const float s = self ? 1.0f : jac_score(inter[i][j], pc, pq[j]);
         bool take = s >= tau[j] && !(self && a.skip_self);
         if (take && key_bound) take = make_key(s, c, self) >= tkey[j];
