@@ -179,6 +179,7 @@ def serving_bench(torch, E, n_req, rate, slots, budget_gb, seed=42):
     for t in tickets:
         t.wait(timeout=300.0)
     makespan = time.perf_counter() - t0
+    batching = srv.batch_stats()
     srv.close()
     lat = np.array([t.stats.queue_seconds + t.stats.seconds for t in tickets]) * 1e3
     queue = np.array([t.stats.queue_seconds for t in tickets]) * 1e3
@@ -190,6 +191,8 @@ def serving_bench(torch, E, n_req, rate, slots, budget_gb, seed=42):
             "makespan_s": makespan, "requests_per_s": n_req / makespan, "points_per_s": pts / makespan,
             "scoring_tflops": flops / makespan / 1e12,
             "mfma_frac": flops / makespan / 1e12 / PEAK["bf16"],
+            "batch_lane": {"batches": batching["batches"], "requests_batched": batching["requests"],
+                           "largest_batch": batching["largest"], "graph_replays": batching["graph_replays"]},
             "workload": "Poisson arrivals; n log-uniform [1e3, 2e5]; d in {384, 768, 1024}; "
                         "k in {10, 15, 30}; Gaussian blobs (10/32/100 centres, std 0.25), L2-normalised, "
                         "device-resident; exact BF16 + FP32 re-score"}

@@ -129,7 +129,8 @@ int bound_rank(int K, int stride) {
   return J;
 }
 
-Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu, bool symmetric = false) {
+Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu, bool symmetric = false,
+               bool guaranteed = false) {
   Plan p;
   p.K = K;
   p.J = K;
@@ -163,7 +164,7 @@ Plan make_plan(int kind, int64_t n, int64_t nq, int K, int n_cu, bool symmetric 
     }
     stride++;
   }
-  p.J = bound_rank(K, p.stride);
+  p.J = guaranteed ? K : bound_rank(K, p.stride);
   // a query's bound ranks near its J-th .. 2J-th sample group, i.e. about
   // stride * 2J candidates of the full set pass it; keep 4x headroom
   const int64_t expect = (int64_t)p.stride * 2 * p.J;
@@ -545,7 +546,9 @@ int exact_plan_info(int kind, int64_t n, int64_t row_bytes, int K, int n_cu, boo
 }
 
 SlabPlan exact_slab_plan(int kind, int64_t n, int K, int n_cu) {
-  const Plan p = make_plan(kind, n, n, K, n_cu);
+  // the batched search tests the guaranteed bound (the K-th group maximum):
+  // its buffers are sized for that bound's survivors
+  const Plan p = make_plan(kind, n, n, K, n_cu, false, true);
   SlabPlan o;
   o.n_splits = p.n_splits;
   o.tiles_per_split = p.tiles_per_split;
