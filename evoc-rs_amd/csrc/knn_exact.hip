@@ -36,20 +36,24 @@ namespace evoc {
 // MFMA tile engine.  WM x WN waves, each owning BM x BN MFMA 32x32 blocks, so a
 // workgroup tile is (32 BM WM) candidates x (32 BN WN) queries.  Shapes (see
 // tile_shape()), LDS = stage ring + pass-2 per-query arrays:
-//   <2,2> x (2,2)  128 x 128, 64 accumulators per lane, 64 KiB: two workgroups per CU
-//   <2,4> x (2,2)  128 x 256, 128 accumulators, 96 KiB
-//   <2,4> x (4,2)  256 x 256, 8 waves of 64 x 128, 128 KiB: two waves per SIMD
-//   <4,4> x (2,2)  256 x 256, 4 waves of 128 x 128, 256 accumulators in AGPRs,
-//                  128 KiB: a third less LDS operand traffic per MFMA than 64 x 128
+//   0: <2,2> x (2,2)  128 x 128, 64 accumulators per lane, 64 KiB: two workgroups per CU
+//   1: <2,4> x (2,2)  128 x 256, 128 accumulators, a 3-stage ring of 72 KiB: two per CU
+//   2: <2,4> x (4,2)  256 x 256, 8 waves of 64 x 128 (two per SIMD), 128-byte stages in
+//                     a 2-stage ring of 128 KiB (the default: half the stage barriers
+//                     and load groups of 64-byte stages, 4-5 % faster)
+//   3: <4,4> x (2,2)  256 x 256, 4 waves of 128 x 128, 256 accumulators in AGPRs,
+//                     128 KiB: a third less LDS operand traffic per MFMA than 64 x 128
+//   4: as 2 with 64-byte stages in a 4-stage ring (EVOC_MI355X_TILE=4 only)
 // MFMA orientation: M (rows) = candidates (A operand), N (cols) = queries
 // (B operand), so lane l of a 32x32 block holds query column l % 32 and the 16
 // candidate rows 8*(j/4) + 4*(l/32) + j%4, j = 0..15.
 // ---------------------------------------------------------------------------
 constexpr int STAGE_BYTES = 128;     // row bytes per stage of the Jaccard tiles
-constexpr int MFMA_SB = 64;          // row bytes per stage of the MFMA tiles
-// Stages in flight (LDS ring buffers): four, three for the 128 x 256 tile,
-// whose ring then fits two workgroups in a CU's 160 KiB.
-constexpr int mfma_ring(int stage_rows) { return stage_rows == 384 ? 3 : 4; }
+constexpr int MFMA_SB = 64;          // row bytes per stage of the MFMA tiles (shape 2: 128)
+// Stages in flight (LDS ring buffers): four 64-byte stages, three for the
+// 128 x 256 tile (its ring then fits two workgroups in a CU's 160 KiB), two
+// 128-byte stages for the default 256 x 256 shape (2).
+constexpr int mfma_ring(int stage_rows, int sb = MFMA_SB) { return sb == 128 ? 2 : stage_rows == 384 ? 3 : 4; }
 
 // MFMA stage loads go straight from global memory into LDS
 // (global_load_lds_dwordx4: lane i of a wave writes 16 bytes at M0 + 16 * i),
@@ -62,27 +66,37 @@ constexpr int mfma_ring(int stage_rows) { return stage_rows == 384 ? 3 : 4; }
 // conflict-free as well.  Rows past the end read the last valid row; the
 // epilogues mask them.
 __device__ __forceinline__ int lds_chunk_key(int r) { return ((r >> 2) ^ (r >> 1)) & 3; }
+// 128-byte rows (eight 16-byte chunks, two rows per 256-byte bank line): chunk
+// c of row r at slot c ^ ((r >> 1) & 7), so the 16 rows one ds_read_b128 pass
+// takes, eight even and eight odd, land on 16 distinct groups of 4 banks
+template <int SB>
+__device__ __forceinline__ int stage_chunk_key(int r) {
+  if constexpr (SB == 128) return (r >> 1) & 7;
+  else return lds_chunk_key(r);
+}
 
-template <int ROWS, int WAVES>
+template <int ROWS, int WAVES, int SB = MFMA_SB>
 __device__ __forceinline__ void dma_stage(uint8_t* lds_operand, const uint8_t* __restrict__ base,
                                           int64_t row_bytes, int row0, int nrows, int rstride, int kbyte) {
+  constexpr int LPR = SB / 16;    // lanes per row (16 bytes each)
+  constexpr int RPI = 64 / LPR;   // rows per load instruction (1 KiB)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // the same wave index as a scalar, for the LDS destination (M0) of each
   // load: scalar arithmetic rather than a vector value read back per load
   const int wave_s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int rl = lane >> 2;                 // row within the 16-row group
-  static_assert(ROWS % (16 * WAVES) == 0, "stage rows must split evenly over the waves");
+  const int rl = lane / LPR;                // row within the instruction's rows
+  static_assert(ROWS % (RPI * WAVES) == 0, "stage rows must split evenly over the waves");
   // uniform 64-bit base of the tile's first row; per-lane 32-bit offsets
   // (a tile spans ROWS * rstride rows, well under 4 GiB)
   const uint32_t pitch = (uint32_t)(rstride * row_bytes);
   const uint8_t* tile = base + (int64_t)row0 * rstride * row_bytes;
   const int last = nrows - 1 - row0;
 #pragma unroll
-  for (int i = 0; i < ROWS / 16 / WAVES; i++) {
+  for (int i = 0; i < ROWS / RPI / WAVES; i++) {
     const int grp = wave + WAVES * i;
-    // logical chunk stored at physical slot lane & 3 of local row grp * 16 + rl
-    const int chunk = (lane & 3) ^ lds_chunk_key(grp * 16 + rl);
-    const int r = min(grp * 16 + rl, last);
+    // logical chunk stored at physical slot lane % LPR of local row grp * RPI + rl
+    const int chunk = (lane % LPR) ^ stage_chunk_key<SB>(grp * RPI + rl);
+    const int r = min(grp * RPI + rl, last);
     const uint32_t off = (uint32_t)r * pitch + (uint32_t)(kbyte + chunk * 16);
     __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(tile + off),
                                      (__attribute__((address_space(3))) void*)(lds_operand + (wave_s + WAVES * i) * 1024), 16, 0,
@@ -120,7 +134,7 @@ __device__ __forceinline__ const uint8_t* opaque_uniform_ptr(const uint8_t* p) {
   return (const uint8_t*)(((uint64_t)hi << 32) | lo);
 }
 
-template <int KIND, int BM, int BN, int WM = 2, int WN = 2>
+template <int KIND, int BM, int BN, int WM = 2, int WN = 2, int SB = MFMA_SB>
 struct Engine {
   using M = Mfma<KIND>;
   typedef typename M::acc_t acc_t;
@@ -128,26 +142,26 @@ struct Engine {
   static constexpr int THREADS = 64 * WAVES;
   static constexpr int TC = 32 * BM * WM;               // candidate rows per tile
   static constexpr int TQ = 32 * BN * WN;               // query rows per tile
-  static constexpr int STAGE_LDS = (TC + TQ) * MFMA_SB;
-  static constexpr int NS = mfma_ring(TC + TQ);         // ring depth
+  static constexpr int STAGE_LDS = (TC + TQ) * SB;
+  static constexpr int NS = mfma_ring(TC + TQ, SB);     // ring depth
   static constexpr int LDS = NS * STAGE_LDS;            // ring of NS stages
   static constexpr int GROUPS = WM * BM * 2;            // pass-1 groups per tile
   // DMA instructions each wave issues per stage
-  static constexpr int DMA_PER_STAGE = (TC + TQ) / 16 / WAVES;
+  static constexpr int DMA_PER_STAGE = (TC + TQ) / (1024 / SB) / WAVES;
   __device__ static int wave_m() { return (threadIdx.x >> 6) / WN; }
   __device__ static int wave_n() { return (threadIdx.x >> 6) % WN; }
 
   // stage `st` of the tile (c0, q0) into ring buffer `buf`
   __device__ __forceinline__ static void issue_stage(uint8_t* buf, const uint8_t* c_rows, const uint8_t* q_rows,
                                                      const TileArgs& a, int c0, int q0, int st) {
-    dma_stage<TC, WAVES>(buf, c_rows, a.row_bytes, c0, a.n_c, a.c_stride, st * MFMA_SB);
-    dma_stage<TQ, WAVES>(buf + TC * MFMA_SB, q_rows, a.row_bytes, q0, a.n_q, 1, st * MFMA_SB);
+    dma_stage<TC, WAVES, SB>(buf, c_rows, a.row_bytes, c0, a.n_c, a.c_stride, st * SB);
+    dma_stage<TQ, WAVES, SB>(buf + TC * SB, q_rows, a.row_bytes, q0, a.n_q, 1, st * SB);
   }
 
   // Whether a workgroup's next tile can be primed by this one: its first
   // NS stages are loaded during this tile's last NS stages, which
   // needs at least NS stages per tile.
-  __device__ static bool can_prime(const TileArgs& a) { return a.row_bytes / MFMA_SB >= NS; }
+  __device__ static bool can_prime(const TileArgs& a) { return a.row_bytes / SB >= NS; }
 
   // Accumulate the TC x TQ score tile (candidates c0.., queries q0..) over all
   // stages.  Stage st of the tile lives in ring buffer (ph + st) % NS;
@@ -172,15 +186,15 @@ struct Engine {
     const int wm = wave_m(), wn = wave_n();
     // (the accumulators start at zero through the first K step's MFMAs,
     // which add to an inline zero instead of the previous tile's values)
-    const int nst = (int)(a.row_bytes / MFMA_SB);
+    const int nst = (int)(a.row_bytes / SB);
     // fragment rows of this lane; every block starts at a multiple of 32 rows,
     // so the swizzle key of the lane's row is the same in every block
-    const int arow = (wm * 32 * BM + (lane & 31)) * MFMA_SB;
-    const int brow = TC * MFMA_SB + (wn * 32 * BN + (lane & 31)) * MFMA_SB;
-    const int sw = lds_chunk_key(lane & 31);
+    const int arow = (wm * 32 * BM + (lane & 31)) * SB;
+    const int brow = TC * SB + (wn * 32 * BN + (lane & 31)) * SB;
+    const int sw = stage_chunk_key<SB>(lane & 31);
     const uint8_t* c_rows = opaque_uniform_ptr(a.c_rows);
     const uint8_t* q_rows = opaque_uniform_ptr(a.q_rows);
-    constexpr int KS = MFMA_SB / M::KSTEP_BYTES;
+    constexpr int KS = SB / M::KSTEP_BYTES;
     static_assert(KS % 2 == 0, "fragment double buffer parity must carry across stages");
     typename M::frag_t fa[2][BM], fb[2][BN];
     auto ring = [&](int st) __attribute__((always_inline)) { return lds + ((ph + st) % NS) * STAGE_LDS; };
@@ -188,9 +202,9 @@ struct Engine {
       const int kb = ks * M::KSTEP_BYTES + (lane >> 5) * M::LANE_BYTES;
       const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
 #pragma unroll
-      for (int b = 0; b < BM; b++) fa[par][b] = M::load(buf + arow + off + b * 32 * MFMA_SB);
+      for (int b = 0; b < BM; b++) fa[par][b] = M::load(buf + arow + off + b * 32 * SB);
 #pragma unroll
-      for (int b = 0; b < BN; b++) fb[par][b] = M::load(buf + brow + off + b * 32 * MFMA_SB);
+      for (int b = 0; b < BN; b++) fb[par][b] = M::load(buf + brow + off + b * 32 * SB);
     };
     // wait until this wave's loads of `stage` have landed; `newer` later stages may stay in flight
     auto wait_stage = [&](int newer) __attribute__((always_inline)) {
@@ -335,9 +349,9 @@ __device__ __forceinline__ uint32_t valid_rows(int cb, int lane, int n_c) {
 // wave row half x BM blocks x lane half, 16 candidates each).
 // ---------------------------------------------------------------------------
 // The work of one (query block, candidate split): group maxima of its tiles.
-template <int KIND, int BM, int BN, int WM, int WN>
+template <int KIND, int BM, int BN, int WM, int WN, int SB = MFMA_SB>
 __device__ __forceinline__ void pass1_work(const TileArgs& a, int qb, int split, uint8_t* lds) {
-  using En = Engine<KIND, BM, BN, WM, WN>;
+  using En = Engine<KIND, BM, BN, WM, WN, SB>;
   using M = Mfma<KIND>;
   typename En::acc_t acc[BM][BN];
   const int lane = threadIdx.x & 63;
@@ -392,12 +406,12 @@ __device__ __forceinline__ void pass1_work(const TileArgs& a, int qb, int split,
   }
 }
 
-template <int KIND, int BM, int BN, int WM, int WN>
+template <int KIND, int BM, int BN, int WM, int WN, int SB = MFMA_SB>
 __global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass1_groupmax_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   int qb, split;
   if (!tile_coords(a, qb, split)) return;
-  pass1_work<KIND, BM, BN, WM, WN>(a, qb, split, lds);
+  pass1_work<KIND, BM, BN, WM, WN, SB>(a, qb, split, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -407,9 +421,9 @@ __global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass1_gro
 // ---------------------------------------------------------------------------
 // The work of one (query block, candidate split): appends of its tiles and the
 // split's per-query counts.
-template <int KIND, int BM, int BN, int WM, int WN>
+template <int KIND, int BM, int BN, int WM, int WN, int SB = MFMA_SB>
 __device__ __forceinline__ void pass2_work(const TileArgs& a, int qb, int split, uint8_t* lds) {
-  using En = Engine<KIND, BM, BN, WM, WN>;
+  using En = Engine<KIND, BM, BN, WM, WN, SB>;
   using M = Mfma<KIND>;
   // LDS after the stage ring: per-query append counters and bounds (the bounds
   // live here rather than in registers, which the MFMA tile needs)
@@ -506,12 +520,12 @@ __device__ __forceinline__ void pass2_work(const TileArgs& a, int qb, int split,
   }
 }
 
-template <int KIND, int BM, int BN, int WM, int WN>
+template <int KIND, int BM, int BN, int WM, int WN, int SB = MFMA_SB>
 __global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass2_append_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   int qb, split;
   if (!tile_coords(a, qb, split)) return;
-  pass2_work<KIND, BM, BN, WM, WN>(a, qb, split, lds);
+  pass2_work<KIND, BM, BN, WM, WN, SB>(a, qb, split, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -540,10 +554,10 @@ __device__ __forceinline__ int early_int_bound(float t) {
   return b;
 }
 
-template <int KIND, int BM, int BN, int WM, int WN>
+template <int KIND, int BM, int BN, int WM, int WN, int SB = MFMA_SB>
 __global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass2_sym_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
-  using En = Engine<KIND, BM, BN, WM, WN>;
+  using En = Engine<KIND, BM, BN, WM, WN, SB>;
   using M = Mfma<KIND>;
   static_assert(En::TC == En::TQ, "the symmetric pass needs square tiles");
   constexpr int T = En::TC;
@@ -1574,7 +1588,7 @@ __global__ __launch_bounds__(256) void batch_begin_kernel(BatchReq* __restrict__
   }
 }
 
-template <int KIND, int BM, int BN, int WM, int WN>
+template <int KIND, int BM, int BN, int WM, int WN, int SB = MFMA_SB>
 __global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass1_batch_kernel(const BatchReq* __restrict__ reqs,
                                                                    const int4* __restrict__ items,
                                                                    const BatchHeader* __restrict__ hdr) {
@@ -1588,7 +1602,7 @@ __global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass1_bat
   }
 }
 
-template <int KIND, int BM, int BN, int WM, int WN>
+template <int KIND, int BM, int BN, int WM, int WN, int SB = MFMA_SB>
 __global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass2_batch_kernel(const BatchReq* __restrict__ reqs,
                                                                    const int4* __restrict__ items,
                                                                    const BatchHeader* __restrict__ hdr) {
@@ -1689,8 +1703,8 @@ int tile_shape(int kind) {
     const char* e = getenv("EVOC_MI355X_TILE");
     g_env_shape = e ? atoi(e) : -1;
   }
-  if (g_env_shape >= 0 && g_env_shape <= 3) return g_env_shape;
-  if (t_shape >= 0 && t_shape <= 3) return t_shape;
+  if (g_env_shape >= 0 && g_env_shape <= 4) return g_env_shape;
+  if (t_shape >= 0 && t_shape <= 4) return t_shape;
   return 2;
 }
 
@@ -1705,7 +1719,7 @@ static void shape_dims(int kind, int& tc, int& tq, int& gpt) {
   switch (tile_shape(kind)) {
     case 1: tc = 128; tq = 256; gpt = 8; return;
     case 2: tc = 256; tq = 256; gpt = 16; return;
-    case 3: tc = 256; tq = 256; gpt = 16; return;
+    case 3: case 4: tc = 256; tq = 256; gpt = 16; return;
     default: tc = 128; tq = 128; gpt = 8; return;
   }
 }
@@ -1719,10 +1733,11 @@ size_t tile_lds_bytes_shape(int shape, int pass) {
   int tc, tq;
   switch (shape) {
     case 1: tc = 128; tq = 256; break;
-    case 2: case 3: tc = 256; tq = 256; break;
+    case 2: case 3: case 4: tc = 256; tq = 256; break;
     default: tc = 128; tq = 128; break;
   }
-  const size_t ring = (size_t)mfma_ring(tc + tq) * (tc + tq) * MFMA_SB;
+  const int sb = shape == 2 ? 128 : MFMA_SB;
+  const size_t ring = (size_t)mfma_ring(tc + tq, sb) * (tc + tq) * sb;
   if (pass == 3) return ring + (size_t)tq * 40 + (size_t)tc * SYM_SLOTS * 8 + 16 + (size_t)(tq + tc) * 4;
   return ring + (pass == 2 ? (size_t)tq * (4 + 4 + 4 + 8) : 0);
 }
@@ -1750,14 +1765,15 @@ static hipError_t launch_tile(Kern kern, size_t lds, dim3 grid, hipStream_t s, c
   return hipGetLastError();
 }
 
-int tile_threads(int kind) { return tile_shape(kind) == 2 ? 512 : 256; }  // shape 3: 4 waves
+int tile_threads(int kind) { return tile_shape(kind) == 2 || tile_shape(kind) == 4 ? 512 : 256; }  // shape 3: 4 waves
 
 template <int KIND>
 static hipError_t launch_pass1_kind(const TileArgs& a, dim3 g, size_t lds, hipStream_t s) {
   switch (tile_shape(KIND)) {
     case 1: return launch_tile(pass1_groupmax_kernel<KIND, 2, 4, 2, 2>, lds, g, s, a);
-    case 2: return launch_tile(pass1_groupmax_kernel<KIND, 2, 4, 4, 2>, lds, g, s, a, 512);
+    case 2: return launch_tile(pass1_groupmax_kernel<KIND, 2, 4, 4, 2, 128>, lds, g, s, a, 512);
     case 3: return launch_tile(pass1_groupmax_kernel<KIND, 4, 4, 2, 2>, lds, g, s, a);
+    case 4: return launch_tile(pass1_groupmax_kernel<KIND, 2, 4, 4, 2, 64>, lds, g, s, a, 512);
     default: return launch_tile(pass1_groupmax_kernel<KIND, 2, 2, 2, 2>, lds, g, s, a);
   }
 }
@@ -1766,8 +1782,9 @@ template <int KIND>
 static hipError_t launch_pass2_kind(const TileArgs& a, dim3 g, size_t lds, hipStream_t s) {
   switch (tile_shape(KIND)) {
     case 1: return launch_tile(pass2_append_kernel<KIND, 2, 4, 2, 2>, lds, g, s, a);
-    case 2: return launch_tile(pass2_append_kernel<KIND, 2, 4, 4, 2>, lds, g, s, a, 512);
+    case 2: return launch_tile(pass2_append_kernel<KIND, 2, 4, 4, 2, 128>, lds, g, s, a, 512);
     case 3: return launch_tile(pass2_append_kernel<KIND, 4, 4, 2, 2>, lds, g, s, a);
+    case 4: return launch_tile(pass2_append_kernel<KIND, 2, 4, 4, 2, 64>, lds, g, s, a, 512);
     default: return launch_tile(pass2_append_kernel<KIND, 2, 2, 2, 2>, lds, g, s, a);
   }
 }
@@ -1811,8 +1828,9 @@ hipError_t launch_pass2(int kind, const TileArgs& a0, int n_blocks, int n_splits
 template <int KIND>
 static hipError_t launch_pass2_sym_kind(const TileArgs& a, dim3 g, size_t lds, hipStream_t s) {
   switch (tile_shape(KIND)) {
-    case 2: return launch_tile(pass2_sym_kernel<KIND, 2, 4, 4, 2>, lds, g, s, a, 512);
+    case 2: return launch_tile(pass2_sym_kernel<KIND, 2, 4, 4, 2, 128>, lds, g, s, a, 512);
     case 3: return launch_tile(pass2_sym_kernel<KIND, 4, 4, 2, 2>, lds, g, s, a);
+    case 4: return launch_tile(pass2_sym_kernel<KIND, 2, 4, 4, 2, 64>, lds, g, s, a, 512);
     case 0: return launch_tile(pass2_sym_kernel<KIND, 2, 2, 2, 2>, lds, g, s, a);
     default: return hipErrorInvalidValue;  // 128 x 256 tiles are not square
   }
