@@ -454,6 +454,10 @@ __device__ __forceinline__ void pass2_work(const TileArgs& a, int qb, int split,
   for (int t = t_begin; t < t_end; t++) {
     const int c0 = t * En::TC;
     En::score_tile(acc, lds, a, c0, q0, ph, chain && t > t_begin, chain && t + 1 < t_end, c0 + En::TC, q0);
+    // the next tile's first stages may still be landing in LDS: waiting for
+    // them once here spares each block's LDS atomic a wait (vmcnt) that would
+    // also cover the previous block's global stores
+    wait_vmcnt<0>();
     // lane / wave indices made opaque here: what the epilogue derives from
     // them is recomputed per tile instead of living in registers through the
     // MFMA loop (which needs them all)
@@ -683,16 +687,35 @@ __global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass2_sym
           B tb;
           if constexpr (KIND == EVOC_ELEM_I8) tb = tfast_i[ql];
           else tb = tfast_l[ql];
-          uint64_t any = 0;
+          // One ballot per block: the lane's largest value against its query
+          // row's bound, and its largest margin x - cb over its candidate
+          // rows' bounds against zero (max trees, one scalar OR per block
+          // rather than two per value).  f32: x >= cb gives x - cb >= +0, and
+          // a difference that rounds to -0 only lets the block through to the
+          // exact test; i32: the saturating difference keeps the sign of
+          // x - cb.
+          B mq = acc[bm][bn][0], mc;
+          if constexpr (KIND == EVOC_ELEM_I8) mc = __builtin_elementwise_sub_sat(acc[bm][bn][0], cb[0]);
+          else mc = acc[bm][bn][0] - cb[0];
 #pragma unroll
-          for (int j = 0; j < 16; j++) {
-            const B x = acc[bm][bn][j];
-            any |= __ballot(x >= tb) | __ballot(x >= cb[j]);
+          for (int j = 1; j < 16; j++) {
+            if constexpr (KIND == EVOC_ELEM_I8) {
+              mq = max(mq, acc[bm][bn][j]);
+              mc = max(mc, __builtin_elementwise_sub_sat(acc[bm][bn][j], cb[j]));
+            } else {
+              mq = fmaxf(mq, acc[bm][bn][j]);
+              mc = fmaxf(mc, acc[bm][bn][j] - cb[j]);
+            }
           }
-          if (any) need |= 1u << (bm * BN + bn);
+          if (__ballot(mq >= tb || mc >= (B)0)) need |= 1u << (bm * BN + bn);
         });
       });
     }
+    // The next tile's first stages may still be landing in LDS.  The compiler
+    // makes every LDS write or atomic wait (vmcnt) for such loads, which also
+    // waits for every global store issued before it: one wait here, before the
+    // appends, lets the LDS atomics below go without waiting on the stores.
+    if (need) wait_vmcnt<0>();
     // The blocks that need it, one at a time, rolled: the block's 16 values are
     // copied out under a uniform branch, so the epilogue's code is one block's
     // worth (unrolled over all BM x BN blocks it outgrows the instruction cache
