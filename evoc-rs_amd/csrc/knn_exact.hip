@@ -197,7 +197,10 @@ struct Engine {
     constexpr int KS = SB / M::KSTEP_BYTES;
     static_assert(KS % 2 == 0, "fragment double buffer parity must carry across stages");
     typename M::frag_t fa[2][BM], fb[2][BN];
-    auto ring = [&](int st) __attribute__((always_inline)) { return lds + ((ph + st) % NS) * STAGE_LDS; };
+    // (ph + st >= 0: unsigned, so a ring of 2 or 4 indexes with one AND)
+    auto ring = [&](int st) __attribute__((always_inline)) {
+      return lds + ((unsigned)(ph + st) % (unsigned)NS) * STAGE_LDS;
+    };
     auto read_frags = [&](const uint8_t* buf, int ks, int par) __attribute__((always_inline)) {
       const int kb = ks * M::KSTEP_BYTES + (lane >> 5) * M::LANE_BYTES;
       const int off = (((kb >> 4) ^ sw) << 4) + (kb & 15);
@@ -287,7 +290,7 @@ struct Engine {
     // waited for all); saying so lets the epilogue's LDS accesses go without
     // further vmcnt waits
     if (!has_next) wait_vmcnt<0>();
-    ph = (ph + nst) % NS;
+    ph = (int)((unsigned)(ph + nst) % (unsigned)NS);
   }
 };
 
