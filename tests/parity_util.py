@@ -10,7 +10,8 @@ FIXTURES = [("small_200", 42), ("medium_800", 42), ("large_2000", 7)]
 
 def load_fixture(name):
     d = os.path.join(GOLDEN, name)
-    meta = json.load(open(os.path.join(d, "meta.json")))
+    with open(os.path.join(d, "meta.json")) as f:
+        meta = json.load(f)
     return (np.load(os.path.join(d, "data.npy")), np.load(os.path.join(d, "nn_inds.npy")),
             np.load(os.path.join(d, "nn_dists.npy")), meta)
 

@@ -16,7 +16,8 @@ HEADER = os.path.join(ROOT, "include", "evoc_mi355x.h")
 
 
 def declared_functions():
-    txt = open(HEADER).read()
+    with open(HEADER) as f:
+        txt = f.read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(evoc_[a-z0-9_]+)\s*\(", txt)))
 
@@ -36,7 +37,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_library_is_gfx950_only():
     """The offload bundle ids embedded in the library name only gfx950."""
-    blob = open(E.LIB_PATH, "rb").read()
+    with open(E.LIB_PATH, "rb") as f:
+        blob = f.read()
     targets = set(re.findall(rb"amdgcn-amd-amdhsa--(gfx[0-9a-z]+)", blob))
     assert targets == {b"gfx950"}
 
