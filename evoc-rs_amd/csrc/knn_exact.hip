@@ -179,9 +179,11 @@ struct Engine {
   // epilogue, and the next call, `primed`, starts on them.  Memory operations
   // issued between two tiles (the epilogue's) only make the vmcnt waits
   // stricter: everything older than a stage's loads has landed with them.
+  // `published`: the primed stages have landed in every wave and a workgroup
+  // barrier has passed since (the caller's), so the first stage needs no wait.
   __device__ __forceinline__ static void score_tile(acc_t (&acc)[BM][BN], uint8_t* lds, const TileArgs& a, int c0,
                                                     int q0, int& ph, bool primed, bool has_next, int nc0,
-                                                    int nq0) {
+                                                    int nq0, bool published = false) {
     const int lane = threadIdx.x & 63;
     const int wm = wave_m(), wn = wave_n();
     // (the accumulators start at zero through the first K step's MFMAs,
@@ -230,8 +232,10 @@ struct Engine {
       for (int st = 0; st < NS; st++)
         if (st < nst) issue(st);
     }
-    wait_stage(min(NS - 1, nst - 1));
-    ring_barrier();
+    if (!published) {
+      wait_stage(min(NS - 1, nst - 1));
+      ring_barrier();
+    }
     read_frags(ring(0), 0, 0);
     // MFMAs of one K step from fragment buffer `par`: one MFMA, then all the
     // fragment reads of the next step issued just before (the first MFMA's
@@ -646,11 +650,18 @@ __global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass2_sym
   for (int t = w.y; t < w.z; t++) {
     const int c0 = t * T;
     const bool diag = t == w.x;  // the query block's own tile: query side only
-    __syncthreads();             // the last tile's epilogue and flush are done with these
+    // The previous tile ended with a workgroup barrier after every wave's
+    // epilogue (and after each wave's loads of this tile's first stages had
+    // landed: the barrier waits for them), then the flush by the threads that
+    // now store this tile's candidate bounds (thread i: row i, both times).
+    // So no barrier here, and this tile's first stage is already published:
+    // the waves that do not flush start on its MFMAs at once.  The bounds
+    // written here reach the other waves through the stage barriers below,
+    // every one of which follows an LDS wait (lgkmcnt(0)).
     if (t != w.y && threadIdx.x < T) store_cbounds(c0, pf_tau, pf_key);  // (t > w.y >= w.x: never diagonal)
     if (t + 1 < w.z && threadIdx.x < T) load_bound(c0 + T + (int)threadIdx.x, pf_tau, pf_key);
-    // (begins with a barrier, which publishes them)
-    En::score_tile(acc, lds, a, c0, q0, ph, chain && t > w.y, chain && t + 1 < w.z, c0 + T, q0);
+    const bool primed = chain && t > w.y;
+    En::score_tile(acc, lds, a, c0, q0, ph, primed, chain && t + 1 < w.z, c0 + T, q0, primed);
     int lane_e = lane, wm_e = wm, wn_e = wn;
     asm volatile("" : "+v"(lane_e), "+v"(wm_e), "+v"(wn_e));
     // every candidate row of the tile has an exact cheap test
@@ -894,8 +905,12 @@ __global__ __launch_bounds__(64 * WM * WN, BM * BN == 16 ? 1 : 2) void pass2_sym
         }
       }
     }
+    // every wave's staged keys are in, every wave is done reading this tile's
+    // bounds, and every wave's loads of the next tile's first stages have
+    // landed (on the diagonal tile too, which has no flush: the next tile
+    // relies on this barrier)
+    __syncthreads();
     if (!diag) {
-      __syncthreads();  // every wave's staged keys are in
       for (int i = threadIdx.x; i < T; i += En::THREADS) {
         const int k = min(scnt[i], SYM_SLOTS);
         const int c = c0 + i;
