@@ -31,7 +31,7 @@ PEAK = {"bf16": 2500.0, "fp32": 157.3, "int8": 5000.0}  # dense TFLOP/s (TOP/s f
 MEASURED_TRAFFIC = {
     (100000, 768, "pass2_sym_kernel<bf16>"): (15.5e9, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated "
                                                       "(profiles/pass2_sym_bf16_100k_768_traffic.md)"),
-    (1000000, 768, "pass2_sym_kernel<bf16>"): (3.33e12, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated "
+    (1000000, 768, "pass2_sym_kernel<bf16>"): (3.45e12, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated "
                                                         "(profiles/pass2_sym_bf16_1m_768_traffic.md)"),
     # the NN-descent local join: the mean over a call's 23 launches
     (100000, 768, "nnd_join_mfma_kernel<bf16>"): (1.957e9, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated, "
