@@ -275,6 +275,35 @@ def test_symmetric_pass_matches_split_pass(ctx, dtype, prec_name, n):
         np.testing.assert_array_equal(out["1"][0], ei)
 
 
+@pytest.mark.parametrize("dtype,d", [("f32", 48), ("f32", 128), ("i8", 64), ("i8", 256)])
+def test_symmetric_pass_narrow_rows(ctx, dtype, d):
+    """Rows of one or two 128-byte K stages (BF16 d = 48 and 128, int8 d = 64 and 256): a tile
+    of one stage is not primed by the one before; a tile of two starts on a first stage that the
+    previous tile's loads put in LDS.  The symmetric pass gives the per-split pass's graph, and
+    int8 the oracle's."""
+    n = 6000
+    print(f"[gpu] symmetric narrow rows {dtype} d={d} n={n}", flush=True)
+    if dtype == "f32":
+        x = synth.blobs_f32(n, d, 25, 0.3, seed=d)
+    else:
+        x = synth.blobs_i8(n, d, 25, 0.3, seed=d)
+    kw = {"n_neighbors": 20}
+    if dtype == "f32":
+        kw["precision"] = E.PREC_BF16_RESCORE
+    out = {}
+    for sym in ("1", "0"):
+        os.environ["EVOC_MI355X_SYMMETRIC"] = sym
+        try:
+            out[sym] = run(ctx, x, 5, **kw)
+        finally:
+            del os.environ["EVOC_MI355X_SYMMETRIC"]
+    np.testing.assert_array_equal(out["1"][0], out["0"][0])
+    np.testing.assert_array_equal(out["1"][1].view(np.uint32), out["0"][1].view(np.uint32))
+    if dtype == "i8":
+        ei, _ = O.exact_knn(x, 20)
+        np.testing.assert_array_equal(out["1"][0], ei)
+
+
 @pytest.mark.parametrize("sym", ["1", "0"])
 @pytest.mark.parametrize("dtype,prec_name", [("f32", "bf16"), ("f32", "fp32"), ("i8", None), ("u8", None)])
 def test_estimated_bound_misses_are_scored_again(ctx, dtype, prec_name, sym):
