@@ -142,14 +142,12 @@ __global__ __launch_bounds__(256, 2) void nnd_join_mfma_kernel(JoinArgs a) {
   // the first nN of them), each read once
   if (a.bytes && lane == 0) atomicAdd(a.bytes, (unsigned long long)nC * (unsigned long long)a.row_bytes);
   const int nbm = (nN + 31) >> 5, nbn = (nC + 31) >> 5;
-  // fragment row pointers (invalid rows read a valid row; masked below)
-  const uint8_t* arow[2];
+  // Fragment row pointers of C only: C starts with N (cidx[r] = nidx[r] for
+  // r < nN), so the A operand's block bm is C's block bm.  Rows of that block
+  // at or past nN are C's old rows (or row 0 past nC) where N would have had
+  // row 0; every such row is masked below, so the scores that are kept are the
+  // same products in the same K order.  Invalid rows read a valid row.
   const uint8_t* brow[4];
-#pragma unroll
-  for (int bm = 0; bm < 2; bm++) {
-    const int r = bm * 32 + (lane & 31);
-    arow[bm] = a.rows + (int64_t)sh.nidx[r < nN ? r : 0] * a.row_bytes + (lane >> 5) * M::LANE_BYTES;
-  }
 #pragma unroll
   for (int bn = 0; bn < 4; bn++) {
     const int c = bn * 32 + (lane & 31);
@@ -160,34 +158,45 @@ __global__ __launch_bounds__(256, 2) void nnd_join_mfma_kernel(JoinArgs a) {
   for (int bm = 0; bm < 2; bm++)
 #pragma unroll
     for (int bn = 0; bn < 4; bn++) acc[bm][bn] = M::zero();
-  // Fragments straight from HBM, double-buffered in registers: the next K
-  // step's loads are in flight while this step's MFMAs run.  Every block is
-  // multiplied (blocks past nN / nC read a valid row and are masked below),
-  // and the step count is even (rows are padded to 128 bytes), so the two
-  // buffers alternate with static indices.
+  // Fragments straight from HBM through a ring of D register sets: the loads
+  // of the D - 1 steps after the current one are in flight while its MFMAs
+  // run.  Every load is unconditional (the blocks past nbn read row cidx[0],
+  // the steps past the end re-read the last step into sets nothing reads), so
+  // the compiler's vmcnt waits leave the D - 1 newer steps in flight; a load
+  // under a branch would make it drain them all at every step.  Only the
+  // nbm x nbn blocks that hold pairs are multiplied (wave-uniform tests; the
+  // blocks past them are never read below).  The step count is a multiple of
+  // D (rows are padded to 128 bytes), so the sets rotate with static indices.
+  constexpr int D = KIND == EVOC_ELEM_F32 ? 8 : 4;
+  static_assert(128 % (D * M::KSTEP_BYTES) == 0, "128-byte rows hold whole rounds of the ring");
   const int ksteps = (int)(a.row_bytes / M::KSTEP_BYTES);
-  typename M::frag_t fa[2][2], fb[2][4];
-  auto load_step = [&](typename M::frag_t (&ra)[2], typename M::frag_t (&rb)[4], int ks)
-      __attribute__((always_inline)) {
-    const int off = ks * M::KSTEP_BYTES;
-#pragma unroll
-    for (int bm = 0; bm < 2; bm++) ra[bm] = M::load(arow[bm] + off);
+  typename M::frag_t fb[D][4];
+  auto load_step = [&](typename M::frag_t (&rb)[4], int ks) __attribute__((always_inline)) {
+    const int off = min(ks, ksteps - 1) * M::KSTEP_BYTES;
 #pragma unroll
     for (int bn = 0; bn < 4; bn++) rb[bn] = M::load(brow[bn] + off);
   };
-  auto mma_step = [&](const typename M::frag_t (&ra)[2], const typename M::frag_t (&rb)[4])
-      __attribute__((always_inline)) {
+  auto mma_step = [&](const typename M::frag_t (&rb)[4]) __attribute__((always_inline)) {
 #pragma unroll
     for (int bm = 0; bm < 2; bm++)
 #pragma unroll
-      for (int bn = 0; bn < 4; bn++) M::mma(acc[bm][bn], ra[bm], rb[bn]);
+      for (int bn = 0; bn < 4; bn++)
+        if (bm < nbm && bn < nbn) M::mma(acc[bm][bn], rb[bm], rb[bn]);
   };
-  load_step(fa[0], fb[0], 0);
-  for (int ks = 0; ks < ksteps; ks += 2) {
-    load_step(fa[1], fb[1], ks + 1);
-    mma_step(fa[0], fb[0]);
-    if (ks + 2 < ksteps) load_step(fa[0], fb[0], ks + 2);
-    mma_step(fa[1], fb[1]);
+  // (the scheduling barriers keep the steps' loads in step order, in the
+  // first D - 1 steps as in the loop: the vmcnt each step waits for is then
+  // the same on entry as around the loop)
+  static_for<D - 1>([&](auto sc) __attribute__((always_inline)) {
+    load_step(fb[decltype(sc)::value], decltype(sc)::value);
+    __builtin_amdgcn_sched_barrier(0);
+  });
+  for (int ks = 0; ks < ksteps; ks += D) {
+    static_for<D>([&](auto ic) __attribute__((always_inline)) {
+      constexpr int i = decltype(ic)::value;
+      load_step(fb[(i + D - 1) % D], ks + i + D - 1);
+      __builtin_amdgcn_sched_barrier(0);
+      mma_step(fb[i]);
+    });
   }
   // Offers, one 32 x 32 block at a time (rolled: the block's values are
   // copied out under a uniform branch, so no accumulator is indexed at run time)
