@@ -140,7 +140,7 @@ struct NndParams {
 struct NndStats {
   int64_t iterations = 0;
   int64_t leaves = 0;
-  uint32_t dropped = 0;
+  int64_t dropped = -1;  // lost offers, -1 when not counted
   std::vector<uint32_t> changed;
   uint64_t join_bytes = 0;   // algorithmic bytes the local joins gathered
   double join_ms = 0.0;      // device time of the local-join launches

@@ -38,6 +38,7 @@ struct JoinArgs {
   int32_t n;
   uint32_t salt;            // varies the slot hash between launches
   uint32_t* dropped;        // offers that met a better offer on both slots when they arrived
+                            // (null: not counted, and the offers' atomics return nothing)
   unsigned long long* bytes;  // algorithmic bytes gathered: each item's candidate rows once (may be null)
 };
 

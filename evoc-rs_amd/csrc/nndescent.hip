@@ -75,7 +75,9 @@ struct JoinShared {
 // An offer of cand to target: the 64-bit maximum into two hashed slots of the
 // target's buffer.  A slot's final content is the best key hashed to it, so the
 // merged lists do not depend on the order in which waves arrive; an offer is
-// lost only when a better one holds both of its slots.
+// lost only when a better one holds both of its slots.  The atomics return
+// nothing unless a.dropped asks for the count of lost offers: a returning
+// atomic makes the wave wait for its round trip before the next offer.
 __device__ __forceinline__ void offer(const JoinArgs& a, int32_t target, int32_t cand, float s) {
   const unsigned long long key = gkey(s, cand, 0, 1);
   const uint64_t h = mix64((((uint64_t)(uint32_t)target << 32) | (uint32_t)cand) ^
@@ -83,9 +85,14 @@ __device__ __forceinline__ void offer(const JoinArgs& a, int32_t target, int32_t
   const uint32_t s1 = (uint32_t)(((h & 0xffffffffull) * (uint64_t)a.ucap) >> 32);
   const uint32_t s2 = (uint32_t)(((h >> 32) * (uint64_t)a.ucap) >> 32);
   unsigned long long* b = (unsigned long long*)(a.ubuf + (int64_t)target * a.ucap);
-  const unsigned long long o1 = atomicMax(b + s1, key);
-  const unsigned long long o2 = s2 != s1 ? atomicMax(b + s2, key) : o1;
-  if (o1 > key && o2 > key) atomicAdd(a.dropped, 1u);
+  if (a.dropped) {
+    const unsigned long long o1 = atomicMax(b + s1, key);
+    const unsigned long long o2 = s2 != s1 ? atomicMax(b + s2, key) : o1;
+    if (o1 > key && o2 > key) atomicAdd(a.dropped, 1u);
+  } else {
+    (void)atomicMax(b + s1, key);
+    if (s2 != s1) (void)atomicMax(b + s2, key);
+  }
   a.umark[target] = 1;
 }
 
@@ -393,41 +400,36 @@ __device__ __forceinline__ uint64_t cand_key(uint32_t prio, int32_t c) {
 }
 __device__ __forceinline__ int32_t cand_of(uint64_t k) { return (int32_t)(0xffffffffu - (uint32_t)k); }
 
-// keep the mc best of (best ∪ chunk): best holds one key per lane (lane < mc)
-__device__ __forceinline__ uint64_t keep_best(uint64_t best, uint64_t chunk, int mc) {
+// keep the mc best of (best ∪ chunk): best holds one key per lane (lane < mc).
+// The kept keys (non-empty, first of a run of equal keys) are compacted in
+// sorted order through the wave's 64-key LDS stash: kept key i goes to slot i,
+// and lane L then reads slot L.
+__device__ __forceinline__ uint64_t keep_best(uint64_t best, uint64_t chunk, int mc, uint64_t* stash) {
   uint64_t x[2] = {best, chunk};
   wave_sort_desc<2>(x);
   const int lane = threadIdx.x & 63;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  bool keep[2];
   uint64_t last_prev = 0;
+  int base = 0;
 #pragma unroll
   for (int e = 0; e < 2; e++) {
     const uint64_t up = (uint64_t)__shfl_up((long long)x[e], 1, 64);
     const uint64_t prev = lane == 0 ? last_prev : up;
-    keep[e] = x[e] != EVOC_KEY_NONE && ((e == 0 && lane == 0) || x[e] != prev);
+    const bool keep = x[e] != EVOC_KEY_NONE && ((e == 0 && lane == 0) || x[e] != prev);
     last_prev = (uint64_t)__shfl((long long)x[e], 63, 64);
+    const uint64_t m = __ballot(keep);
+    const int pos = base + __popcll(m & below);
+    if (keep && pos < mc) stash[pos] = x[e];
+    base += __popcll(m);
   }
-  // scatter the first mc kept keys to lanes 0..mc-1 through a register shuffle
-  uint64_t out = EVOC_KEY_NONE;
-  int base = 0;
-#pragma unroll
-  for (int e = 0; e < 2; e++) {
-    const uint64_t m = __ballot(keep[e]);
-    // lane L wants the (L - base)-th kept element of register e
-    const int want = lane - base;
-    const int total = __popcll(m);
-    int src = 0;
-    if (want >= 0 && want < total) {
-      // position of the want-th set bit of m
-      uint64_t mm = m;
-      for (int t = 0; t < want; t++) mm &= mm - 1;
-      src = __ffsll((long long)mm) - 1;
-    }
-    const uint64_t got = (uint64_t)__shfl((long long)x[e], src, 64);
-    if (want >= 0 && want < total && lane < mc) out = got;
-    base += total;
-  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint64_t out = lane < min(base, mc) ? stash[lane] : EVOC_KEY_NONE;
+  // the next call's writes must not pass this read
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   return out;
 }
 
@@ -435,9 +437,11 @@ __global__ __launch_bounds__(256) void nnd_build_candidates_kernel(
     uint64_t* __restrict__ keys, int n, int Kg, const int32_t* __restrict__ revoff,
     const int32_t* __restrict__ rev, int mc, uint64_t seed, int it, int32_t* __restrict__ newc,
     int32_t* __restrict__ oldc) {
+  __shared__ uint64_t stash_all[4][64];
   const int lane = threadIdx.x & 63;
   const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (v >= n) return;
+  uint64_t* stash = stash_all[threadIdx.x >> 6];
   const uint64_t fk = lane < Kg ? keys[(int64_t)v * Kg + lane] : EVOC_KEY_NONE;
   const int32_t fc = fk != EVOC_KEY_NONE ? gkey_idx(fk) : -1;
   const int fflag = (int)(fk & 1ull);
@@ -448,7 +452,7 @@ __global__ __launch_bounds__(256) void nnd_build_candidates_kernel(
     uint64_t b = EVOC_KEY_NONE;
     uint64_t ck = (fc >= 0 && fc != v && fflag == cls) ? cand_key(edge_priority(v, fc, seed, it), fc)
                                                        : EVOC_KEY_NONE;
-    b = keep_best(b, ck, mc);
+    b = keep_best(b, ck, mc, stash);
     for (int base = r0; base < r1; base += 64) {
       const int i = base + lane;
       uint64_t rk = EVOC_KEY_NONE;
@@ -457,7 +461,7 @@ __global__ __launch_bounds__(256) void nnd_build_candidates_kernel(
         const int c = e & 0x7fffffff, f = (int)((uint32_t)e >> 31);
         if (f == cls) rk = cand_key(edge_priority(v, c, seed, it), c);
       }
-      b = keep_best(b, rk, mc);
+      b = keep_best(b, rk, mc, stash);
     }
     best[cls] = b;
   }

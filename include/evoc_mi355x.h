@@ -110,7 +110,9 @@ typedef struct evoc_knn_output {
   int64_t rescore_rounds;  /* exact: extra scoring rounds for overflowing buffers */
   int64_t offers_dropped;  /* NN-descent: offers that found a better offer in both of their
                               hashed buffer slots on arrival (a diagnostic that may vary between
-                              calls; the lists do not depend on arrival order) */
+                              calls; the lists do not depend on arrival order).  Counted only
+                              when the environment sets EVOC_MI355X_COUNT_DROPPED=1 (counting
+                              makes every offer wait for its atomics), else -1 */
   double queue_seconds;    /* serving: time from submit until a slot started the request */
   int64_t score_pairs;     /* exact: row pairs the scoring kernel scored (one per unordered pair
                               in the symmetric pass, one per query and candidate otherwise) */

@@ -183,3 +183,24 @@ def test_nndescent_c2_100k(ctx):
               f"time={st.seconds*1e3:.1f} ms iters={st.iterations}", flush=True)
         assert st.iterations == iters
         assert r >= r_ref
+
+
+@pytest.mark.parametrize("kind", ["f32", "i8", "u8"])
+def test_lost_offer_count_is_opt_in_and_changes_nothing(ctx, kind, monkeypatch):
+    """The offers' atomics return their old values only when the lost offers are
+    counted (EVOC_MI355X_COUNT_DROPPED=1); the lists are the same either way."""
+    if kind == "f32":
+        x = synth.blobs_f32(6000, 96, 20, 0.3, seed=5)
+    elif kind == "i8":
+        x = synth.blobs_i8(6000, 96, 20, 0.3, seed=5)
+    else:
+        x = synth.bits_u8(6000, 256, 20, 0.3, 0.1, seed=5)
+    monkeypatch.delenv("EVOC_MI355X_COUNT_DROPPED", raising=False)
+    i0, d0, s0 = nnd(ctx, x, 15, 6)
+    monkeypatch.setenv("EVOC_MI355X_COUNT_DROPPED", "1")
+    i1, d1, s1 = nnd(ctx, x, 15, 6)
+    print(f"[gpu] nnd {kind}: dropped uncounted={s0.offers_dropped} counted={s1.offers_dropped}", flush=True)
+    assert s0.offers_dropped == -1
+    assert s1.offers_dropped >= 0
+    assert np.array_equal(i0, i1)
+    assert np.array_equal(d0.view(np.uint32), d1.view(np.uint32))
