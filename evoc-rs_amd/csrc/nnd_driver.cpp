@@ -191,7 +191,7 @@ uint64_t nndescent_bytes(int kind, int64_t n, int Kg, int max_candidates, int64_
   const int64_t ls = std::min<int64_t>(std::max<int64_t>(leaf_size, 2), 64);
   const int64_t nb = (n + 1023) / 1024;
   uint64_t b = r(n * Kg * 8) + 2 * r(n * 4) + r(n * UCAP * 8) + 2 * r(n * mc * 4) + 2 * r(n * 4) +
-               r((n + 1) * 4) + r(n * Kg * 4) + r((nb + 2) * 4) + 2 * 256;
+               r((n + 1) * 4) + r(n * Kg * 4) + r((nb + 2) * 4) + r(NND_COUNTER_BYTES) + 256;
   b += forest_bytes(kind, n, dim, row_bytes, (int)ls, NND_FOREST_TREES);
   return b;
 }
@@ -219,11 +219,14 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
       (rc = revcur.alloc(ctx, (size_t)n * 4, "reverse cursors")) ||
       (rc = rev.alloc(ctx, (size_t)n * Kg * 4, "reverse edges")) ||
       (rc = sums.alloc(ctx, (size_t)(nb + 2) * 4, "scan sums")) ||
-      (rc = counters.alloc(ctx, 16, "counters")) || (rc = sumd.alloc(ctx, 8, "distance sum")))
+      (rc = counters.alloc(ctx, NND_COUNTER_BYTES, "counters")) || (rc = sumd.alloc(ctx, 8, "distance sum")))
     return rc;
+  // inserted-entry slots, byte slots, lost-offer count (NND_COUNTER_BYTES)
   uint32_t* d_changed = (uint32_t*)counters.ptr;
-  uint32_t* d_dropped = d_changed + 1;
-  unsigned long long* d_jbytes = (unsigned long long*)(d_changed + 2);  // 8-byte aligned
+  unsigned long long* d_jbytes = (unsigned long long*)((uint8_t*)counters.ptr + NND_COUNTER_SLOTS * 128);
+  uint32_t* d_dropped = (uint32_t*)((uint8_t*)counters.ptr + NND_COUNTER_SLOTS * 128 * 2);
+  std::vector<uint32_t> h_changed((size_t)NND_COUNTER_SLOTS * NND_SLOT_U32);
+  std::vector<unsigned long long> h_jbytes((size_t)NND_COUNTER_SLOTS * NND_SLOT_U64);
   Check ck{ctx};
   JoinTimer jt{ctx};
   auto join = [&](const JoinArgs& args, const char* what) {
@@ -231,7 +234,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
     ck(nnd_launch_join(kind, args, s), what);
     jt.end(s);
   };
-  ck(launch_fill_i32((int32_t*)counters.ptr, 4, 0, s), "counters");
+  ck(launch_fill_i32((int32_t*)counters.ptr, (int64_t)(NND_COUNTER_BYTES / 4), 0, s), "counters");
   ck(nnd_launch_init((uint64_t*)keys.ptr, (float*)thr.ptr, (int32_t*)umark.ptr, n, Kg, s), "list init");
   ck(launch_fill_i32((int32_t*)ubuf.ptr, (int64_t)n * UCAP * 2, 0, s), "offer slots");  // EVOC_KEY_NONE
 
@@ -313,7 +316,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
   if (const char* e = std::getenv("EVOC_MI355X_NND_BLOCK")) block = std::max<int64_t>(1024, std::atoll(e));
   int it = 0;
   for (; it < P.n_iters && ck.e == hipSuccess; it++) {
-    ck(launch_fill_i32((int32_t*)d_changed, 1, 0, s), "counters");
+    ck(launch_fill_i32((int32_t*)d_changed, (int64_t)NND_COUNTER_SLOTS * NND_SLOT_U32, 0, s), "counters");
     ck(nnd_launch_candidates((uint64_t*)keys.ptr, n, Kg, (int32_t*)revcnt.ptr, (int32_t*)revoff.ptr,
                              (int32_t*)revcur.ptr, (int32_t*)rev.ptr, (int32_t*)sums.ptr, mc, P.seed, it,
                              (int32_t*)newc.ptr, (int32_t*)oldc.ptr, s),
@@ -328,8 +331,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
       merge_all();
     }
     block = std::min<int64_t>(block, n) * 2;  // float.rs:872-873
-    uint32_t c = 0;
-    ck(hipMemcpyAsync(&c, d_changed, 4, hipMemcpyDeviceToHost, s), "counters");
+    ck(hipMemcpyAsync(h_changed.data(), d_changed, h_changed.size() * 4, hipMemcpyDeviceToHost, s), "counters");
     if (P.use_delta_improv) {
       const double zero = 0.0;
       ck(upload_sync(sumd.ptr, &zero, 8, s), "distance sum");
@@ -338,6 +340,8 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
     double sum = 0.0;
     if (P.use_delta_improv) ck(hipMemcpyAsync(&sum, sumd.ptr, 8, hipMemcpyDeviceToHost, s), "distance sum");
     ck(hipStreamSynchronize(s), "iteration");
+    uint32_t c = 0;
+    for (int i = 0; i < NND_COUNTER_SLOTS; i++) c += h_changed[(size_t)i * NND_SLOT_U32];
     if (st) st->changed.push_back(c);
     // float.rs:847: c <= (delta * k * n) as u32
     const float lim = P.delta * (float)P.k * (float)n;
@@ -357,10 +361,11 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
   if (st) st->iterations = it;
   ck(nnd_launch_export((const uint64_t*)keys.ptr, n, Kg, out_keys, s), "export");
   uint32_t dropped = 0;
-  unsigned long long jbytes = 0;
   ck(hipMemcpyAsync(&dropped, d_dropped, 4, hipMemcpyDeviceToHost, s), "counters");
-  ck(hipMemcpyAsync(&jbytes, d_jbytes, 8, hipMemcpyDeviceToHost, s), "counters");
+  ck(hipMemcpyAsync(h_jbytes.data(), d_jbytes, h_jbytes.size() * 8, hipMemcpyDeviceToHost, s), "counters");
   ck(hipStreamSynchronize(s), "NN-descent");
+  unsigned long long jbytes = 0;
+  for (int i = 0; i < NND_COUNTER_SLOTS; i++) jbytes += h_jbytes[(size_t)i * NND_SLOT_U64];
   jt.collect();
   if (st) {
     st->dropped = count_dropped ? (int64_t)dropped : -1;

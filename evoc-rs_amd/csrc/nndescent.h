@@ -12,6 +12,16 @@ namespace evoc {
 // the arrival order, and repeated offers of one pair land on the same slots.
 constexpr int NND_OFFER_CAP = 448;
 
+// The iteration's inserted-entry count and the joins' gathered bytes are
+// summed in NND_COUNTER_SLOTS slots one 128-byte line apart (slot chosen by
+// vertex or item): one address taking an atomic from every wave of a launch
+// serialises them (about 1 ms per 100k waves); the host adds the slots up.
+constexpr int NND_COUNTER_SLOTS = 64;
+constexpr int NND_SLOT_U32 = 32;  // uint32 stride of the inserted-entry slots
+constexpr int NND_SLOT_U64 = 16;  // uint64 stride of the byte slots
+// counters buffer: inserted-entry slots, byte slots, lost-offer count
+constexpr size_t NND_COUNTER_BYTES = (size_t)NND_COUNTER_SLOTS * 128 * 2 + 128;
+
 // Trees of the random-projection forest split together (one set of launches
 // per level for all of them); a larger forest is built in groups of this many.
 constexpr int NND_FOREST_TREES = 8;
@@ -39,7 +49,8 @@ struct JoinArgs {
   uint32_t salt;            // varies the slot hash between launches
   uint32_t* dropped;        // offers that met a better offer on both slots when they arrived
                             // (null: not counted, and the offers' atomics return nothing)
-  unsigned long long* bytes;  // algorithmic bytes gathered: each item's candidate rows once (may be null)
+  unsigned long long* bytes;  // algorithmic bytes gathered: each item's candidate rows once, summed in
+                              // NND_COUNTER_SLOTS slots NND_SLOT_U64 apart (may be null)
 };
 
 // The random-projection forest of n_trees trees, built level by level for all

@@ -147,7 +147,8 @@ __global__ __launch_bounds__(256, 2) void nnd_join_mfma_kernel(JoinArgs a) {
   if (nN == 0 || nC < 2) return;
   // the item's algorithmic traffic: its nC candidate rows (the new rows are
   // the first nN of them), each read once
-  if (a.bytes && lane == 0) atomicAdd(a.bytes, (unsigned long long)nC * (unsigned long long)a.row_bytes);
+  if (a.bytes && lane == 0)
+    atomicAdd(a.bytes + NND_SLOT_U64 * (item % NND_COUNTER_SLOTS), (unsigned long long)nC * (unsigned long long)a.row_bytes);
   const int nbm = (nN + 31) >> 5, nbn = (nC + 31) >> 5;
   // Fragment row pointers of C only: C starts with N (cidx[r] = nidx[r] for
   // r < nN), so the A operand's block bm is C's block bm.  Rows of that block
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(256) void nnd_join_jaccard_kernel(JoinArgs a) {
   __threadfence_block();
   __builtin_amdgcn_wave_barrier();
   if (a.bytes && lane == 0 && nN > 0 && nC >= 2)
-    atomicAdd(a.bytes, (unsigned long long)nC * (unsigned long long)a.row_bytes);
+    atomicAdd(a.bytes + NND_SLOT_U64 * (item % NND_COUNTER_SLOTS), (unsigned long long)nC * (unsigned long long)a.row_bytes);
   const int words = (int)(a.row_bytes / 4);
   for (int pi = lane; pi < nN * nC; pi += 64) {
     const int r = pi / nC, c = pi % nC;
@@ -357,7 +358,8 @@ __global__ __launch_bounds__(256) void nnd_merge_kernel(uint64_t* __restrict__ k
   else merge_sorted<1 + SLOTS>(st, cnt, keys, thr, v, Kg, inserted);
   inserted = wave_sum_i32(inserted);
   if (lane == 0) {
-    atomicAdd(changed, (uint32_t)inserted);
+    // (slots NND_SLOT_U32 apart, see NND_COUNTER_SLOTS)
+    if (inserted) atomicAdd(changed + NND_SLOT_U32 * (vi % NND_COUNTER_SLOTS), (uint32_t)inserted);
     umark[v] = 0;
   }
 }
