@@ -533,12 +533,14 @@ def run_c2_exact(args, torch, E, n=100000):
     return res
 
 
-def run_nndescent(args, torch, E):
+def run_nndescent(args, torch, E, n=None, cpu=True):
     """C2 with the reference's algorithm: device NN-descent, 8 iterations,
-    delta = delta_improv = 0, FP32 and BF16 + FP32 re-score, recall vs FP64."""
+    delta = delta_improv = 0, FP32 and BF16 + FP32 re-score, recall vs FP64
+    (n rows, default --n; cpu: also time the CPU baseline on them)."""
     from evoc_mi355x import synth
 
-    n, d, k = args.n, args.d, args.k
+    n = args.n if n is None else n
+    d, k = args.d, args.k
     x = synth.blobs_f32_device(n, d, 50, 0.25, seed=42, zipf=1.1)
     truth = fp64_truth(torch, x, k)
     ctx = E.Context(0)
@@ -586,7 +588,7 @@ def run_nndescent(args, torch, E):
                        "unit": "GB/s", "frac": (best["join_gb_per_s"] or 0.0) / 8000.0, "traffic": traffic,
                        "traffic_source": tsrc, "algorithmic_bytes_per_launch": per_launch,
                        "avg_launch_ms": best["join_ms_per_call"] / max(best["join_launches_per_call"], 1.0)}
-    if not args.no_cpu_baseline:
+    if cpu and not args.no_cpu_baseline:
         keep = {}
         res["cpu_baseline"] = cpu_baseline(x.cpu().numpy(), k, args.cpu_seconds, keep=keep)
         ref_idx = keep.get("indices")
@@ -667,6 +669,11 @@ def main():
     if not args.no_extra:
         log("[bench] 100k x 768 line")
         result["c2_100k"] = run_c2_exact(args, torch, E, 100000)
+        log("[bench] 100k x 768 device NN-descent line")
+        nd = run_nndescent(args, torch, E, n=100000, cpu=False)
+        result["c2_nndescent"] = {"workload": nd["config"]["workload"], "points_per_s": nd["value"],
+                                  "ms_per_step": nd["ms_per_step"], "recall_at_k": nd["recall_at_k"],
+                                  "roofline": nd["roofline"], "bf16": nd["bf16"], "fp32": nd["fp32"]}
     if args.serving_requests > 0:
         log(f"[bench] serving: {args.serving_requests} requests at {args.serving_rate}/s")
         result["serving"] = serving_bench(torch, E, args.serving_requests, args.serving_rate,

@@ -402,7 +402,7 @@ __device__ __forceinline__ uint64_t cand_key(uint32_t prio, int32_t c) {
 }
 __device__ __forceinline__ int32_t cand_of(uint64_t k) { return (int32_t)(0xffffffffu - (uint32_t)k); }
 
-// keep the mc best of (best ∪ chunk): best holds one key per lane (lane < mc).
+// keep the mc best distinct keys of (best ∪ chunk), one key per lane each.
 // The kept keys (non-empty, first of a run of equal keys) are compacted in
 // sorted order through the wave's 64-key LDS stash: kept key i goes to slot i,
 // and lane L then reads slot L.
@@ -451,11 +451,14 @@ __global__ __launch_bounds__(256) void nnd_build_candidates_kernel(
   uint64_t best[2];
 #pragma unroll
   for (int cls = 0; cls < 2; cls++) {  // cls 1 = new, 0 = old
-    uint64_t b = EVOC_KEY_NONE;
-    uint64_t ck = (fc >= 0 && fc != v && fflag == cls) ? cand_key(edge_priority(v, fc, seed, it), fc)
-                                                       : EVOC_KEY_NONE;
-    b = keep_best(b, ck, mc, stash);
-    for (int base = r0; base < r1; base += 64) {
+    // the forward entries (distinct neighbours, one per lane) go straight into
+    // the first selection with the first chunk of reverse edges: the mc best
+    // of (the mc best of F) ∪ C are the mc best of F ∪ C, so no selection of
+    // F alone is needed; one selection runs even when there is no reverse edge
+    uint64_t b = (fc >= 0 && fc != v && fflag == cls) ? cand_key(edge_priority(v, fc, seed, it), fc)
+                                                      : EVOC_KEY_NONE;
+    int base = r0;
+    do {
       const int i = base + lane;
       uint64_t rk = EVOC_KEY_NONE;
       if (i < r1) {
@@ -464,7 +467,8 @@ __global__ __launch_bounds__(256) void nnd_build_candidates_kernel(
         if (f == cls) rk = cand_key(edge_priority(v, c, seed, it), c);
       }
       b = keep_best(b, rk, mc, stash);
-    }
+      base += 64;
+    } while (base < r1);
     best[cls] = b;
   }
   if (lane < mc) {
@@ -624,6 +628,55 @@ __global__ __launch_bounds__(256) void rp_side_kernel(const int32_t* __restrict_
     m = wave_sum_f32(m);
   }
   if (lane == 0) {
+    int8_t sd;
+    if (fabsf(m) < 1e-8f) {
+      const int t = (int)(pos / f.n);
+      sd = (int8_t)(mix64(f.seeds[t] ^ ((uint64_t)level << 40) ^ ((uint64_t)(uint32_t)p << 1) ^ 0x5bd1e995ull) &
+                    1ull);
+    } else {
+      sd = m > 0.0f ? 0 : 1;
+    }
+    side[pos] = sd;
+  }
+}
+
+// BF16 rows: 16 lanes per position, four positions per wave; each lane takes 16
+// bytes (8 elements) of the row and the matching 32 bytes of the hyperplane
+// per step (entries at or past dim are masked: the rows are zero padded, the
+// hyperplane's padding is not written).  A quarter of the waves of
+// rp_side_kernel, and 16-byte loads instead of 4-byte ones.
+__global__ __launch_bounds__(256) void rp_side_bf16_kernel(const int32_t* __restrict__ perm,
+                                                           const int32_t* __restrict__ seg_of, int64_t n_pos,
+                                                           ForestArgs f, int level, int8_t* __restrict__ side) {
+  const int sub = threadIdx.x & 15;
+  const int64_t pos = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int s = pos < n_pos ? seg_of[pos] : -1;
+  int p = 0;
+  float m = 0.0f;
+  if (s >= 0) {
+    p = perm[pos];
+    const uint4* x = (const uint4*)(f.rows + (int64_t)p * f.row_bytes);
+    const float* hv = f.hyper + (int64_t)s * f.hyper_ld;
+    for (int i = sub; 8 * i < f.dim; i += 16) {
+      const int e0 = 8 * i;
+      const uint4 u = x[i];
+      // e0 and hyper_ld are multiples of 4, so e0 < dim keeps e0 + 3 (and
+      // e0 + 4 < dim keeps e0 + 7) inside this segment's hyperplane
+      const float4 h0 = *(const float4*)(hv + e0);
+      const float4 h1 = e0 + 4 < f.dim ? *(const float4*)(hv + e0 + 4) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const float hh[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+      const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const float xe = __uint_as_float((q & 1) ? (w[q >> 1] & 0xffff0000u) : (w[q >> 1] << 16));
+        if (e0 + q < f.dim) m = fmaf(hh[q], xe, m);
+      }
+    }
+  }
+  // sum over the position's 16 lanes (every lane of the wave takes part)
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) m += __shfl_xor(m, o, 16);
+  if (sub == 0 && s >= 0) {
     int8_t sd;
     if (fabsf(m) < 1e-8f) {
       const int t = (int)(pos / f.n);
@@ -871,8 +924,12 @@ hipError_t rp_launch_level(const int4* segs, int nseg, const int32_t* perm, int3
   hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, seg_of, total, -1);
   hipLaunchKernelGGL(rp_seg_fill_kernel, dim3(nseg), dim3(256), 0, s, segs, nseg, seg_of);
   hipLaunchKernelGGL(rp_hyperplane_kernel, dim3(blocks_for(nseg, 4)), dim3(256), 0, s, segs, nseg, perm, f, level);
-  hipLaunchKernelGGL(rp_side_kernel, dim3(blocks_for(total, 4)), dim3(256), 0, s, perm, seg_of, segs, total, f,
-                     level, side);
+  if (f.kind == EVOC_ELEM_BF16)
+    hipLaunchKernelGGL(rp_side_bf16_kernel, dim3(blocks_for(total, 16)), dim3(256), 0, s, perm, seg_of, total, f,
+                       level, side);
+  else
+    hipLaunchKernelGGL(rp_side_kernel, dim3(blocks_for(total, 4)), dim3(256), 0, s, perm, seg_of, segs, total, f,
+                       level, side);
   hipLaunchKernelGGL(rp_copy_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s, perm, perm2, seg_of, total);
   hipLaunchKernelGGL(rp_partition_kernel, dim3(nseg), dim3(256), 0, s, segs, nseg, perm, perm2, side, f, level,
                      next_segs, n_next, leaves, n_leaves, last_level);
