@@ -166,45 +166,91 @@ __global__ __launch_bounds__(256, 2) void nnd_join_mfma_kernel(JoinArgs a) {
   for (int bm = 0; bm < 2; bm++)
 #pragma unroll
     for (int bn = 0; bn < 4; bn++) acc[bm][bn] = M::zero();
-  // Fragments straight from HBM through a ring of D register sets: the loads
-  // of the D - 1 steps after the current one are in flight while its MFMAs
-  // run.  Every load is unconditional (the blocks past nbn read row cidx[0],
+  // Fragments straight from HBM through a ring of D = 4 register sets: the
+  // loads of the D - 1 steps after the current one are in flight while its
+  // MFMAs run.  Every load is unconditional (the blocks past nbn read row cidx[0],
   // the steps past the end re-read the last step into sets nothing reads), so
   // the compiler's vmcnt waits leave the D - 1 newer steps in flight; a load
   // under a branch would make it drain them all at every step.  Only the
   // nbm x nbn blocks that hold pairs are multiplied (wave-uniform tests; the
   // blocks past them are never read below).  The step count is a multiple of
   // D (rows are padded to 128 bytes), so the sets rotate with static indices.
-  constexpr int D = KIND == EVOC_ELEM_F32 ? 8 : 4;
-  static_assert(128 % (D * M::KSTEP_BYTES) == 0, "128-byte rows hold whole rounds of the ring");
-  const int ksteps = (int)(a.row_bytes / M::KSTEP_BYTES);
-  typename M::frag_t fb[D][4];
-  auto load_step = [&](typename M::frag_t (&rb)[4], int ks) __attribute__((always_inline)) {
-    const int off = min(ks, ksteps - 1) * M::KSTEP_BYTES;
+  if constexpr (KIND == EVOC_ELEM_F32) {
+    // FP32 rows: 16 bytes per lane cover four K steps of 32x32x2 (32 bytes of
+    // a row: lane half h holds elements 8S + 4h .. 8S + 4h + 3 of super-step
+    // S), so a gather is one 16-byte load per lane instead of four 4-byte ones.
+    // K step j of S pairs elements 8S + j and 8S + 4 + j, the same in both
+    // operands: every product of the row pair is summed once (in another
+    // order than element pairs 2i, 2i + 1).  Ring of 4 super-steps, loads as
+    // above (unconditional, clamped, in step order).
+    constexpr int DS = 4;
+    static_assert(128 % (DS * 32) == 0, "128-byte rows hold whole rounds of the ring");
+    const int nsup = (int)(a.row_bytes / 32);
+    const uint8_t* crow[4];
 #pragma unroll
-    for (int bn = 0; bn < 4; bn++) rb[bn] = M::load(brow[bn] + off);
-  };
-  auto mma_step = [&](const typename M::frag_t (&rb)[4]) __attribute__((always_inline)) {
+    for (int bn = 0; bn < 4; bn++) crow[bn] = brow[bn] - (lane >> 5) * M::LANE_BYTES + (lane >> 5) * 16;
+    float4 fq[DS][4];
+    auto load_sup = [&](float4 (&rb)[4], int S) __attribute__((always_inline)) {
+      const int off = min(S, nsup - 1) * 32;
 #pragma unroll
-    for (int bm = 0; bm < 2; bm++)
+      for (int bn = 0; bn < 4; bn++) rb[bn] = *(const float4*)(crow[bn] + off);
+    };
+    auto comp = [](const float4& v, int j) __attribute__((always_inline)) {
+      return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w;
+    };
+    auto mma_sup = [&](const float4 (&rb)[4]) __attribute__((always_inline)) {
 #pragma unroll
-      for (int bn = 0; bn < 4; bn++)
-        if (bm < nbm && bn < nbn) M::mma(acc[bm][bn], rb[bm], rb[bn]);
-  };
-  // (the scheduling barriers keep the steps' loads in step order, in the
-  // first D - 1 steps as in the loop: the vmcnt each step waits for is then
-  // the same on entry as around the loop)
-  static_for<D - 1>([&](auto sc) __attribute__((always_inline)) {
-    load_step(fb[decltype(sc)::value], decltype(sc)::value);
-    __builtin_amdgcn_sched_barrier(0);
-  });
-  for (int ks = 0; ks < ksteps; ks += D) {
-    static_for<D>([&](auto ic) __attribute__((always_inline)) {
-      constexpr int i = decltype(ic)::value;
-      load_step(fb[(i + D - 1) % D], ks + i + D - 1);
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int bm = 0; bm < 2; bm++)
+#pragma unroll
+          for (int bn = 0; bn < 4; bn++)
+            if (bm < nbm && bn < nbn) M::mma(acc[bm][bn], comp(rb[bm], j), comp(rb[bn], j));
+    };
+    static_for<DS - 1>([&](auto sc) __attribute__((always_inline)) {
+      load_sup(fq[decltype(sc)::value], decltype(sc)::value);
       __builtin_amdgcn_sched_barrier(0);
-      mma_step(fb[i]);
     });
+    for (int S = 0; S < nsup; S += DS) {
+      static_for<DS>([&](auto ic) __attribute__((always_inline)) {
+        constexpr int i = decltype(ic)::value;
+        load_sup(fq[(i + DS - 1) % DS], S + i + DS - 1);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_sup(fq[i]);
+      });
+    }
+  } else {
+    constexpr int D = 4;
+    static_assert(128 % (D * M::KSTEP_BYTES) == 0, "128-byte rows hold whole rounds of the ring");
+    const int ksteps = (int)(a.row_bytes / M::KSTEP_BYTES);
+    typename M::frag_t fb[D][4];
+    auto load_step = [&](typename M::frag_t (&rb)[4], int ks) __attribute__((always_inline)) {
+      const int off = min(ks, ksteps - 1) * M::KSTEP_BYTES;
+#pragma unroll
+      for (int bn = 0; bn < 4; bn++) rb[bn] = M::load(brow[bn] + off);
+    };
+    auto mma_step = [&](const typename M::frag_t (&rb)[4]) __attribute__((always_inline)) {
+#pragma unroll
+      for (int bm = 0; bm < 2; bm++)
+#pragma unroll
+        for (int bn = 0; bn < 4; bn++)
+          if (bm < nbm && bn < nbn) M::mma(acc[bm][bn], rb[bm], rb[bn]);
+    };
+    // (the scheduling barriers keep the steps' loads in step order, in the
+    // first D - 1 steps as in the loop: the vmcnt each step waits for is then
+    // the same on entry as around the loop)
+    static_for<D - 1>([&](auto sc) __attribute__((always_inline)) {
+      load_step(fb[decltype(sc)::value], decltype(sc)::value);
+      __builtin_amdgcn_sched_barrier(0);
+    });
+    for (int ks = 0; ks < ksteps; ks += D) {
+      static_for<D>([&](auto ic) __attribute__((always_inline)) {
+        constexpr int i = decltype(ic)::value;
+        load_step(fb[(i + D - 1) % D], ks + i + D - 1);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_step(fb[i]);
+      });
+    }
   }
   // Offers, one 32 x 32 block at a time (rolled: the block's values are
   // copied out under a uniform branch, so no accumulator is indexed at run time)
