@@ -686,14 +686,16 @@ __global__ __launch_bounds__(256) void rp_side_kernel(const int32_t* __restrict_
   }
 }
 
-// BF16 rows: 16 lanes per position, four positions per wave; each lane takes 16
-// bytes (8 elements) of the row and the matching 32 bytes of the hyperplane
-// per step (entries at or past dim are masked: the rows are zero padded, the
-// hyperplane's padding is not written).  A quarter of the waves of
-// rp_side_kernel, and 16-byte loads instead of 4-byte ones.
-__global__ __launch_bounds__(256) void rp_side_bf16_kernel(const int32_t* __restrict__ perm,
+// BF16 and FP32 rows: 16 lanes per position, four positions per wave; each
+// lane takes 16 bytes of the row per step (8 BF16 or 4 FP32 elements) and the
+// matching hyperplane floats (entries at or past dim are masked: the rows are
+// zero padded, the hyperplane's padding is not written).  A quarter of the
+// waves of rp_side_kernel, and 16-byte loads instead of 4-byte ones.
+template <int KIND>
+__global__ __launch_bounds__(256) void rp_side_wide_kernel(const int32_t* __restrict__ perm,
                                                            const int32_t* __restrict__ seg_of, int64_t n_pos,
                                                            ForestArgs f, int level, int8_t* __restrict__ side) {
+  constexpr int EPS = KIND == EVOC_ELEM_BF16 ? 8 : 4;  // elements per 16-byte step
   const int sub = threadIdx.x & 15;
   const int64_t pos = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
   const int s = pos < n_pos ? seg_of[pos] : -1;
@@ -701,21 +703,30 @@ __global__ __launch_bounds__(256) void rp_side_bf16_kernel(const int32_t* __rest
   float m = 0.0f;
   if (s >= 0) {
     p = perm[pos];
-    const uint4* x = (const uint4*)(f.rows + (int64_t)p * f.row_bytes);
+    const uint4* x = KIND == EVOC_ELEM_BF16 ? (const uint4*)(f.rows + (int64_t)p * f.row_bytes)
+                                            : (const uint4*)(f.frows + (int64_t)p * f.frow_ld);
     const float* hv = f.hyper + (int64_t)s * f.hyper_ld;
-    for (int i = sub; 8 * i < f.dim; i += 16) {
-      const int e0 = 8 * i;
+    for (int i = sub; EPS * i < f.dim; i += 16) {
+      const int e0 = EPS * i;
       const uint4 u = x[i];
       // e0 and hyper_ld are multiples of 4, so e0 < dim keeps e0 + 3 (and
       // e0 + 4 < dim keeps e0 + 7) inside this segment's hyperplane
       const float4 h0 = *(const float4*)(hv + e0);
-      const float4 h1 = e0 + 4 < f.dim ? *(const float4*)(hv + e0 + 4) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      const float hh[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
-      const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+      if constexpr (KIND == EVOC_ELEM_BF16) {
+        const float4 h1 = e0 + 4 < f.dim ? *(const float4*)(hv + e0 + 4) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float hh[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const float xe = __uint_as_float((q & 1) ? (w[q >> 1] & 0xffff0000u) : (w[q >> 1] << 16));
-        if (e0 + q < f.dim) m = fmaf(hh[q], xe, m);
+        for (int q = 0; q < 8; q++) {
+          const float xe = __uint_as_float((q & 1) ? (w[q >> 1] & 0xffff0000u) : (w[q >> 1] << 16));
+          if (e0 + q < f.dim) m = fmaf(hh[q], xe, m);
+        }
+      } else {
+        const float hh[4] = {h0.x, h0.y, h0.z, h0.w};
+        const float xx[4] = {__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z), __uint_as_float(u.w)};
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (e0 + q < f.dim) m = fmaf(hh[q], xx[q], m);
       }
     }
   }
@@ -971,8 +982,11 @@ hipError_t rp_launch_level(const int4* segs, int nseg, const int32_t* perm, int3
   hipLaunchKernelGGL(rp_seg_fill_kernel, dim3(nseg), dim3(256), 0, s, segs, nseg, seg_of);
   hipLaunchKernelGGL(rp_hyperplane_kernel, dim3(blocks_for(nseg, 4)), dim3(256), 0, s, segs, nseg, perm, f, level);
   if (f.kind == EVOC_ELEM_BF16)
-    hipLaunchKernelGGL(rp_side_bf16_kernel, dim3(blocks_for(total, 16)), dim3(256), 0, s, perm, seg_of, total, f,
-                       level, side);
+    hipLaunchKernelGGL(rp_side_wide_kernel<EVOC_ELEM_BF16>, dim3(blocks_for(total, 16)), dim3(256), 0, s, perm,
+                       seg_of, total, f, level, side);
+  else if (f.kind == EVOC_ELEM_F32)
+    hipLaunchKernelGGL(rp_side_wide_kernel<EVOC_ELEM_F32>, dim3(blocks_for(total, 16)), dim3(256), 0, s, perm,
+                       seg_of, total, f, level, side);
   else
     hipLaunchKernelGGL(rp_side_kernel, dim3(blocks_for(total, 4)), dim3(256), 0, s, perm, seg_of, segs, total, f,
                        level, side);
