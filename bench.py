@@ -34,7 +34,7 @@ MEASURED_TRAFFIC = {
     (1000000, 768, "pass2_sym_kernel<bf16>"): (3.45e12, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated "
                                                         "(profiles/pass2_sym_bf16_1m_768_traffic.md)"),
     # the NN-descent local join: the mean over a call's 23 launches
-    (100000, 768, "nnd_join_mfma_kernel<bf16>"): (2.021e9, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated, "
+    (100000, 768, "nnd_join_mfma_kernel<bf16>"): (2.022e9, "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated, "
                                                            "mean per launch (profiles/nndescent_100k_768_k30_kernel_trace.md)"),
 }
 
