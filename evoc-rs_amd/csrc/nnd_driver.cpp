@@ -225,8 +225,10 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
   uint32_t* d_changed = (uint32_t*)counters.ptr;
   unsigned long long* d_jbytes = (unsigned long long*)((uint8_t*)counters.ptr + NND_COUNTER_SLOTS * 128);
   uint32_t* d_dropped = (uint32_t*)((uint8_t*)counters.ptr + NND_COUNTER_SLOTS * 128 * 2);
-  std::vector<uint32_t> h_changed((size_t)NND_COUNTER_SLOTS * NND_SLOT_U32);
-  std::vector<unsigned long long> h_jbytes((size_t)NND_COUNTER_SLOTS * NND_SLOT_U64);
+  // host copies of the slots (on the stack: the allocation-free entry point
+  // runs this search too)
+  uint32_t h_changed[NND_COUNTER_SLOTS * NND_SLOT_U32];
+  unsigned long long h_jbytes[NND_COUNTER_SLOTS * NND_SLOT_U64];
   Check ck{ctx};
   JoinTimer jt{ctx};
   auto join = [&](const JoinArgs& args, const char* what) {
@@ -331,7 +333,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
       merge_all();
     }
     block = std::min<int64_t>(block, n) * 2;  // float.rs:872-873
-    ck(hipMemcpyAsync(h_changed.data(), d_changed, h_changed.size() * 4, hipMemcpyDeviceToHost, s), "counters");
+    ck(hipMemcpyAsync(h_changed, d_changed, sizeof(h_changed), hipMemcpyDeviceToHost, s), "counters");
     if (P.use_delta_improv) {
       const double zero = 0.0;
       ck(upload_sync(sumd.ptr, &zero, 8, s), "distance sum");
@@ -362,7 +364,7 @@ int nndescent_search(Ctx* ctx, int kind, const uint8_t* rows, int64_t row_bytes,
   ck(nnd_launch_export((const uint64_t*)keys.ptr, n, Kg, out_keys, s), "export");
   uint32_t dropped = 0;
   ck(hipMemcpyAsync(&dropped, d_dropped, 4, hipMemcpyDeviceToHost, s), "counters");
-  ck(hipMemcpyAsync(h_jbytes.data(), d_jbytes, h_jbytes.size() * 8, hipMemcpyDeviceToHost, s), "counters");
+  ck(hipMemcpyAsync(h_jbytes, d_jbytes, sizeof(h_jbytes), hipMemcpyDeviceToHost, s), "counters");
   ck(hipStreamSynchronize(s), "NN-descent");
   unsigned long long jbytes = 0;
   for (int i = 0; i < NND_COUNTER_SLOTS; i++) jbytes += h_jbytes[(size_t)i * NND_SLOT_U64];

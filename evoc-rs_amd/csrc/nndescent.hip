@@ -148,7 +148,8 @@ __global__ __launch_bounds__(256, 2) void nnd_join_mfma_kernel(JoinArgs a) {
   // the item's algorithmic traffic: its nC candidate rows (the new rows are
   // the first nN of them), each read once
   if (a.bytes && lane == 0)
-    atomicAdd(a.bytes + NND_SLOT_U64 * (item % NND_COUNTER_SLOTS), (unsigned long long)nC * (unsigned long long)a.row_bytes);
+    atomicAdd(a.bytes + NND_SLOT_U64 * (item % NND_COUNTER_SLOTS),
+              (unsigned long long)nC * (unsigned long long)a.row_bytes);
   const int nbm = (nN + 31) >> 5, nbn = (nC + 31) >> 5;
   // Fragment row pointers of C only: C starts with N (cidx[r] = nidx[r] for
   // r < nN), so the A operand's block bm is C's block bm.  Rows of that block
@@ -168,8 +169,9 @@ __global__ __launch_bounds__(256, 2) void nnd_join_mfma_kernel(JoinArgs a) {
     for (int bn = 0; bn < 4; bn++) acc[bm][bn] = M::zero();
   // Fragments straight from HBM through a ring of D = 4 register sets: the
   // loads of the D - 1 steps after the current one are in flight while its
-  // MFMAs run.  Every load is unconditional (the blocks past nbn read row cidx[0],
-  // the steps past the end re-read the last step into sets nothing reads), so
+  // MFMAs run.  Every load is unconditional (the blocks past nbn read row
+  // cidx[0], the steps past the end re-read the last step into sets nothing
+  // reads), so
   // the compiler's vmcnt waits leave the D - 1 newer steps in flight; a load
   // under a branch would make it drain them all at every step.  Only the
   // nbm x nbn blocks that hold pairs are multiplied (wave-uniform tests; the
@@ -293,7 +295,8 @@ __global__ __launch_bounds__(256) void nnd_join_jaccard_kernel(JoinArgs a) {
   __threadfence_block();
   __builtin_amdgcn_wave_barrier();
   if (a.bytes && lane == 0 && nN > 0 && nC >= 2)
-    atomicAdd(a.bytes + NND_SLOT_U64 * (item % NND_COUNTER_SLOTS), (unsigned long long)nC * (unsigned long long)a.row_bytes);
+    atomicAdd(a.bytes + NND_SLOT_U64 * (item % NND_COUNTER_SLOTS),
+              (unsigned long long)nC * (unsigned long long)a.row_bytes);
   const int words = (int)(a.row_bytes / 4);
   for (int pi = lane; pi < nN * nC; pi += 64) {
     const int r = pi / nC, c = pi % nC;
