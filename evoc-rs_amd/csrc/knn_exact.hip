@@ -1001,6 +1001,21 @@ __device__ __forceinline__ bool jac_may_reach(uint32_t inter, uint32_t uni, floa
   return !((float)inter < tau * (float)uni * (1.0f - 0x1p-20f));
 }
 
+// An integer count below which no pair with one row of `pop` bits reaches the
+// bound tau, for a test of one compare per pair ahead of jac_may_reach: the
+// union of that row with any other is at least pop, so for tau > 0 the
+// rounded products keep t = fl(fl(tau pop) (1 - 2^-20)) at or below
+// jac_may_reach's fl(fl(tau uni) (1 - 2^-20)), and inter < ceil(t) gives
+// inter < t: jac_may_reach is false.  Bounds <= 0 or NaN (and pop 0 with an
+// infinite bound) give 0, which rules out no pair; t beyond the int range
+// (tau = +inf) rules out every count.
+__device__ __forceinline__ int jac_quick_bound(float tau, uint32_t pop) {
+  const float t = tau * (float)pop * (1.0f - 0x1p-20f);
+  if (!(t > 0.0f)) return 0;
+  if (t >= 2147483520.0f) return INT_MAX;
+  return (int)ceilf(t);
+}
+
 __global__ __launch_bounds__(256) void jac_pass1_kernel(TileArgs a) {
   extern __shared__ __align__(16) uint8_t lds[];
   uint32_t inter[8][8];
@@ -1130,6 +1145,10 @@ __global__ __launch_bounds__(256) void jac_pass2_sym_kernel(TileArgs a) {
       ckey_l[threadIdx.x] = (c < a.n_c && key_bound) ? a.tau_key[c] : 0ull;
     }
     jac_tile(inter, lds, a, c0, q0);  // (its barriers publish the bounds)
+    // (per tile, so that they are not held in registers through the popcounts)
+    int tq[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) tq[j] = jac_quick_bound(tau[j], pq[j]);
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int cl = ty + 16 * i;
@@ -1138,14 +1157,18 @@ __global__ __launch_bounds__(256) void jac_pass2_sym_kernel(TileArgs a) {
       const uint32_t pc = a.row_pop[c];
       const float ctau = diag ? INFINITY : ctau_l[cl];
       const uint64_t ckey = diag ? 0ull : ckey_l[cl];
+      const int tc = jac_quick_bound(ctau, pc);
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const int ql = tx + 16 * j;
         const int q = q0 + ql;
         if (q >= a.n_q) continue;
         const bool self = (c == q);
+        // most pairs reach neither row's bound: one integer compare per side
+        // rules most of them out, jac_may_reach most of the rest, and only
+        // what is left takes the division
+        if (!self && (int)inter[i][j] < tq[j] && (int)inter[i][j] < tc) continue;
         const uint32_t uni = pc + pq[j] - inter[i][j];
-        // most pairs reach neither row's bound: they skip the division
         if (!self && !jac_may_reach(inter[i][j], uni, tau[j]) && !jac_may_reach(inter[i][j], uni, ctau)) continue;
         const float s = self ? 1.0f : jac_score(inter[i][j], pc, pq[j]);
         bool take = s >= tau[j] && !(self && a.skip_self);
