@@ -1074,6 +1074,10 @@ __global__ __launch_bounds__(256) void jac_pass2_kernel(TileArgs a) {
   for (int t = t_begin; t < t_end; t++) {
     const int c0 = t * TILE;
     jac_tile(inter, lds, a, c0, q0);
+    // (per tile, so that they are not held in registers through the popcounts)
+    int tq[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) tq[j] = jac_quick_bound(tau[j], pq[j]);
 #pragma unroll
     for (int i = 0; i < 8; i++) {
       const int c = c0 + ty + 16 * i;
@@ -1085,6 +1089,9 @@ __global__ __launch_bounds__(256) void jac_pass2_kernel(TileArgs a) {
         const int q = q0 + ql;
         if (q >= a.n_q) continue;
         const bool self = (c == q + a.q_offset);
+        // one integer compare rules most pairs out (jac_quick_bound), then
+        // jac_may_reach most of the rest, before the division
+        if (!self && (int)inter[i][j] < tq[j]) continue;
         if (!self && !jac_may_reach(inter[i][j], pc + pq[j] - inter[i][j], tau[j])) continue;
         const float s = self ? 1.0f : jac_score(inter[i][j], pc, pq[j]);
         bool take = s >= tau[j] && !(self && a.skip_self);
