@@ -1028,22 +1028,43 @@ __global__ __launch_bounds__(256) void jac_pass1_kernel(TileArgs a) {
   for (int t = t_begin; t < t_end; t++) {
     const int c0 = t * TILE;
     jac_tile(inter, lds, a, c0, q0);
+    // the popcounts of the thread's 8 candidate rows, once per tile
+    uint32_t pc[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int c = c0 + ty + 16 * i;
+      pc[i] = c < a.n_c ? a.row_pop[(int64_t)c * a.c_stride] : 0u;
+    }
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       const int q = q0 + tx + 16 * j;
       if (q >= a.n_q) continue;
       const uint32_t pq = a.row_pop[q + a.q_offset];
-      float m = -INFINITY;
+      // The group maximum of the IEEE quotients inter / uni is the quotient of
+      // the largest fraction (a correctly rounded division is monotone in its
+      // exact value), so the fractions are compared by cross-multiplication
+      // and one division is left per group; uni = 0 scores 0, as jac_score.
+      uint32_t bn = 0, bd = 1;
+      bool any = false, self_in = false;
 #pragma unroll
       for (int i = 0; i < 8; i++) {
         const int c = c0 + ty + 16 * i;
-        const int64_t cg = (int64_t)c * a.c_stride;
-        const bool self = cg == (int64_t)q + a.q_offset;
+        const bool self = (int64_t)c * a.c_stride == (int64_t)q + a.q_offset;
         if (c < a.n_c && !(self && a.skip_self)) {
-          const float s = self ? 1.0f : jac_score(inter[i][j], a.row_pop[cg], pq);
-          m = fmaxf(m, s);
+          if (self) {
+            self_in = true;
+          } else {
+            const uint32_t uni = pc[i] + pq - inter[i][j];
+            const uint32_t num = uni > 0 ? inter[i][j] : 0u, den = uni > 0 ? uni : 1u;
+            if (!any || (uint64_t)num * bd > (uint64_t)bn * den) {
+              bn = num;
+              bd = den;
+            }
+            any = true;
+          }
         }
       }
+      const float m = self_in ? 1.0f : !any ? -INFINITY : bn == 0 ? 0.0f : __fdiv_rn((float)bn, (float)bd);
       // 16 groups (one per ty) of 8 candidates per tile
       a.gmax[(int64_t)q * a.ng + t * 16 + ty] = m;
     }
