@@ -498,6 +498,7 @@ __global__ __launch_bounds__(256) void nnd_build_candidates_kernel(
   const int fflag = (int)(fk & 1ull);
   const int r0 = revoff[v], r1 = revoff[v + 1];
   uint64_t best[2];
+  uint64_t fwd_new = EVOC_KEY_NONE;  // this lane's forward entry as a new-class candidate key
 #pragma unroll
   for (int cls = 0; cls < 2; cls++) {  // cls 1 = new, 0 = old
     // the forward entries (distinct neighbours, one per lane) go straight into
@@ -506,6 +507,7 @@ __global__ __launch_bounds__(256) void nnd_build_candidates_kernel(
     // F alone is needed; one selection runs even when there is no reverse edge
     uint64_t b = (fc >= 0 && fc != v && fflag == cls) ? cand_key(edge_priority(v, fc, seed, it), fc)
                                                       : EVOC_KEY_NONE;
+    if (cls == 1) fwd_new = b;
     int base = r0;
     do {
       const int i = base + lane;
@@ -524,12 +526,13 @@ __global__ __launch_bounds__(256) void nnd_build_candidates_kernel(
     newc[(int64_t)v * mc + lane] = best[1] != EVOC_KEY_NONE ? cand_of(best[1]) : -1;
     oldc[(int64_t)v * mc + lane] = best[0] != EVOC_KEY_NONE ? cand_of(best[0]) : -1;
   }
-  // clear the flag of forward entries sampled into the new list (heap.rs:319-340)
-  bool sampled = false;
-  for (int t = 0; t < mc; t++) {
-    const uint64_t bk = (uint64_t)__shfl((long long)best[1], t, 64);
-    if (bk != EVOC_KEY_NONE && cand_of(bk) == fc) sampled = true;
-  }
+  // clear the flag of forward entries sampled into the new list (heap.rs:319-340).
+  // The new list holds the mc largest distinct keys, descending in lanes
+  // 0..mc-1 (a candidate's key is the same from either direction), so a
+  // forward entry was sampled when its key is at least the list's last one
+  // (EVOC_KEY_NONE, below every key, while the list is not full).
+  const uint64_t last = (uint64_t)__shfl((long long)best[1], mc - 1, 64);
+  const bool sampled = fwd_new != EVOC_KEY_NONE && fwd_new >= last;
   if (lane < Kg && fc >= 0 && fflag && sampled) keys[(int64_t)v * Kg + lane] = fk & ~1ull;
 }
 
