@@ -754,7 +754,10 @@ static int knn_graph_run(evoc_mi355x_ctx* ctx, const evoc_knn_input* in, const e
     P.max_depth = o->max_rptree_depth;
     P.delta = o->delta;
     P.delta_improv = o->delta_improv;
-    P.use_delta_improv = true;
+    // the relative-improvement stop (float.rs:853-870) can only fire for a
+    // positive threshold (the relative change is >= 0, and a NaN threshold
+    // compares false), so its per-iteration distance sum is skipped otherwise
+    P.use_delta_improv = o->delta_improv > 0.0f;
     P.seed = ((uint64_t)tau_state[0] * 0x9e3779b97f4a7c15ull) ^ ((uint64_t)tau_state[1] << 21) ^
              ((uint64_t)tau_state[2] << 42);
     for (int t = 0; t < P.n_trees; t++) {
